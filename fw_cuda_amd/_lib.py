@@ -1,0 +1,92 @@
+"""ctypes binding of libfwx.so, the C ABI declared in include/fwx.h.
+
+The library is built in-tree (fw_cuda_amd/lib/libfwx.so by
+`make -C fw_cuda_amd`, or __graft_entry__.build()).  Loading fails loudly if
+it is missing: there is no fallback implementation.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libfwx.so")
+
+INF = 1_000_000_000      # include/constants.h:3
+MAX_N = 32769            # include/constants.h:5
+MAX_EDGE_WEIGHT = 25_000  # include/constants.h:7
+
+OK = 0
+ERR_INVALID_ARG = 1
+ERR_UNKNOWN_ALGO = 2
+ERR_HIP = 3
+ERR_OOM = 4
+ERR_UNSUPPORTED = 5
+
+
+class FwxError(RuntimeError):
+    """A libfwx call returned a non-zero status."""
+
+    def __init__(self, code, msg):
+        super().__init__(f"libfwx status {code}: {msg}")
+        self.code = code
+
+
+_i32 = ctypes.c_int32
+_i64 = ctypes.c_int64
+_vp = ctypes.c_void_p
+_i32p = ctypes.POINTER(ctypes.c_int32)
+
+# name -> (restype, argtypes)
+SIGNATURES = {
+    "fwx_last_error": (ctypes.c_char_p, []),
+    "fwx_version": (ctypes.c_int, []),
+    "fwx_gpu_algorithm_count": (ctypes.c_int, []),
+    "fwx_gpu_algorithm_name": (ctypes.c_char_p, [ctypes.c_int]),
+    "fwx_is_gpu_algorithm": (ctypes.c_int, [ctypes.c_char_p]),
+    "fwx_run": (ctypes.c_int, [ctypes.c_char_p, _vp, _i32, _i32, _i32, _vp]),
+    "fwx_baseline_gpu": (ctypes.c_int, [_vp, _i32, _vp]),
+    "fwx_tiling_gpu": (ctypes.c_int, [_vp, _i32, _i32, _vp]),
+    "fwx_tiling_gpu_refshape": (ctypes.c_int, [_vp, _i32, _vp]),
+    "fwx_multi_layer_tiling_gpu": (ctypes.c_int, [_vp, _i32, _i32, _i32, _vp]),
+    "fwx_tuned_config": (ctypes.c_int, [_i32, _i32p, _i32p]),
+    "fwx_device_alloc": (ctypes.c_int, [ctypes.POINTER(_vp), _i32]),
+    "fwx_copy_to_device": (ctypes.c_int, [_vp, _vp, _i32]),
+    "fwx_copy_to_host": (ctypes.c_int, [_vp, _vp, _i32]),
+    "fwx_device_free": (ctypes.c_int, [_vp]),
+    "fwx_device_synchronize": (ctypes.c_int, []),
+    "fwx_warmup": (ctypes.c_int, []),
+    "fwx_generate_graph": (ctypes.c_int, [_vp, _i32, ctypes.c_double, ctypes.c_uint32, _i32, _i32]),
+    "fwx_has_negative_cycle": (ctypes.c_int, [_vp, _i32]),
+}
+
+_lib = None
+
+
+def lib():
+    """Load libfwx.so once (raises OSError if it is not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise OSError(f"{LIB_PATH} is not built: run `make -C fw_cuda_amd` "
+                          "or __graft_entry__.build()")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name, None)
+            if f is None:
+                continue
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc):
+    if rc != OK:
+        raise FwxError(rc, lib().fwx_last_error().decode(errors="replace"))
+    return rc
+
+
+def ptr(a):
+    """Address of a numpy array or torch tensor's data as a ctypes void*."""
+    if hasattr(a, "data_ptr"):
+        return ctypes.c_void_p(a.data_ptr())
+    return ctypes.c_void_p(a.ctypes.data)

@@ -1,0 +1,42 @@
+// fwx_internal.hpp -- host-side plumbing shared by the libfwx translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+
+#include "../../include/fwx.h"
+
+namespace fwx {
+
+// Thread-local last-error message (fwx_last_error).
+void set_error(const std::string& msg);
+void clear_error();
+
+int fail(int code, const std::string& msg);
+int hip_fail(hipError_t e, const char* what, const char* file, int line);
+
+#define FWX_HIP(call)                                                  \
+  do {                                                                 \
+    hipError_t fwx_e_ = (call);                                        \
+    if (fwx_e_ != hipSuccess) return ::fwx::hip_fail(fwx_e_, #call, __FILE__, __LINE__); \
+  } while (0)
+
+#define FWX_LAUNCHED(what)                                             \
+  do {                                                                 \
+    hipError_t fwx_e_ = hipGetLastError();                             \
+    if (fwx_e_ != hipSuccess) return ::fwx::hip_fail(fwx_e_, what, __FILE__, __LINE__); \
+  } while (0)
+
+inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+// Plain-FW drivers (enqueue only; the C entry points add the final sync).
+int enqueue_baseline(int32_t* d, int32_t n, hipStream_t st);
+int enqueue_blocked(int32_t* d, int32_t n, int32_t tile, int32_t kappa, hipStream_t st);
+int enqueue_refshape(int32_t* d, int32_t n, hipStream_t st);
+
+// Tuned (tile, kappa) for a plain run of size n.
+void tuned_config(int32_t n, int32_t* tile, int32_t* kappa);
+
+}  // namespace fwx

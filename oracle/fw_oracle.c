@@ -1,0 +1,295 @@
+/*
+ * fw_oracle.c -- CPU restatement of the reference's Floyd-Warshall path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker: only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it.
+ * The product (libfwx.so and its CLIs) never links it and never falls
+ * back to it.
+ *
+ * Every function cites the reference file:line it restates.  How each one
+ * is pinned (golden vectors, closed forms, independent algorithms) is listed
+ * next to it; the tests that do the pinning live in tests/test_oracle.py.
+ *
+ * Integer arithmetic follows the reference's `int` semantics on the
+ * relaxation `D[i*n+k] + D[k*n+j]` (src/fw_baseline_cpu.cpp:9): the sum is
+ * formed with two's-complement wrap (computed through uint32_t so the C
+ * code itself has no signed overflow), then compared signed and strictly.
+ */
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#include <math.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define ORACLE_INF 1000000000 /* include/constants.h:3 */
+
+static inline int32_t wrap_add(int32_t a, int32_t b) {
+  return (int32_t)((uint32_t)a + (uint32_t)b);
+}
+
+/*
+ * oracle_fw_serial -- restates fwBaselineCPU (src/fw_baseline_cpu.cpp:3-16):
+ * k-i-j order, in place, strict `<`, store only on improvement.
+ *
+ * The reference re-reads D[i*n+k] on every j because the store to D[i*n+j]
+ * may alias it (at j == k).  The split loop below is the same computation:
+ * columns j < k see the old D[i][k]; at j == k the cell may change (only when
+ * D[k][k] < 0); columns j > k see the value after that step, exactly as the
+ * reference's in-order j loop does.  Row i == k aliases the b-row with the
+ * output row cell by cell, which an in-order loop handles identically.
+ *
+ * Pinned by: the reference's KATs (tests/golden/kats.json, taken from
+ * src/tests.cpp:59-132), bit-equality with the reference's own
+ * fwBaselineCPU built into oracle/_ref, brute-force shortest walks on tiny
+ * graphs, and Dijkstra (an independent algorithm) on non-negative graphs.
+ */
+void oracle_fw_serial(int32_t* D, int32_t n) {
+  const size_t N = (size_t)n;
+  for (size_t k = 0; k < N; ++k) {
+    const int32_t* rowk = D + k * N;
+    for (size_t i = 0; i < N; ++i) {
+      int32_t* rowi = D + i * N;
+      int32_t dik = rowi[k];
+      for (size_t j = 0; j < k; ++j) {
+        int32_t np = wrap_add(dik, rowk[j]);
+        if (np < rowi[j]) rowi[j] = np;
+      }
+      {
+        int32_t np = wrap_add(dik, rowk[k]);
+        if (np < rowi[k]) rowi[k] = np;
+        dik = rowi[k];
+      }
+      for (size_t j = k + 1; j < N; ++j) {
+        int32_t np = wrap_add(dik, rowk[j]);
+        if (np < rowi[j]) rowi[j] = np;
+      }
+    }
+  }
+}
+
+/*
+ * oracle_fw_rowpar -- the same k-i-j relaxation with the i loop split over
+ * threads.  When D[k][k] >= 0, step k leaves row k unchanged, so rows are
+ * independent within a step and the result is bit-identical to the serial
+ * order.  (With a negative cycle the two may differ; callers use it only for
+ * graphs without one, and tests check it equals oracle_fw_serial.)
+ */
+void oracle_fw_rowpar(int32_t* D, int32_t n, int32_t threads) {
+  const long N = (long)n;
+#ifdef _OPENMP
+  if (threads <= 0) threads = omp_get_max_threads();
+#else
+  (void)threads;
+#endif
+  for (long k = 0; k < N; ++k) {
+    const int32_t* rowk = D + k * N;
+#pragma omp parallel for schedule(static) num_threads(threads)
+    for (long i = 0; i < N; ++i) {
+      int32_t* rowi = D + i * N;
+      const int32_t dik = rowi[k];
+      if (i == k) {
+        for (long j = 0; j < N; ++j) {
+          int32_t np = wrap_add(rowi[k], rowk[j]);
+          if (np < rowi[j]) rowi[j] = np;
+        }
+        continue;
+      }
+      for (long j = 0; j < N; ++j) {
+        int32_t np = wrap_add(dik, rowk[j]);
+        rowi[j] = np < rowi[j] ? np : rowi[j];
+      }
+    }
+  }
+}
+
+/* oracle_has_negative_cycle -- restates hasNegativeCycle (src/utils.cpp:5-12). */
+int oracle_has_negative_cycle(const int32_t* D, int32_t n) {
+  for (int32_t i = 0; i < n; ++i)
+    if (D[(size_t)i * n + i] < 0) return 1;
+  return 0;
+}
+
+/*
+ * oracle_dijkstra_rows -- an algorithm independent of Floyd-Warshall, used to
+ * pin the FW oracle and to spot-check rows where an O(n^3) oracle is too slow.
+ *
+ * It solves single-source shortest paths on the *INF-completed* graph that
+ * the reference's FW computes over: every off-diagonal cell, INF included,
+ * is an edge with that weight (src/fw_baseline_cpu.cpp:9 never treats INF
+ * specially).  Dense O(n^2) Dijkstra; requires every weight >= 0 (returns -1
+ * otherwise).  dist[s][s] = min(0, D[s][s]) follows FW's own treatment of a
+ * non-negative diagonal (it is never improved).  Sums stay <= 2*INF < 2^31.
+ */
+int oracle_dijkstra_rows(const int32_t* W, int32_t n, const int32_t* src, int32_t nsrc,
+                         int32_t* out, int32_t threads) {
+  const size_t N = (size_t)n;
+  for (size_t c = 0; c < N * N; ++c)
+    if (W[c] < 0) return -1;
+  int bad = 0;
+#ifdef _OPENMP
+  if (threads <= 0) threads = omp_get_max_threads();
+#else
+  (void)threads;
+#endif
+#pragma omp parallel num_threads(threads)
+  {
+    int64_t* dist = (int64_t*)malloc(N * sizeof(int64_t));
+    unsigned char* done = (unsigned char*)malloc(N);
+    if (!dist || !done) {
+#pragma omp atomic write
+      bad = 1;
+    } else {
+#pragma omp for schedule(dynamic, 1)
+      for (int32_t q = 0; q < nsrc; ++q) {
+        const size_t s = (size_t)src[q];
+        for (size_t v = 0; v < N; ++v) { dist[v] = INT64_MAX; done[v] = 0; }
+        dist[s] = 0;
+        for (size_t it = 0; it < N; ++it) {
+          size_t u = N;
+          int64_t best = INT64_MAX;
+          for (size_t v = 0; v < N; ++v)
+            if (!done[v] && dist[v] < best) { best = dist[v]; u = v; }
+          if (u == N) break;
+          done[u] = 1;
+          const int32_t* wu = W + u * N;
+          for (size_t v = 0; v < N; ++v) {
+            if (done[v] || v == u) continue;
+            int64_t nd = best + (int64_t)wu[v];
+            if (nd < dist[v]) dist[v] = nd;
+          }
+        }
+        int32_t* o = out + (size_t)q * N;
+        for (size_t v = 0; v < N; ++v) o[v] = (int32_t)dist[v];
+        /* the diagonal keeps its input value when that is <= 0 (it is 0 for
+           every generated graph); a positive diagonal is improved to the
+           shortest cycle through s, as FW would do. */
+        {
+          int64_t cyc = (int64_t)W[s * N + s];
+          for (size_t v = 0; v < N; ++v) {
+            if (v == s) continue;
+            int64_t c2 = dist[v] + (int64_t)W[v * N + s];
+            if (c2 < cyc) cyc = c2;
+          }
+          o[s] = (int32_t)cyc;
+        }
+      }
+    }
+    free(dist);
+    free(done);
+  }
+  return bad ? -2 : 0;
+}
+
+/*
+ * oracle_brute_walks -- shortest walks of at most n-1 edges by repeated
+ * min-plus squaring of (W with a zero diagonal) -- no FW ordering at all.
+ * Used on tiny graphs only (O(n^3 log n)).  Valid when there is no negative
+ * cycle; the zero diagonal is taken as the empty walk, which matches FW on
+ * graphs with a non-negative diagonal.
+ */
+void oracle_brute_walks(const int32_t* W, int32_t n, int32_t* out) {
+  const size_t N = (size_t)n;
+  int64_t* A = (int64_t*)malloc(N * N * sizeof(int64_t));
+  int64_t* B = (int64_t*)malloc(N * N * sizeof(int64_t));
+  for (size_t c = 0; c < N * N; ++c) A[c] = W[c];
+  for (size_t i = 0; i < N; ++i) if (A[i * N + i] > 0) A[i * N + i] = 0;
+  for (size_t len = 1; len < N; len *= 2) {
+    for (size_t i = 0; i < N; ++i)
+      for (size_t j = 0; j < N; ++j) {
+        int64_t best = A[i * N + j];
+        for (size_t k = 0; k < N; ++k) {
+          int64_t v = A[i * N + k] + A[k * N + j];
+          if (v < best) best = v;
+        }
+        B[i * N + j] = best;
+      }
+    memcpy(A, B, N * N * sizeof(int64_t));
+  }
+  for (size_t c = 0; c < N * N; ++c) out[c] = (int32_t)A[c];
+  free(A);
+  free(B);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Differentiable (fp32) variant.  No reference code exists for it; these   */
+/* restate the textbook definitions the product's kernels implement.        */
+/* ------------------------------------------------------------------------ */
+
+/*
+ * oracle_fw_f32_pred -- fp32 Floyd-Warshall with predecessor tracking, over a
+ * weight matrix where an absent edge is +inf and the diagonal is taken as 0.
+ *   D[i][j] <- D[i][k] + D[k][j] and P[i][j] <- P[k][j]  iff strictly smaller.
+ * P[i][j] is the vertex before j on the chosen i->j path (i for a direct
+ * edge), -1 when j is unreachable or j == i.
+ * Pinned by: comparison with oracle_fw_serial on integer-valued fp32 weights
+ * (exact in fp32), and by path consistency (walking P reproduces D).
+ */
+void oracle_fw_f32_pred(const float* W, float* D, int32_t* P, int32_t n) {
+  const size_t N = (size_t)n;
+  for (size_t i = 0; i < N; ++i)
+    for (size_t j = 0; j < N; ++j) {
+      float w = W[i * N + j];
+      if (i == j) { D[i * N + j] = 0.0f; P[i * N + j] = -1; }
+      else if (isinf(w)) { D[i * N + j] = INFINITY; P[i * N + j] = -1; }
+      else { D[i * N + j] = w; P[i * N + j] = (int32_t)i; }
+    }
+  for (size_t k = 0; k < N; ++k)
+    for (size_t i = 0; i < N; ++i) {
+      const float dik = D[i * N + k];
+      if (isinf(dik)) continue;
+      for (size_t j = 0; j < N; ++j) {
+        float np = dik + D[k * N + j];
+        if (np < D[i * N + j]) { D[i * N + j] = np; P[i * N + j] = P[k * N + j]; }
+      }
+    }
+}
+
+/*
+ * oracle_grad_paths -- path-enumeration gradient of L = sum(G * D):
+ * for every reachable (s,t), s != t, walk the predecessor chain t -> s and add
+ * G[s][t] to dW of every edge on it.  dW must be zeroed by the caller.
+ * This is the definition of dL/dW for a piecewise-linear D (each D[s][t] is
+ * the sum of the weights on its path) and does not use distance ordering.
+ * Returns -1 if a predecessor chain is longer than n (a corrupt P).
+ */
+int oracle_grad_paths(const int32_t* P, const float* G, float* dW, int32_t n) {
+  const size_t N = (size_t)n;
+  for (size_t s = 0; s < N; ++s)
+    for (size_t t = 0; t < N; ++t) {
+      if (t == s) continue;
+      int32_t v = (int32_t)t;
+      if (P[s * N + (size_t)v] < 0) continue;
+      const float g = G[s * N + t];
+      size_t steps = 0;
+      while (v != (int32_t)s) {
+        int32_t u = P[s * N + (size_t)v];
+        if (u < 0 || ++steps > N) return -1;
+        dW[(size_t)u * N + (size_t)v] += g;
+        v = u;
+      }
+    }
+  return 0;
+}
+
+/*
+ * oracle_fw_f64 -- float64 min-plus closure used for central finite
+ * differences of L = sum(G * D) (absent edges +inf, diagonal 0).  k-i-j order.
+ * The weights are float64 so that W +/- eps is formed without fp32 rounding.
+ */
+void oracle_fw_f64(const double* W, double* D, int32_t n) {
+  const size_t N = (size_t)n;
+  for (size_t i = 0; i < N; ++i)
+    for (size_t j = 0; j < N; ++j)
+      D[i * N + j] = (i == j) ? 0.0 : W[i * N + j];
+  for (size_t k = 0; k < N; ++k)
+    for (size_t i = 0; i < N; ++i) {
+      const double dik = D[i * N + k];
+      if (isinf(dik)) continue;
+      for (size_t j = 0; j < N; ++j) {
+        double np = dik + D[k * N + j];
+        if (np < D[i * N + j]) D[i * N + j] = np;
+      }
+    }
+}

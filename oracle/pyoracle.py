@@ -1,0 +1,156 @@
+"""ctypes bindings to the C oracle (liboracle.so) and, when built, to the
+reference's own CPU sources (_ref/libfwref.so).  TEST INFRASTRUCTURE ONLY.
+
+Each wrapper names the C function it calls; the C file cites the reference
+file:line each one restates.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+INF = 1_000_000_000  # include/constants.h:3
+
+_i32p = ctypes.POINTER(ctypes.c_int32)
+_f32p = ctypes.POINTER(ctypes.c_float)
+_f64p = ctypes.POINTER(ctypes.c_double)
+
+
+def _ptr(a, t):
+    return a.ctypes.data_as(t)
+
+
+def _load(path):
+    if not os.path.exists(path):
+        raise FileNotFoundError(f"{path} is not built (run `make -C oracle`)")
+    return ctypes.CDLL(path)
+
+
+_lib = None
+_ref = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        _lib = _load(os.path.join(HERE, "liboracle.so"))
+        _lib.oracle_fw_serial.argtypes = [_i32p, ctypes.c_int32]
+        _lib.oracle_fw_rowpar.argtypes = [_i32p, ctypes.c_int32, ctypes.c_int32]
+        _lib.oracle_has_negative_cycle.argtypes = [_i32p, ctypes.c_int32]
+        _lib.oracle_has_negative_cycle.restype = ctypes.c_int
+        _lib.oracle_dijkstra_rows.argtypes = [_i32p, ctypes.c_int32, _i32p, ctypes.c_int32,
+                                              _i32p, ctypes.c_int32]
+        _lib.oracle_dijkstra_rows.restype = ctypes.c_int
+        _lib.oracle_brute_walks.argtypes = [_i32p, ctypes.c_int32, _i32p]
+        _lib.oracle_fw_f32_pred.argtypes = [_f32p, _f32p, _i32p, ctypes.c_int32]
+        _lib.oracle_grad_paths.argtypes = [_i32p, _f32p, _f32p, ctypes.c_int32]
+        _lib.oracle_grad_paths.restype = ctypes.c_int
+        _lib.oracle_fw_f64.argtypes = [_f64p, _f64p, ctypes.c_int32]
+    return _lib
+
+
+def ref_available():
+    return os.path.exists(os.path.join(HERE, "_ref", "libfwref.so"))
+
+
+def ref():
+    """The reference's own fwBaselineCPU / generateGraph1D / hasNegativeCycle."""
+    global _ref
+    if _ref is None:
+        _ref = _load(os.path.join(HERE, "_ref", "libfwref.so"))
+        _ref.ref_fw_baseline_cpu.argtypes = [_i32p, ctypes.c_int]
+        _ref.ref_generate_graph.argtypes = [_i32p, ctypes.c_int, ctypes.c_double, ctypes.c_uint]
+        _ref.ref_generate_graph.restype = ctypes.c_int
+        _ref.ref_has_negative_cycle.argtypes = [_i32p, ctypes.c_int]
+        _ref.ref_has_negative_cycle.restype = ctypes.c_int
+    return _ref
+
+
+def _sq(D):
+    D = np.ascontiguousarray(D, dtype=np.int32)
+    n = int(round(D.size ** 0.5))
+    assert n * n == D.size
+    return D.reshape(n, n), n
+
+
+def fw_serial(D):
+    """oracle_fw_serial: returns the closed copy (input untouched)."""
+    out, n = _sq(np.array(D, dtype=np.int32, copy=True))
+    lib().oracle_fw_serial(_ptr(out, _i32p), n)
+    return out
+
+
+def fw_rowpar(D, threads=0):
+    """oracle_fw_rowpar (OpenMP row-parallel, bit-identical without negative cycles)."""
+    out, n = _sq(np.array(D, dtype=np.int32, copy=True))
+    lib().oracle_fw_rowpar(_ptr(out, _i32p), n, int(threads))
+    return out
+
+
+def has_negative_cycle(D):
+    D, n = _sq(D)
+    return bool(lib().oracle_has_negative_cycle(_ptr(D, _i32p), n))
+
+
+def dijkstra_rows(W, rows, threads=0):
+    """oracle_dijkstra_rows: exact rows of the APSP over the INF-completed graph."""
+    W, n = _sq(W)
+    rows = np.ascontiguousarray(rows, dtype=np.int32)
+    out = np.empty((rows.size, n), dtype=np.int32)
+    rc = lib().oracle_dijkstra_rows(_ptr(W, _i32p), n, _ptr(rows, _i32p), rows.size,
+                                    _ptr(out, _i32p), int(threads))
+    if rc != 0:
+        raise ValueError(f"oracle_dijkstra_rows failed ({rc}): negative weights?")
+    return out
+
+
+def brute_walks(W):
+    W, n = _sq(W)
+    out = np.empty_like(W)
+    lib().oracle_brute_walks(_ptr(W, _i32p), n, _ptr(out, _i32p))
+    return out
+
+
+def fw_f32_pred(W):
+    """oracle_fw_f32_pred: fp32 FW over W (inf = absent, diagonal taken as 0)."""
+    W = np.ascontiguousarray(W, dtype=np.float32)
+    n = W.shape[0]
+    D = np.empty_like(W)
+    P = np.empty(W.shape, dtype=np.int32)
+    lib().oracle_fw_f32_pred(_ptr(W, _f32p), _ptr(D, _f32p), _ptr(P, _i32p), n)
+    return D, P
+
+
+def grad_paths(P, G):
+    """oracle_grad_paths: dL/dW for L = sum(G*D) by walking every path."""
+    P = np.ascontiguousarray(P, dtype=np.int32)
+    G = np.ascontiguousarray(G, dtype=np.float32)
+    n = P.shape[0]
+    dW = np.zeros((n, n), dtype=np.float32)
+    rc = lib().oracle_grad_paths(_ptr(P, _i32p), _ptr(G, _f32p), _ptr(dW, _f32p), n)
+    if rc != 0:
+        raise ValueError("corrupt predecessor matrix")
+    return dW
+
+
+def fw_f64(W):
+    W = np.ascontiguousarray(W, dtype=np.float64)
+    n = W.shape[0]
+    D = np.empty(W.shape, dtype=np.float64)
+    lib().oracle_fw_f64(_ptr(W, _f64p), _ptr(D, _f64p), n)
+    return D
+
+
+def ref_fw_baseline_cpu(D):
+    out, n = _sq(np.array(D, dtype=np.int32, copy=True))
+    ref().ref_fw_baseline_cpu(_ptr(out, _i32p), n)
+    return out
+
+
+def ref_generate_graph(n, density, seed):
+    out = np.empty((n, n), dtype=np.int32)
+    rc = ref().ref_generate_graph(_ptr(out, _i32p), n, float(density), int(seed))
+    if rc != 0:
+        raise ValueError("reference generator rejected the arguments")
+    return out

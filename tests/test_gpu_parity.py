@@ -1,0 +1,163 @@
+"""Bit-exact parity of the HIP kernels (through the C ABI of libfwx.so) with the
+CPU oracle, on the reference's own test cases (src/tests_gpu.cu:156-185) and
+on the gaps it leaves: ragged n, every tile/kappa, negative weights, the
+workload sizes of the benchmark configurations.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import fw_cuda_amd as fx
+from oracle import pyoracle as po
+
+INF = fx.INF
+pytestmark = pytest.mark.gpu
+
+# (algorithm, tile, kappa) combinations exercised on every case
+CONFIGS = [
+    ("baseline_gpu", 0, 0),
+    ("tiling_gpu_refshape", 0, 0),
+    ("tiling_gpu", 32, 0),
+    ("tiling_gpu", 64, 0),
+    ("tiling_gpu", 128, 0),
+    ("multi_layer_tiling_gpu", 0, 0),
+    ("multi_layer_tiling_gpu", 32, 4),
+    ("multi_layer_tiling_gpu", 64, 1),
+    ("multi_layer_tiling_gpu", 64, 2),
+    ("multi_layer_tiling_gpu", 64, 3),
+    ("multi_layer_tiling_gpu", 64, 4),
+    ("multi_layer_tiling_gpu", 64, 8),
+    ("multi_layer_tiling_gpu", 128, 2),
+    ("multi_layer_tiling_gpu", 32, 16),
+]
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def gpu(D, algo, tile, kappa):
+    return fx.floyd_warshall(D, algo, tile, kappa)
+
+
+def reweighted(rng, n, p):
+    W = fx.generate_graph(n, p, int(rng.integers(1, 2**31)), 0, 50).astype(np.int64)
+    h = rng.integers(0, 40, n)
+    adj = W != INF
+    W2 = np.where(adj, W + h[:, None] - h[None, :], INF).astype(np.int32)
+    np.fill_diagonal(W2, 0)
+    return W2
+
+
+@pytest.mark.parametrize("algo,tile,kappa", CONFIGS)
+def test_kats(kats, algo, tile, kappa):
+    for k in kats:
+        n = k["n"]
+        D = np.array(k["input"], dtype=np.int32).reshape(n, n)
+        say("KAT", k["name"], algo, tile, kappa)
+        out = gpu(D, algo, tile, kappa)
+        if k.get("check") == "negative_cycle":
+            assert fx.has_negative_cycle(out), k["name"]
+        else:
+            assert (out.ravel() == np.array(k["expected"])).all(), k["name"]
+
+
+# src/tests_gpu.cu:172-180
+REF_CASES = [(128, 0.9, 42), (256, 0.9, 123), (512, 0.9, 456), (1024, 0.9, 789),
+             (128, 0.1, 101), (256, 0.1, 202), (512, 0.1, 303), (1024, 0.1, 404)]
+
+
+@pytest.fixture(scope="module")
+def ref_cases():
+    out = []
+    for n, p, seed in REF_CASES:
+        g = fx.generate_graph(n, p, seed)
+        out.append((n, p, seed, g, po.fw_rowpar(g)))
+    return out
+
+
+@pytest.mark.parametrize("algo,tile,kappa", CONFIGS)
+def test_reference_seeded_graphs(ref_cases, algo, tile, kappa):
+    for n, p, seed, g, expect in ref_cases:
+        if algo == "baseline_gpu" and n > 512:
+            continue  # n launches; covered at 1024 by test_config1
+        say("seeded", n, p, seed, algo, tile, kappa)
+        assert (gpu(g, algo, tile, kappa) == expect).all(), (n, p, seed)
+
+
+RAGGED = [1, 2, 3, 5, 31, 33, 63, 65, 97, 127, 129, 130, 191, 257, 300, 383, 513, 777]
+
+
+@pytest.mark.parametrize("algo,tile,kappa", CONFIGS)
+def test_ragged_sizes(algo, tile, kappa):
+    rng = np.random.default_rng(1234)
+    for n in RAGGED:
+        p = float(rng.uniform(0.05, 0.9))
+        g = fx.generate_graph(n, p, int(rng.integers(1, 2**31)), 1, 1000)
+        say("ragged", n, algo, tile, kappa)
+        assert (gpu(g, algo, tile, kappa) == po.fw_rowpar(g)).all(), n
+
+
+@pytest.mark.parametrize("algo,tile,kappa", CONFIGS)
+def test_negative_weights(algo, tile, kappa):
+    """Negative edges without negative cycles: exercises the min3 (signed)
+    arithmetic path; the result must still be bit-exact."""
+    rng = np.random.default_rng(99)
+    for n in (7, 64, 100, 256, 401):
+        g = reweighted(rng, n, 0.3)
+        assert g.min() < 0
+        say("negative", n, algo, tile, kappa)
+        assert (gpu(g, algo, tile, kappa) == po.fw_rowpar(g)).all(), n
+
+
+def test_large_values_take_exact_path():
+    """Operands >= 2^30 (above INF) must take the exact min3 form."""
+    rng = np.random.default_rng(5)
+    n = 300
+    g = fx.generate_graph(n, 0.5, 77, 1, 1000)
+    mask = rng.random((n, n)) < 0.05
+    g = np.where(mask & (g == INF), 1_100_000_000, g).astype(np.int32)
+    np.fill_diagonal(g, 0)
+    expect = po.fw_rowpar(g)
+    for algo, tile, kappa in CONFIGS:
+        assert (gpu(g, algo, tile, kappa) == expect).all(), (algo, tile, kappa)
+
+
+def test_config1_n1024():
+    """n=1024, density 0.9, weights U[1,1000], seed 42: every algorithm."""
+    g = fx.generate_graph(1024, 0.9, 42, 1, 1000)
+    expect = po.fw_serial(g)
+    for algo, tile, kappa in CONFIGS:
+        say("config1", algo, tile, kappa)
+        assert (gpu(g, algo, tile, kappa) == expect).all(), (algo, tile, kappa)
+
+
+def test_idempotent_on_solved_matrix():
+    """The reference times its second call on the already-solved matrix
+    (src/timer.cu:43-55); running again must leave it unchanged."""
+    g = fx.generate_graph(700, 0.4, 3, 1, 1000)
+    once = gpu(g, "multi_layer_tiling_gpu", 0, 0)
+    twice = gpu(once, "multi_layer_tiling_gpu", 0, 0)
+    assert (once == twice).all()
+
+
+@pytest.mark.slow
+def test_config2_n8192_all_kappas():
+    """n=8192, density 0.9, U[1,1000], seed 42: multi_layer_tiling_gpu with
+    kappa in {1,2,4,6,8} bit-exact against the row-parallel oracle."""
+    g = fx.generate_graph(8192, 0.9, 42, 1, 1000)
+    say("config2: oracle start")
+    expect = po.fw_rowpar(g)
+    say("config2: oracle done")
+    with fx.DeviceMatrix(8192) as dm:
+        for kappa in (1, 2, 4, 6, 8):
+            dm.upload(g)
+            say("config2 kappa", kappa)
+            fx.run_device("multi_layer_tiling_gpu", dm.ptr, 8192, 64, kappa)
+            assert (dm.download() == expect).all(), kappa
+        dm.upload(g)
+        say("config2 tiling_gpu")
+        fx.run_device("tiling_gpu", dm.ptr, 8192, 0, 0)
+        assert (dm.download() == expect).all()

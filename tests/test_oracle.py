@@ -1,0 +1,166 @@
+"""Pins the CPU oracle (oracle/fw_oracle.c) before anything is checked against it.
+
+Each check is chosen so that a plausible mistake in the oracle fails it:
+the reference's own known-answer tests and its own fwBaselineCPU output
+(golden vectors and, where built, oracle/_ref), brute-force shortest walks by
+min-plus squaring (no FW ordering at all), Dijkstra (another algorithm), and
+exactness of the fp32 / gradient restatements on integer-valued inputs.
+"""
+import numpy as np
+import pytest
+
+from oracle import pyoracle as po
+
+INF = po.INF
+
+
+def random_graph(rng, n, p, lo=1, hi=1000):
+    W = np.where(rng.random((n, n)) < p, rng.integers(lo, hi + 1, (n, n)), INF).astype(np.int32)
+    np.fill_diagonal(W, 0)
+    return W
+
+
+def reweighted(rng, n, p):
+    """Graph with negative edges but no negative cycle: w(u,v) + h(u) - h(v)
+    with w >= 0 (Johnson reweighting in reverse); INF cells stay INF."""
+    W = random_graph(rng, n, p, 0, 50)
+    h = rng.integers(0, 40, n)
+    adj = W != INF
+    W2 = W.astype(np.int64) + h[:, None] - h[None, :]
+    W2 = np.where(adj, W2, INF).astype(np.int32)
+    np.fill_diagonal(W2, 0)
+    return W2
+
+
+def test_kats(kats):
+    for k in kats:
+        n = k["n"]
+        D = np.array(k["input"], dtype=np.int32).reshape(n, n)
+        out = po.fw_serial(D)
+        if k.get("check") == "negative_cycle":
+            assert po.has_negative_cycle(out), k["name"]
+        else:
+            assert (out.ravel() == np.array(k["expected"])).all(), k["name"]
+
+
+def test_reference_golden_outputs(gen_ref):
+    # fwBaselineCPU outputs captured from the reference's own sources
+    for c in gen_ref:
+        n = c["n"]
+        g = np.array(c["graph"], dtype=np.int32).reshape(n, n)
+        assert (po.fw_serial(g).ravel() == np.array(c["fw"])).all(), (n, c["seed"])
+
+
+@pytest.mark.skipif(not po.ref_available(), reason="oracle/_ref not built")
+def test_against_reference_build():
+    rng = np.random.default_rng(5)
+    for n, p in [(1, 0.5), (2, 1.0), (3, 0.5), (17, 0.3), (64, 0.9), (100, 0.1), (257, 0.5)]:
+        W = random_graph(rng, n, p)
+        assert (po.fw_serial(W) == po.ref_fw_baseline_cpu(W)).all(), n
+        W = reweighted(rng, n, p)
+        assert (po.fw_serial(W) == po.ref_fw_baseline_cpu(W)).all(), n
+    # negative cycle: both orders must agree cell for cell (same k-i-j loop)
+    W = np.array([[0, 2, INF], [INF, 0, 1], [INF, -3, 0]], dtype=np.int32)
+    assert (po.fw_serial(W) == po.ref_fw_baseline_cpu(W)).all()
+
+
+def test_brute_force_walks():
+    rng = np.random.default_rng(7)
+    for n in (1, 2, 5, 9, 16, 31):
+        for p in (0.1, 0.5, 1.0):
+            W = random_graph(rng, n, p)
+            assert (po.fw_serial(W) == po.brute_walks(W)).all(), (n, p)
+            W = reweighted(rng, n, p)
+            assert (po.fw_serial(W) == po.brute_walks(W)).all(), (n, p, "neg")
+
+
+def test_dijkstra_independent_algorithm():
+    rng = np.random.default_rng(11)
+    for n, p in [(50, 0.05), (200, 0.3), (333, 0.9)]:
+        W = random_graph(rng, n, p)
+        rows = np.arange(n, dtype=np.int32)
+        assert (po.dijkstra_rows(W, rows) == po.fw_serial(W)).all(), n
+
+
+def test_rowpar_equals_serial():
+    rng = np.random.default_rng(13)
+    for n, p in [(1, 1.0), (7, 0.5), (128, 0.9), (1000, 0.1)]:
+        W = random_graph(rng, n, p)
+        assert (po.fw_rowpar(W, 4) == po.fw_serial(W)).all(), n
+        W = reweighted(rng, n, p)
+        assert (po.fw_rowpar(W, 4) == po.fw_serial(W)).all(), n
+
+
+def test_planted_mistake_is_caught():
+    """A classic wrong loop order (k innermost) must fail the same checks."""
+    rng = np.random.default_rng(17)
+    W = random_graph(rng, 24, 0.2)
+    bad = W.astype(np.int64).copy()
+    n = W.shape[0]
+    for i in range(n):
+        for j in range(n):
+            for k in range(n):
+                bad[i, j] = min(bad[i, j], bad[i, k] + bad[k, j])
+    assert not (bad == po.brute_walks(W)).all()
+    assert (po.fw_serial(W) == po.brute_walks(W)).all()
+
+
+def test_f32_pred_matches_integer_oracle():
+    rng = np.random.default_rng(19)
+    for n, p in [(1, 0.5), (40, 0.2), (150, 0.6)]:
+        Wi = random_graph(rng, n, p)
+        Wf = np.where(Wi == INF, np.inf, Wi).astype(np.float32)
+        D, P = po.fw_f32_pred(Wf)
+        Di = po.fw_serial(Wi)
+        # with INF edges the integer oracle may route through them; compare
+        # only cells whose integer distance is below INF (truly reachable)
+        reach = Di < INF
+        assert (D[reach] == Di[reach].astype(np.float32)).all()
+        assert np.isinf(D[~reach]).all()
+        # walking P reproduces D exactly (integer-valued weights)
+        for s in range(n):
+            for t in range(n):
+                if s == t or not reach[s, t]:
+                    continue
+                v, tot = t, 0.0
+                while v != s:
+                    u = P[s, v]
+                    tot += Wf[u, v]
+                    v = u
+                assert tot == D[s, t]
+
+
+def _fd_grad(W, G, edges, eps):
+    g = []
+    W = W.astype(np.float64)
+    for (u, v) in edges:
+        Wp = W.copy()
+        Wm = W.copy()
+        Wp[u, v] += eps
+        Wm[u, v] -= eps
+        Dp = po.fw_f64(Wp)
+        Dm = po.fw_f64(Wm)
+        fin = np.isfinite(Dp)
+        g.append(((G[fin] * Dp[fin]).sum() - (G[fin] * Dm[fin]).sum()) / (2 * eps))
+    return np.array(g)
+
+
+def test_grad_paths_matches_finite_differences():
+    rng = np.random.default_rng(23)
+    n = 24
+    W = np.where(rng.random((n, n)) < 0.3, rng.uniform(0.5, 10.0, (n, n)), np.inf).astype(np.float32)
+    np.fill_diagonal(W, 0)
+    G = rng.standard_normal((n, n)).astype(np.float32)
+    D, P = po.fw_f32_pred(W)
+    dW = po.grad_paths(P, G)
+    edges = [(u, v) for u in range(n) for v in range(n) if u != v and np.isfinite(W[u, v])]
+    fd = _fd_grad(W, G.astype(np.float64), edges, 1e-3)
+    ana = np.array([dW[u, v] for (u, v) in edges], dtype=np.float64)
+    assert np.allclose(ana, fd, rtol=1e-5, atol=1e-4)
+    # a planted mistake (dropping the last edge of every path) is caught
+    bad = dW.copy()
+    for s in range(n):
+        for t in range(n):
+            if s != t and P[s, t] >= 0:
+                bad[P[s, t], t] -= G[s, t]
+    assert not np.allclose([bad[u, v] for (u, v) in edges], fd, rtol=1e-5, atol=1e-4)

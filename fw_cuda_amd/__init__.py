@@ -108,3 +108,70 @@ def floyd_warshall_(t, algo="multi_layer_tiling_gpu", tile=0, kappa=0):
     stream = torch.cuda.current_stream(t.device).cuda_stream
     run_device(algo, ptr(t), t.shape[0], tile, kappa, stream)
     return t
+
+
+class BatchPlan:
+    """Many graphs of uneven size solved together (fwx_batch_plan_*).
+
+    `sizes[g]` and element `offsets[g]` locate graph g inside the int32 device
+    buffer at address `d_data`.  run(use_graph=True) replays the captured
+    hipGraph; run(False) enqueues the same launches directly."""
+
+    def __init__(self, sizes, offsets, d_data, stream=None):
+        self.sizes = np.ascontiguousarray(sizes, dtype=np.int32)
+        self.offsets = np.ascontiguousarray(offsets, dtype=np.int64)
+        p = ctypes.c_void_p()
+        s = ctypes.c_void_p(stream) if stream else None
+        check(lib().fwx_batch_plan_create(ctypes.byref(p), ptr(self.sizes), ptr(self.offsets),
+                                          len(self.sizes), d_data, s))
+        self.plan = p
+
+    def run(self, use_graph=True):
+        check(lib().fwx_batch_plan_run(self.plan, 1 if use_graph else 0))
+
+    def stats(self):
+        st, la, it = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int64()
+        check(lib().fwx_batch_plan_stats(self.plan, ctypes.byref(st), ctypes.byref(la), ctypes.byref(it)))
+        return {"steps": st.value, "launches": la.value, "items": it.value}
+
+    def destroy(self):
+        if self.plan is not None and self.plan.value:
+            check(lib().fwx_batch_plan_destroy(self.plan))
+        self.plan = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+def batch_loop(sizes, offsets, d_data, stream=None):
+    """Comparator: one plain multi_layer_tiling_gpu call per graph."""
+    sizes = np.ascontiguousarray(sizes, dtype=np.int32)
+    offsets = np.ascontiguousarray(offsets, dtype=np.int64)
+    s = ctypes.c_void_p(stream) if stream else None
+    check(lib().fwx_batch_loop(ptr(sizes), ptr(offsets), len(sizes), d_data, s))
+
+
+def batch_layout(sizes, align=4):
+    """Element offsets packing the graphs back to back, each start rounded up
+    to `align` elements (16 bytes for align=4)."""
+    offs, pos = [], 0
+    for n in sizes:
+        offs.append(pos)
+        pos += int(n) * int(n)
+        pos = (pos + align - 1) // align * align
+    return np.array(offs, dtype=np.int64), pos
+
+
+def batch_config4(seed=11, count=512, lo=64, hi=2048):
+    """The batched workload: `count` graphs with n log-uniform in [lo, hi]
+    (numpy PCG64 seeded with `seed`), density per graph U[0.05, 0.9], int32
+    weights U[1, 1000]; graph g uses generator seed seed*100000 + g + 1."""
+    rng = np.random.default_rng(seed)
+    ns = np.exp(rng.uniform(np.log(lo), np.log(hi), count)).astype(np.int64)
+    ns = np.clip(ns, lo, hi).astype(np.int32)
+    dens = rng.uniform(0.05, 0.9, count)
+    seeds = [seed * 100000 + g + 1 for g in range(count)]
+    return ns, dens, seeds

@@ -56,7 +56,18 @@ SIGNATURES = {
     "fwx_warmup": (ctypes.c_int, []),
     "fwx_generate_graph": (ctypes.c_int, [_vp, _i32, ctypes.c_double, ctypes.c_uint32, _i32, _i32]),
     "fwx_has_negative_cycle": (ctypes.c_int, [_vp, _i32]),
+    "fwx_batch_plan_create": (ctypes.c_int, [ctypes.POINTER(_vp), _vp, _vp, _i32, _vp, _vp]),
+    "fwx_batch_plan_run": (ctypes.c_int, [_vp, _i32]),
+    "fwx_batch_plan_stats": (ctypes.c_int, [_vp, _i32p, _i32p, ctypes.POINTER(_i64)]),
+    "fwx_batch_plan_destroy": (ctypes.c_int, [_vp]),
+    "fwx_batch_loop": (ctypes.c_int, [_vp, _vp, _i32, _vp, _vp]),
+    "fwx_diff_forward": (ctypes.c_int, [_vp, _vp, _vp, _i32, _i32, _vp]),
+    "fwx_diff_backward": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _i32, _i32, _vp]),
+    "fwx_profile_begin": (ctypes.c_int, []),
+    "fwx_profile_end": (ctypes.c_int, [_vp, _vp, _vp]),
 }
+
+PROF_REST, PROF_LEAD, PROF_BASELINE, PROF_NCLASS = 0, 1, 2, 3
 
 _lib = None
 

@@ -242,11 +242,17 @@ static int run_blocked(int32_t* d, int32_t n, int32_t kappa, hipStream_t st) {
   const int R = (span % 128 == 0) ? 128 : (span % 64 == 0) ? 64 : 32;
   for (int kb = 0; kb < N; kb += kappa) {
     const int B = min(kappa, N - kb);
+    const int s0 = kb * T, S = B * T;
+    // algorithmic relaxations: every cell x the stage's real k values
+    const double L = (double)min(S, n - s0), nn = (double)n;
+    hipEvent_t ev;
+    if (int rc = prof_open(st, &ev)) return rc;
     for (int l = 0; l < B; ++l) {
       const int rc = lead_round<T, VEC>(d, n, kb + l, kb, B, N, st);
       if (rc) return rc;
     }
-    const int s0 = kb * T, S = B * T;
+    if (int rc = prof_close(st, ev, FWX_PROF_LEAD, (nn * nn - (nn - L) * (nn - L)) * L)) return rc;
+    if (int rc = prof_open(st, &ev)) return rc;
     int rc;
     if (R == 128)
       rc = rest_pass<128, VEC>(d, n, s0, S, st);
@@ -255,6 +261,7 @@ static int run_blocked(int32_t* d, int32_t n, int32_t kappa, hipStream_t st) {
     else
       rc = rest_pass<32, VEC>(d, n, s0, S, st);
     if (rc) return rc;
+    if (int rc2 = prof_close(st, ev, FWX_PROF_REST, (nn - L) * (nn - L) * L)) return rc2;
   }
   return FWX_OK;
 }

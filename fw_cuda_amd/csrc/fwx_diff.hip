@@ -1,0 +1,456 @@
+// fwx_diff.hip -- differentiable fp32 Floyd-Warshall (no reference counterpart).
+//
+// Forward: D = min-plus closure of W (diagonal taken as 0, +inf = no edge),
+// blocked three-phase FW over 64 x 64 tiles, batched over graphs (grid.y),
+// carrying the predecessor P[i][j] (vertex before j on the chosen i->j path)
+// through every product: when A[i][k] + B[k][j] < C[i][j] strictly, the
+// cell takes B's predecessor of (k, j).  Equal candidates keep the earlier
+// one, so ties resolve deterministically; with continuous weights they have
+// probability ~0.
+//
+// Backward of L = sum(G * D): each D[s][t] is the sum of the weights on the
+// chosen s->t path, so dL/dW[u][v] = sum of G[s][t] over the (s,t) whose path
+// uses edge (u,v).  Per source s (one workgroup): sort the vertices by
+// D[s][.] descending (bitonic sort in LDS), then walk them far to near: a
+// vertex's accumulated gradient g[t] = G[s][t] + sum of its tree children's
+// g is final when it is reached (children are strictly farther, weights > 0),
+// is added to dW[P[s][t]][t] and pushed to its parent.
+//
+// Without a stored P, the backward recomputes each source block's
+// predecessors as P[s][t] = argmin_u D[s][u] + W[u][t] (u != t): the last
+// edge of a shortest path satisfies D[s][t] = D[s][u] + W[u][t].  It needs a
+// 256 x n int32 workspace instead of n x n.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+
+#include "fwx_internal.hpp"
+
+namespace fwx {
+
+namespace {
+
+constexpr int FT = 64;   // tile
+constexpr int FKC = 32;  // k chunk
+constexpr int FLDA = FKC + 4;
+constexpr float kFInf = __builtin_huge_valf();
+
+// Window of a row-major fp32 matrix and its predecessor matrix.
+struct FView {
+  float* p;
+  int32_t* pp;  // may be null
+  int32_t ld, rlim, clim;
+  __device__ __forceinline__ bool in(int r, int c) const { return r < rlim && c < clim; }
+};
+
+__device__ __forceinline__ FView fview(float* D, int32_t* P, int32_t n, int32_t r0, int32_t c0) {
+  const int64_t o = (int64_t)r0 * n + c0;
+  return FView{D + o, P ? P + o : nullptr, n, n - r0, n - c0};
+}
+
+struct FSmem {
+  float a[FT * FLDA];
+  float b[FKC * FT];
+  int32_t bp[FKC * FT];
+};
+
+// acc <- min(acc, A (x) B) over depth k values, predecessor from B's
+// predecessors (PRED) or, when BP_IS_ROW, the operand row index k itself.
+// ExcludeDiag: B[k][j] with k == j (in global indices) is treated as +inf.
+template <bool PRED, bool BP_IS_ROW>
+__device__ void fprod(FSmem& s, const FView& A, const FView& B, int32_t depth, int32_t brow0, int32_t bcol0,
+                      bool exclude_diag, float (&acc)[4][4], int32_t (&pacc)[4][4]) {
+  const int tid = threadIdx.x;
+  const int tx = tid & 15, ty = tid >> 4;
+  for (int kc = 0; kc < depth; kc += FKC) {
+    // stage A: 64 rows x 32 k -> 8 per thread
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const int idx = e * 256 + tid;
+      const int r = idx / FKC, k = idx % FKC;
+      s.a[r * FLDA + k] = A.in(r, kc + k) ? A.p[r * A.ld + kc + k] : kFInf;
+    }
+    // stage B: 32 k x 64 cols -> 8 per thread
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const int idx = e * 256 + tid;
+      const int k = idx / FT, c = idx % FT;
+      const bool ok = B.in(kc + k, c) && !(exclude_diag && brow0 + kc + k == bcol0 + c);
+      s.b[k * FT + c] = ok ? B.p[(kc + k) * B.ld + c] : kFInf;
+      if (PRED) {
+        if (BP_IS_ROW)
+          s.bp[k * FT + c] = brow0 + kc + k;
+        else
+          s.bp[k * FT + c] = ok ? B.pp[(kc + k) * B.ld + c] : -1;
+      }
+    }
+    __syncthreads();
+#pragma unroll 4
+    for (int k = 0; k < FKC; ++k) {
+      float a[4], b[4];
+      int32_t bp[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) a[r] = s.a[(ty + 16 * r) * FLDA + k];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        b[c] = s.b[k * FT + tx + 16 * c];
+        if (PRED) bp[c] = s.bp[k * FT + tx + 16 * c];
+      }
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          const float cand = a[r] + b[c];
+          if (cand < acc[r][c]) {
+            acc[r][c] = cand;
+            if (PRED) pacc[r][c] = bp[c];
+          }
+        }
+    }
+    __syncthreads();
+  }
+}
+
+template <bool PRED>
+__device__ __forceinline__ void fload(const FView& C, float (&acc)[4][4], int32_t (&pacc)[4][4]) {
+  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+#pragma unroll
+  for (int r = 0; r < 4; ++r)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const int rr = ty + 16 * r, cc = tx + 16 * c;
+      const bool ok = C.in(rr, cc);
+      acc[r][c] = ok ? C.p[rr * C.ld + cc] : kFInf;
+      if (PRED) pacc[r][c] = ok ? C.pp[rr * C.ld + cc] : -1;
+    }
+}
+
+template <bool PRED>
+__device__ __forceinline__ void fstore(const FView& C, const float (&acc)[4][4], const int32_t (&pacc)[4][4]) {
+  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+#pragma unroll
+  for (int r = 0; r < 4; ++r)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const int rr = ty + 16 * r, cc = tx + 16 * c;
+      if (C.in(rr, cc)) {
+        C.p[rr * C.ld + cc] = acc[r][c];
+        if (PRED) C.pp[rr * C.ld + cc] = pacc[r][c];
+      }
+    }
+}
+
+// D = W with a zero diagonal; P = i for a present edge, else -1.
+template <bool PRED>
+__global__ __launch_bounds__(256) void k_finit(const float* __restrict__ W, float* __restrict__ D,
+                                               int32_t* __restrict__ P, int32_t n, int64_t total) {
+  const int64_t nn = (int64_t)n * n;
+  for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < total; e += (int64_t)gridDim.x * 256) {
+    const int64_t c = e % nn;
+    const int32_t i = (int32_t)(c / n), j = (int32_t)(c % n);
+    const float w = W[e];
+    const bool edge = i != j && w < kFInf;
+    D[e] = i == j ? 0.0f : w;
+    if (PRED) P[e] = edge ? i : -1;
+  }
+}
+
+template <bool PRED>
+__global__ __launch_bounds__(256) void k_fdiag(float* __restrict__ D, int32_t* __restrict__ P, int32_t n,
+                                               int32_t c) {
+  __shared__ float rowv[2][FT], colv[2][FT];
+  __shared__ int32_t rowp[2][FT];
+  const int64_t g = blockIdx.y;
+  float* Dg = D + g * (int64_t)n * n;
+  int32_t* Pg = PRED ? P + g * (int64_t)n * n : nullptr;
+  const FView V = fview(Dg, Pg, n, c * FT, c * FT);
+  float acc[4][4];
+  int32_t pacc[4][4];
+  fload<PRED>(V, acc, pacc);
+  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+  const int kmax = min(FT, n - c * FT);
+#pragma unroll
+  for (int kk = 0; kk < 4; ++kk) {
+    for (int k16 = 0; k16 < 16; ++k16) {
+      const int k = kk * 16 + k16;
+      if (k >= kmax) break;
+      const int b = k & 1;
+      if (ty == k16) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          rowv[b][tx + 16 * q] = acc[kk][q];
+          if (PRED) rowp[b][tx + 16 * q] = pacc[kk][q];
+        }
+      }
+      if (tx == k16) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) colv[b][ty + 16 * r] = acc[r][kk];
+      }
+      __syncthreads();
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const float ck = colv[b][ty + 16 * r];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const float cand = ck + rowv[b][tx + 16 * q];
+          if (cand < acc[r][q]) {
+            acc[r][q] = cand;
+            if (PRED) pacc[r][q] = rowp[b][tx + 16 * q];
+          }
+        }
+      }
+    }
+  }
+  fstore<PRED>(V, acc, pacc);
+}
+
+template <bool PRED>
+__global__ __launch_bounds__(256) void k_fpanel(float* __restrict__ D, int32_t* __restrict__ P, int32_t n,
+                                                int32_t c, int32_t N) {
+  __shared__ FSmem s;
+  const int64_t g = blockIdx.y;
+  float* Dg = D + g * (int64_t)n * n;
+  int32_t* Pg = PRED ? P + g * (int64_t)n * n : nullptr;
+  const int b = blockIdx.x;
+  const FView diag = fview(Dg, Pg, n, c * FT, c * FT);
+  FView A, B, C;
+  if (b < N - 1) {
+    const int j = b + (b >= c);
+    C = fview(Dg, Pg, n, c * FT, j * FT);
+    A = diag;
+    B = C;
+  } else {
+    int i = b - (N - 1);
+    i += (i >= c);
+    C = fview(Dg, Pg, n, i * FT, c * FT);
+    A = C;
+    B = diag;
+  }
+  float acc[4][4];
+  int32_t pacc[4][4];
+  fload<PRED>(C, acc, pacc);
+  fprod<PRED, false>(s, A, B, FT, 0, 0, false, acc, pacc);
+  fstore<PRED>(C, acc, pacc);
+}
+
+template <bool PRED>
+__global__ __launch_bounds__(256) void k_frest(float* __restrict__ D, int32_t* __restrict__ P, int32_t n,
+                                               int32_t c, int32_t N) {
+  __shared__ FSmem s;
+  const int64_t g = blockIdx.y;
+  float* Dg = D + g * (int64_t)n * n;
+  int32_t* Pg = PRED ? P + g * (int64_t)n * n : nullptr;
+  int i = blockIdx.x / (N - 1), j = blockIdx.x % (N - 1);
+  i += (i >= c);
+  j += (j >= c);
+  const FView A = fview(Dg, Pg, n, i * FT, c * FT);
+  const FView B = fview(Dg, Pg, n, c * FT, j * FT);
+  const FView C = fview(Dg, Pg, n, i * FT, j * FT);
+  float acc[4][4];
+  int32_t pacc[4][4];
+  fload<PRED>(C, acc, pacc);
+  fprod<PRED, false>(s, A, B, FT, 0, 0, false, acc, pacc);
+  fstore<PRED>(C, acc, pacc);
+}
+
+// Predecessors of sources [s0, s0+rows) recomputed from D and W:
+// Pw[r][t] = argmin_{u != t} D[s0+r][u] + W[u][t], -1 for t == s or
+// unreachable t.  Grid: (ceil(rows/64) * ceil(n/64), batch).
+__global__ __launch_bounds__(256) void k_fpred(const float* __restrict__ W, const float* __restrict__ D,
+                                               int32_t* __restrict__ Pw, int32_t n, int32_t s0, int32_t rows) {
+  __shared__ FSmem s;
+  const int64_t g = blockIdx.y;
+  const int nt = (n + FT - 1) / FT;
+  const int bi = blockIdx.x / nt, bj = blockIdx.x % nt;
+  float* Dg = const_cast<float*>(D) + g * (int64_t)n * n;
+  float* Wg = const_cast<float*>(W) + g * (int64_t)n * n;
+  FView A = fview(Dg, nullptr, n, s0 + bi * FT, 0);
+  A.rlim = min(A.rlim, rows - bi * FT);
+  const FView B = fview(Wg, nullptr, n, 0, bj * FT);
+  float acc[4][4];
+  int32_t pacc[4][4];
+#pragma unroll
+  for (int r = 0; r < 4; ++r)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      acc[r][c] = kFInf;
+      pacc[r][c] = -1;
+    }
+  fprod<true, true>(s, A, B, n + (FKC - n % FKC) % FKC, 0, bj * FT, true, acc, pacc);
+  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+  int32_t* Pg = Pw + g * (int64_t)rows * n;
+#pragma unroll
+  for (int r = 0; r < 4; ++r)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const int lr = bi * FT + ty + 16 * r, t = bj * FT + tx + 16 * c;
+      if (lr < rows && t < n) {
+        const int32_t src = s0 + lr;
+        Pg[(int64_t)lr * n + t] = (t == src || !(acc[r][c] < kFInf)) ? -1 : pacc[r][c];
+      }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_ffill(float* __restrict__ x, int64_t total) {
+  for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < total; e += (int64_t)gridDim.x * 256) x[e] = 0.0f;
+}
+
+// Backward for sources [s0, s0 + rows) of every graph.  Pr points at the
+// predecessor rows of those sources (row stride n, graph stride pstride).
+// Dynamic LDS: key[n2] float, idx[n2] int, g[n] float, par[n] int.
+__global__ __launch_bounds__(256) void k_fback(const float* __restrict__ D, const int32_t* __restrict__ Pr,
+                                               int64_t pstride, const float* __restrict__ G,
+                                               float* __restrict__ dW, int32_t n, int32_t n2, int32_t s0) {
+  extern __shared__ unsigned char smem[];
+  float* key = reinterpret_cast<float*>(smem);
+  int32_t* idx = reinterpret_cast<int32_t*>(key + n2);
+  float* gacc = reinterpret_cast<float*>(idx + n2);
+  int32_t* par = reinterpret_cast<int32_t*>(gacc + n);
+  const int64_t g = blockIdx.y;
+  const int32_t lr = blockIdx.x;
+  const int32_t src = s0 + lr;
+  const int64_t nn = (int64_t)n * n;
+  const float* Drow = D + g * nn + (int64_t)src * n;
+  const float* Grow = G + g * nn + (int64_t)src * n;
+  const int32_t* Prow = Pr + g * pstride + (int64_t)lr * n;
+  float* dWg = dW + g * nn;
+  const int tid = threadIdx.x;
+  for (int t = tid; t < n2; t += 256) {
+    if (t < n) {
+      const float d = Drow[t];
+      const int32_t p = Prow[t];
+      key[t] = (t != src && p >= 0 && d < kFInf) ? d : -kFInf;
+      gacc[t] = Grow[t];
+      par[t] = p;
+    } else {
+      key[t] = -kFInf;
+    }
+    idx[t] = t;
+  }
+  __syncthreads();
+  // bitonic sort, descending by key
+  for (int k = 2; k <= n2; k <<= 1) {
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      for (int i = tid; i < n2; i += 256) {
+        const int l = i ^ j;
+        if (l > i) {
+          const bool desc = (i & k) == 0;
+          const float ki = key[i], kl = key[l];
+          if (desc ? (ki < kl) : (ki > kl)) {
+            key[i] = kl;
+            key[l] = ki;
+            const int32_t t = idx[i];
+            idx[i] = idx[l];
+            idx[l] = t;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  // far-to-near accumulation: one lane walks the order
+  if (tid == 0) {
+    for (int q = 0; q < n; ++q) {
+      if (!(key[q] > -kFInf)) break;
+      const int32_t t = idx[q];
+      const float gt = gacc[t];
+      const int32_t p = par[t];
+      gacc[p] += gt;
+      atomicAdd(&dWg[(int64_t)p * n + t], gt);
+    }
+  }
+}
+
+template <bool PRED>
+int forward_impl(const float* W, float* D, int32_t* P, int32_t n, int32_t batch, hipStream_t st) {
+  const int64_t total = (int64_t)n * n * batch;
+  const int fillblocks = (int)std::min<int64_t>((total + 255) / 256, 65536);
+  k_finit<PRED><<<fillblocks, 256, 0, st>>>(W, D, P, n, total);
+  FWX_LAUNCHED("k_finit");
+  const int N = (n + FT - 1) / FT;
+  for (int c = 0; c < N; ++c) {
+    k_fdiag<PRED><<<dim3(1, batch), 256, 0, st>>>(D, P, n, c);
+    FWX_LAUNCHED("k_fdiag");
+    if (N > 1) {
+      k_fpanel<PRED><<<dim3(2 * (N - 1), batch), 256, 0, st>>>(D, P, n, c, N);
+      FWX_LAUNCHED("k_fpanel");
+      k_frest<PRED><<<dim3((N - 1) * (N - 1), batch), 256, 0, st>>>(D, P, n, c, N);
+      FWX_LAUNCHED("k_frest");
+    }
+  }
+  return FWX_OK;
+}
+
+constexpr int32_t kDiffMaxN = 8192;  // backward keeps 4 words per vertex in LDS
+
+int check_diff(const void* a, const void* b, int32_t n, int32_t batch) {
+  if (!a || !b) return fail(FWX_ERR_INVALID_ARG, "null matrix pointer");
+  if (n <= 0 || n > kDiffMaxN) return fail(FWX_ERR_INVALID_ARG, "differentiable FW needs 1 <= n <= 8192");
+  if (batch <= 0 || batch > 65535) return fail(FWX_ERR_INVALID_ARG, "batch must be in [1, 65535]");
+  return FWX_OK;
+}
+
+}  // namespace
+
+}  // namespace fwx
+
+using namespace fwx;
+
+extern "C" {
+
+int fwx_diff_forward(const float* d_W, float* d_D, int32_t* d_P, int32_t n, int32_t batch, void* stream) {
+  clear_error();
+  if (int rc = check_diff(d_W, d_D, n, batch)) return rc;
+  hipStream_t st = as_stream(stream);
+  const int rc = d_P ? forward_impl<true>(d_W, d_D, d_P, n, batch, st)
+                     : forward_impl<false>(d_W, d_D, nullptr, n, batch, st);
+  if (rc) return rc;
+  FWX_HIP(hipStreamSynchronize(st));
+  return FWX_OK;
+}
+
+int fwx_diff_backward(const float* d_W, const float* d_D, const int32_t* d_P, const float* d_G, float* d_dW,
+                      int32_t n, int32_t batch, void* stream) {
+  clear_error();
+  if (int rc = check_diff(d_D, d_G, n, batch)) return rc;
+  if (!d_dW) return fail(FWX_ERR_INVALID_ARG, "null gradient output");
+  if (!d_P && !d_W) return fail(FWX_ERR_INVALID_ARG, "recomputing predecessors needs W");
+  hipStream_t st = as_stream(stream);
+  const int64_t total = (int64_t)n * n * batch;
+  k_ffill<<<(int)std::min<int64_t>((total + 255) / 256, 65536), 256, 0, st>>>(d_dW, total);
+  FWX_LAUNCHED("k_ffill");
+  int32_t n2 = 1;
+  while (n2 < n) n2 <<= 1;
+  const size_t lds = (size_t)n2 * 8 + (size_t)n * 8;
+  if (lds > 65536)
+    FWX_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fback),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  if (d_P) {
+    k_fback<<<dim3(n, batch), 256, lds, st>>>(d_D, d_P, (int64_t)n * n, d_G, d_dW, n, n2, 0);
+    FWX_LAUNCHED("k_fback");
+  } else {
+    // source blocks of 256 rows; workspace 256 x n x batch int32
+    const int32_t rows = std::min<int32_t>(256, n);
+    int32_t* ws = nullptr;
+    FWX_HIP(hipMallocAsync(reinterpret_cast<void**>(&ws), (size_t)rows * n * batch * sizeof(int32_t), st));
+    int rc = FWX_OK;
+    for (int32_t s0 = 0; s0 < n && rc == FWX_OK; s0 += rows) {
+      const int32_t r = std::min(rows, n - s0);
+      const int nt = (n + FT - 1) / FT;
+      k_fpred<<<dim3(((r + FT - 1) / FT) * nt, batch), 256, 0, st>>>(d_W, d_D, ws, n, s0, r);
+      hipError_t e = hipGetLastError();
+      if (e == hipSuccess) {
+        k_fback<<<dim3(r, batch), 256, lds, st>>>(d_D, ws, (int64_t)r * n, d_G, d_dW, n, n2, s0);
+        e = hipGetLastError();
+      }
+      if (e != hipSuccess) rc = hip_fail(e, "k_fpred/k_fback", __FILE__, __LINE__);
+    }
+    (void)hipFreeAsync(ws, st);
+    if (rc) return rc;
+  }
+  FWX_HIP(hipStreamSynchronize(st));
+  return FWX_OK;
+}
+
+}  // extern "C"

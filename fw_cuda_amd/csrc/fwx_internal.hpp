@@ -36,6 +36,12 @@ int enqueue_baseline(int32_t* d, int32_t n, hipStream_t st);
 int enqueue_blocked(int32_t* d, int32_t n, int32_t tile, int32_t kappa, hipStream_t st);
 int enqueue_refshape(int32_t* d, int32_t n, hipStream_t st);
 
+// Event profiling (fwx_profile.hip): prof_open records a start event when
+// profiling is on (else leaves *a null); prof_close records the end event.
+bool profiling();
+int prof_open(hipStream_t st, hipEvent_t* a);
+int prof_close(hipStream_t st, hipEvent_t a, int cls, double work);
+
 // Tuned (tile, kappa) for a plain run of size n.
 void tuned_config(int32_t n, int32_t* tile, int32_t* kappa);
 

@@ -26,7 +26,7 @@
 // Unroll of the k-quad loop inside a chunk (register pressure vs. LDS
 // latency hiding).
 #ifndef FWX_QUNROLL
-#define FWX_QUNROLL 2
+#define FWX_QUNROLL 1
 #endif
 
 namespace fwx {
@@ -93,28 +93,57 @@ struct TileProduct {
     return (uint32_t)x.x | (uint32_t)x.y | (uint32_t)x.z | (uint32_t)x.w;
   }
 
-  // global -> registers for chunk starting at k offset kc0
+  // global -> registers for the chunk starting at k offset kc0.  When the
+  // whole chunk is inside the matrix (the common case) every address is a
+  // uniform row base (SGPRs) plus one per-thread offset (one VGPR) plus an
+  // immediate; the bounds-checked path handles the ragged edge.
   __device__ __forceinline__ void fetch(const View& A, const View& B, int32_t kc0) {
     const int tid = threadIdx.x;
     uint32_t o = 0;
+    static_assert(NT % (BN / 4) == 0, "B staging rows must be uniform per pass");
+    constexpr int BROWS = NT / (BN / 4);  // k pairs per B pass
+    const bool full = VEC && A.rlim >= BM && A.clim >= kc0 + KC && B.rlim >= kc0 + KC && B.clim >= BN;
+    if (full) {
+      const int32_t* pa = A.p + kc0;
+      const int32_t oa = (tid / AG) * A.ld + (tid % AG) * 4;
 #pragma unroll
-    for (int p = 0; p < APASS; ++p) {
-      const int r = p * AROWS + tid / AG;
-      const int k = (tid % AG) * 4;
-      if (BM % AROWS == 0 || r < BM) {
-        ra[p] = load4(A, r, kc0 + k);
-        o |= orbits(ra[p]);
+      for (int p = 0; p < APASS; ++p) {
+        if (BM % AROWS == 0 || p * AROWS + tid / AG < BM) {
+          ra[p] = *reinterpret_cast<const int4*>(pa + (p * AROWS) * A.ld + oa);
+          o |= orbits(ra[p]);
+        }
       }
-    }
+      const int32_t* pb = B.p + kc0 * B.ld;
+      const int32_t ob = (tid / (BN / 4)) * 2 * B.ld + (tid % (BN / 4)) * 4;
 #pragma unroll
-    for (int u = 0; u < BPASS; ++u) {
-      const int idx = u * NT + tid;
-      if (BU % NT == 0 || idx < BU) {
-        const int pr = idx / (BN / 4);
-        const int j = (idx % (BN / 4)) * 4;
-        rb0[u] = load4(B, kc0 + 2 * pr, j);
-        rb1[u] = load4(B, kc0 + 2 * pr + 1, j);
-        o |= orbits(rb0[u]) | orbits(rb1[u]);
+      for (int u = 0; u < BPASS; ++u) {
+        if (BU % NT == 0 || u * NT + tid < BU) {
+          const int32_t* q = pb + (u * BROWS * 2) * B.ld + ob;
+          rb0[u] = *reinterpret_cast<const int4*>(q);
+          rb1[u] = *reinterpret_cast<const int4*>(q + B.ld);
+          o |= orbits(rb0[u]) | orbits(rb1[u]);
+        }
+      }
+    } else {
+#pragma unroll
+      for (int p = 0; p < APASS; ++p) {
+        const int r = p * AROWS + tid / AG;
+        const int k = (tid % AG) * 4;
+        if (BM % AROWS == 0 || r < BM) {
+          ra[p] = load4(A, r, kc0 + k);
+          o |= orbits(ra[p]);
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < BPASS; ++u) {
+        const int idx = u * NT + tid;
+        if (BU % NT == 0 || idx < BU) {
+          const int pr = idx / (BN / 4);
+          const int j = (idx % (BN / 4)) * 4;
+          rb0[u] = load4(B, kc0 + 2 * pr, j);
+          rb1[u] = load4(B, kc0 + 2 * pr + 1, j);
+          o |= orbits(rb0[u]) | orbits(rb1[u]);
+        }
       }
     }
     bad = (o & kPk64Mask) != 0u;
@@ -207,10 +236,13 @@ struct TileProduct {
     const int tx = threadIdx.x & 15;
     const int ty = threadIdx.x >> 4;
     if (full(C)) {
+      const int32_t oc = ty * C.ld + tx;
 #pragma unroll
-      for (int r = 0; r < TM; ++r)
+      for (int r = 0; r < TM; ++r) {
+        const int32_t* pr = C.p + (16 * r) * C.ld;
 #pragma unroll
-        for (int c = 0; c < TN; ++c) acc[r][c] = *C.at(ty + 16 * r, tx + 16 * c);
+        for (int c = 0; c < TN; ++c) acc[r][c] = pr[oc + 16 * c];
+      }
     } else {
 #pragma unroll
       for (int r = 0; r < TM; ++r)
@@ -224,10 +256,13 @@ struct TileProduct {
     const int tx = threadIdx.x & 15;
     const int ty = threadIdx.x >> 4;
     if (full(C)) {
+      const int32_t oc = ty * C.ld + tx;
 #pragma unroll
-      for (int r = 0; r < TM; ++r)
+      for (int r = 0; r < TM; ++r) {
+        int32_t* pr = C.p + (16 * r) * C.ld;
 #pragma unroll
-        for (int c = 0; c < TN; ++c) *C.at(ty + 16 * r, tx + 16 * c) = acc[r][c];
+        for (int c = 0; c < TN; ++c) pr[oc + 16 * c] = acc[r][c];
+      }
     } else {
 #pragma unroll
       for (int r = 0; r < TM; ++r)

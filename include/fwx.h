@@ -111,6 +111,68 @@ int fwx_generate_graph(int32_t* h_D, int32_t n, double density, uint32_t seed,
 /* 1 if any D[i][i] < 0 (reference hasNegativeCycle, src/utils.cpp:5-12). */
 int fwx_has_negative_cycle(const int32_t* h_D, int32_t n);
 
+/* ---------------- batched FW (no reference counterpart) ----------------- */
+/*
+ * Many independent graphs solved together.  Graph g is an n_g x n_g row-major
+ * int32 matrix at d_data + offsets[g] (element offsets; `sizes` and
+ * `offsets` are host arrays of `count` entries; d_data 16-byte aligned).
+ * The plan builds every graph's multi-layer schedule (tile and kappa from
+ * the tuned table per size class), merges step j of all graphs into one
+ * launch per work kind, largest graphs first, and uploads the work lists.
+ * fwx_batch_plan_run(plan, 1) captures the launch sequence into a hipGraph
+ * on first use and replays it; with 0 it enqueues the launches directly.
+ * Results equal a plain fwx_multi_layer_tiling_gpu call per graph.  The plan
+ * owns device memory (its work lists) until fwx_batch_plan_destroy.
+ */
+typedef struct fwx_batch_plan fwx_batch_plan;
+int fwx_batch_plan_create(fwx_batch_plan** plan, const int32_t* sizes, const int64_t* offsets,
+                          int32_t count, int32_t* d_data, void* stream);
+int fwx_batch_plan_run(fwx_batch_plan* plan, int32_t use_graph);
+/* Dependent steps, kernel launches per run, and work items of a plan. */
+int fwx_batch_plan_stats(const fwx_batch_plan* plan, int32_t* steps, int32_t* launches, int64_t* items);
+int fwx_batch_plan_destroy(fwx_batch_plan* plan);
+/* The comparator: one plain fwx_multi_layer_tiling_gpu call (with its final
+ * stream synchronisation) per graph, in order. */
+int fwx_batch_loop(const int32_t* sizes, const int64_t* offsets, int32_t count, int32_t* d_data,
+                   void* stream);
+
+/* ---------------- differentiable fp32 FW (no reference counterpart) ----- */
+/*
+ * Forward, for `batch` contiguous n x n fp32 graphs (1 <= n <= 8192):
+ * D = min-plus closure of W with the diagonal taken as 0 and +inf meaning
+ * no edge; P[i][j] = vertex before j on the chosen i->j path (i for a direct
+ * edge), -1 when there is none.  d_P may be NULL (no predecessors kept).
+ * Weights must be positive for the backward's ordering argument.
+ */
+int fwx_diff_forward(const float* d_W, float* d_D, int32_t* d_P, int32_t n, int32_t batch,
+                     void* stream);
+/*
+ * Backward of L = sum(G * D): dW[u][v] = sum of G[s][t] over every (s,t)
+ * whose chosen path uses edge (u,v); d_dW is overwritten.  Per source s the
+ * vertices are taken in decreasing D[s][.] and each adds its accumulated
+ * gradient to its tree parent.  d_P = NULL recomputes the predecessors from
+ * d_W and d_D, 256 sources at a time (a 256 x n x batch int32 workspace
+ * instead of the n x n x batch predecessor matrix).
+ */
+int fwx_diff_backward(const float* d_W, const float* d_D, const int32_t* d_P, const float* d_G,
+                      float* d_dW, int32_t n, int32_t batch, void* stream);
+
+/* ---------------- profiling (HIP events, no reference counterpart) ----- */
+/*
+ * Between fwx_profile_begin() and fwx_profile_end(), every launch group is
+ * bracketed by HIP events on the stream it runs on.  fwx_profile_end fills
+ * arrays of FWX_PROF_NCLASS entries (any may be NULL): summed event time in
+ * ms, summed algorithmic work (relaxations for REST/LEAD, bytes for
+ * BASELINE: 8 per cell per k), and the number of groups.  The reference
+ * times with cudaEvent pairs around a whole call (src/timer.cu:48-63).
+ */
+#define FWX_PROF_REST 0     /* one k_rest launch per stage */
+#define FWX_PROF_LEAD 1     /* the lead rounds of one stage */
+#define FWX_PROF_BASELINE 2 /* one per-k sweep */
+#define FWX_PROF_NCLASS 3
+int fwx_profile_begin(void);
+int fwx_profile_end(double* ms, double* work, int64_t* groups);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,104 @@
+"""Batched FW: every graph of a batch bit-exact against the oracle, through the
+hipGraph replay, the direct launch sequence and the per-graph loop."""
+import sys
+
+import numpy as np
+import pytest
+
+import fw_cuda_amd as fx
+from oracle import pyoracle as po
+
+pytestmark = pytest.mark.gpu
+INF = fx.INF
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def make_batch(ns, dens, seeds, neg=()):
+    graphs = []
+    for g, (n, p, s) in enumerate(zip(ns, dens, seeds)):
+        G = fx.generate_graph(int(n), float(p), int(s), 1, 1000)
+        if g in neg:  # negative edges, no negative cycle (potential reweighting)
+            h = np.random.default_rng(s).integers(0, 500, int(n))
+            adj = G != INF
+            G = np.where(adj, G.astype(np.int64) + h[:, None] - h[None, :], INF).astype(np.int32)
+            np.fill_diagonal(G, 0)
+        graphs.append(G)
+    return graphs
+
+
+def upload(graphs):
+    import torch
+    sizes = np.array([g.shape[0] for g in graphs], dtype=np.int32)
+    offs, total = fx.batch_layout(sizes)
+    flat = np.full(total, 7, dtype=np.int32)  # gaps hold junk that must survive
+    for g, o in zip(graphs, offs):
+        flat[o:o + g.size] = g.ravel()
+    return sizes, offs, torch.from_numpy(flat).to("cuda:0"), flat
+
+
+def unpack(t, sizes, offs):
+    flat = t.cpu().numpy()
+    return [flat[o:o + int(n) * int(n)].reshape(int(n), int(n)) for n, o in zip(sizes, offs)], flat
+
+
+def check_all(got, expect, flat_after, flat_before, sizes, offs):
+    for g, (a, b) in enumerate(zip(got, expect)):
+        assert (a == b).all(), f"graph {g} (n={a.shape[0]})"
+    # bytes outside the graphs untouched
+    mask = np.ones(flat_before.size, dtype=bool)
+    for n, o in zip(sizes, offs):
+        mask[o:o + int(n) * int(n)] = False
+    assert (flat_after[mask] == flat_before[mask]).all()
+
+
+def test_batch_small_mixed():
+    rng = np.random.default_rng(3)
+    ns = [1, 2, 3, 5, 17, 31, 32, 33, 63, 64, 65, 100, 127, 128, 129, 200, 255, 256, 300, 399, 513, 640]
+    ns = ns + list(rng.integers(1, 400, 18))
+    dens = rng.uniform(0.05, 0.9, len(ns))
+    seeds = list(rng.integers(1, 2**31, len(ns)))
+    graphs = make_batch(ns, dens, seeds, neg={3, 9, 14, 20, 25})
+    expect = [po.fw_rowpar(g) for g in graphs]
+    sizes, offs, t, flat0 = upload(graphs)
+    from fw_cuda_amd import _lib
+    plan = fx.BatchPlan(sizes, offs, _lib.ptr(t))
+    say("plan", plan.stats())
+    for mode in ("graph", "graph-replay", "direct"):
+        t.copy_(__import__("torch").from_numpy(flat0).to("cuda:0"))
+        plan.run(use_graph=(mode != "direct"))
+        got, flat = unpack(t, sizes, offs)
+        say("checking", mode)
+        check_all(got, expect, flat, flat0, sizes, offs)
+    plan.destroy()
+    t.copy_(__import__("torch").from_numpy(flat0).to("cuda:0"))
+    fx.batch_loop(sizes, offs, _lib.ptr(t))
+    got, flat = unpack(t, sizes, offs)
+    check_all(got, expect, flat, flat0, sizes, offs)
+
+
+@pytest.mark.slow
+def test_batch_config4():
+    """512 graphs, n log-uniform in [64, 2048] (seed 11), density U[0.05, 0.9],
+    weights U[1, 1000]: one hipGraph replay, every graph bit-exact."""
+    import torch
+    from fw_cuda_amd import _lib
+    ns, dens, seeds = fx.batch_config4()
+    graphs = make_batch(ns, dens, seeds)
+    sizes, offs, t, flat0 = upload(graphs)
+    plan = fx.BatchPlan(sizes, offs, _lib.ptr(t))
+    plan.run(True)
+    got, flat = unpack(t, sizes, offs)
+    say("config4 gpu done; oracle", plan.stats())
+    for g in range(len(graphs)):
+        assert (got[g] == po.fw_rowpar(graphs[g])).all(), g
+    # a second replay on restored input gives the same matrices
+    t.copy_(torch.from_numpy(flat0).to("cuda:0"))
+    plan.run(True)
+    got2, _ = unpack(t, sizes, offs)
+    for g in range(len(graphs)):
+        assert (got2[g] == got[g]).all(), g
+    plan.destroy()

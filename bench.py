@@ -116,15 +116,16 @@ def main():
         step()
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    ms = (ctypes.c_double * 3)()
-    wk = (ctypes.c_double * 3)()
-    grp = (ctypes.c_int64 * 3)()
+    ms = (ctypes.c_double * _lib.PROF_NCLASS)()
+    wk = (ctypes.c_double * _lib.PROF_NCLASS)()
+    grp = (ctypes.c_int64 * _lib.PROF_NCLASS)()
     _lib.check(L.fwx_profile_end(ms, wk, grp))
 
     ms_per_step = (t1 - t0) * 1e3 / a.steps
     value = float(n) ** 3 / (ms_per_step * 1e-3)
     rest_ms, rest_work, rest_launches = ms[_lib.PROF_REST], wk[_lib.PROF_REST], grp[_lib.PROF_REST]
     lead_ms = ms[_lib.PROF_LEAD]
+    band_ms = ms[_lib.PROF_BAND]
     achieved = rest_work / (rest_ms * 1e-3) if rest_ms > 0 else 0.0
 
     # spot check: 8 rows against Dijkstra on the host (bit-exact)
@@ -173,7 +174,8 @@ def main():
                      "launches": int(rest_launches), "avg_launch_ms": rest_ms / max(1, rest_launches),
                      "work_per_step": rest_work / a.steps,
                      "share_of_step": rest_ms / a.steps / ms_per_step,
-                     "lead_ms_per_step": lead_ms / a.steps},
+                     "lead_ms_per_step": lead_ms / a.steps,
+                     "band_ms_per_step": band_ms / a.steps},
         "parity": check,
         "comparators": compare,
     }

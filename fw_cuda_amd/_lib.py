@@ -67,7 +67,7 @@ SIGNATURES = {
     "fwx_profile_end": (ctypes.c_int, [_vp, _vp, _vp]),
 }
 
-PROF_REST, PROF_LEAD, PROF_BASELINE, PROF_NCLASS = 0, 1, 2, 3
+PROF_REST, PROF_LEAD, PROF_BASELINE, PROF_BAND, PROF_NCLASS = 0, 1, 2, 3, 4
 
 _lib = None
 

@@ -49,50 +49,7 @@ template <int T>
 __global__ __launch_bounds__(256) void k_batch_diag(int32_t* __restrict__ data,
                                                     const DiagItem* __restrict__ items) {
   const DiagItem it = items[blockIdx.x];
-  constexpr int TM = T / 16;
-  __shared__ int32_t rowbuf[2][T];
-  __shared__ int32_t colbuf[2][T];
-  const int tx = threadIdx.x & 15;
-  const int ty = threadIdx.x >> 4;
-  const View V = make_view(data + it.base, it.n, it.c0, it.c0);
-  int32_t acc[TM][TM];
-#pragma unroll
-  for (int r = 0; r < TM; ++r)
-#pragma unroll
-    for (int q = 0; q < TM; ++q)
-      acc[r][q] = V.in(ty + 16 * r, tx + 16 * q) ? *V.at(ty + 16 * r, tx + 16 * q) : kInf;
-  const int kmax = min(T, it.n - it.c0);
-#pragma unroll
-  for (int kk = 0; kk < TM; ++kk) {
-    for (int k16 = 0; k16 < 16; ++k16) {
-      const int k = kk * 16 + k16;
-      if (k >= kmax) break;
-      const int b = k & 1;
-      if (ty == k16) {
-#pragma unroll
-        for (int q = 0; q < TM; ++q) rowbuf[b][tx + 16 * q] = acc[kk][q];
-      }
-      if (tx == k16) {
-#pragma unroll
-        for (int r = 0; r < TM; ++r) colbuf[b][ty + 16 * r] = acc[r][kk];
-      }
-      __syncthreads();
-      int32_t rk[TM], ck[TM];
-#pragma unroll
-      for (int q = 0; q < TM; ++q) rk[q] = rowbuf[b][tx + 16 * q];
-#pragma unroll
-      for (int r = 0; r < TM; ++r) ck[r] = colbuf[b][ty + 16 * r];
-#pragma unroll
-      for (int r = 0; r < TM; ++r)
-#pragma unroll
-        for (int q = 0; q < TM; ++q) acc[r][q] = min(acc[r][q], wadd(ck[r], rk[q]));
-    }
-  }
-#pragma unroll
-  for (int r = 0; r < TM; ++r)
-#pragma unroll
-    for (int q = 0; q < TM; ++q)
-      if (V.in(ty + 16 * r, tx + 16 * q)) *V.at(ty + 16 * r, tx + 16 * q) = acc[r][q];
+  close_tile<T>(make_view(data + it.base, it.n, it.c0, it.c0));
 }
 
 template <int R, bool VEC>

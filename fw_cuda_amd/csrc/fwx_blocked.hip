@@ -30,6 +30,9 @@
 // algorithm, which is what tiling_gpu runs.
 #include <hip/hip_runtime.h>
 
+#include <cstdlib>
+#include <mutex>
+
 #include "fwx_device.hpp"
 #include "fwx_internal.hpp"
 #include "fwx_tile.hpp"
@@ -39,60 +42,10 @@ namespace fwx {
 constexpr int kChunk = 32;
 
 // ---------------------------------------------------------------------------
-// diag: close tile (c, c).  Thread (tx, ty) keeps cells (ty+16r, tx+16q) in
-// VGPRs.  Step k: the owners of row k and column k publish them to a
-// double-buffered LDS line, one barrier, every thread relaxes its cells.
-// Double buffering makes one barrier per step enough: buffer k&1 is next
-// rewritten at step k+2, after every thread has passed barrier k+1, hence
-// finished reading it.  Steps with k >= n (padding) are skipped, so padded
-// vertices are never intermediates.
+// diag: close tile (c, c).
 template <int T>
 __global__ __launch_bounds__(256) void k_diag(int32_t* __restrict__ D, int32_t n, int32_t c) {
-  constexpr int TM = T / 16;
-  __shared__ int32_t rowbuf[2][T];
-  __shared__ int32_t colbuf[2][T];
-  const int tx = threadIdx.x & 15;
-  const int ty = threadIdx.x >> 4;
-  const int32_t base = c * T;
-  const View V = make_view(D, n, base, base);
-  int32_t acc[TM][TM];
-#pragma unroll
-  for (int r = 0; r < TM; ++r)
-#pragma unroll
-    for (int q = 0; q < TM; ++q)
-      acc[r][q] = V.in(ty + 16 * r, tx + 16 * q) ? *V.at(ty + 16 * r, tx + 16 * q) : kInf;
-  const int kmax = min(T, n - base);
-#pragma unroll
-  for (int kk = 0; kk < TM; ++kk) {
-    for (int k16 = 0; k16 < 16; ++k16) {
-      const int k = kk * 16 + k16;
-      if (k >= kmax) break;
-      const int b = k & 1;
-      if (ty == k16) {
-#pragma unroll
-        for (int q = 0; q < TM; ++q) rowbuf[b][tx + 16 * q] = acc[kk][q];
-      }
-      if (tx == k16) {
-#pragma unroll
-        for (int r = 0; r < TM; ++r) colbuf[b][ty + 16 * r] = acc[r][kk];
-      }
-      __syncthreads();
-      int32_t rk[TM], ck[TM];
-#pragma unroll
-      for (int q = 0; q < TM; ++q) rk[q] = rowbuf[b][tx + 16 * q];
-#pragma unroll
-      for (int r = 0; r < TM; ++r) ck[r] = colbuf[b][ty + 16 * r];
-#pragma unroll
-      for (int r = 0; r < TM; ++r)
-#pragma unroll
-        for (int q = 0; q < TM; ++q) acc[r][q] = min(acc[r][q], wadd(ck[r], rk[q]));
-    }
-  }
-#pragma unroll
-  for (int r = 0; r < TM; ++r)
-#pragma unroll
-    for (int q = 0; q < TM; ++q)
-      if (V.in(ty + 16 * r, tx + 16 * q)) *V.at(ty + 16 * r, tx + 16 * q) = acc[r][q];
+  close_tile<T>(make_view(D, n, c * T, c * T));
 }
 
 // ---------------------------------------------------------------------------
@@ -200,6 +153,40 @@ __global__ __launch_bounds__(256) void k_rest(int32_t* __restrict__ D, int32_t n
 }
 
 // ---------------------------------------------------------------------------
+// rest_band: the part of stage s's rest pass that the next stage's lead rounds
+// read -- R x R tiles outside stage s's band [e0, e0+en) with a row or a
+// column in the next stage's band [f0, f0+fn) (R-tile units, f0 = e0 + en).
+// m1 = tiles per side outside band e.  Region 1: band-f rows x every non-e
+// column (fn * m1 tiles); region 2: rows outside both bands x band-f columns.
+template <int R, bool VEC>
+__global__ __launch_bounds__(256) void k_rest_band(int32_t* __restrict__ D, int32_t n, int32_t s0,
+                                                   int32_t S, int32_t e0, int32_t en, int32_t f0,
+                                                   int32_t fn, int32_t m1) {
+  using P = TileProduct<R, R, kChunk, VEC>;
+  __shared__ typename P::Smem s;
+  int b = blockIdx.x;
+  int I, J;
+  if (b < fn * m1) {
+    I = f0 + b / m1;
+    const int Jr = b % m1;
+    J = Jr + (Jr >= e0 ? en : 0);
+  } else {
+    b -= fn * m1;
+    const int Ir = b / fn;  // among rows outside both bands
+    I = Ir + (Ir >= e0 ? en + fn : 0);
+    J = f0 + b % fn;
+  }
+  const View A = make_view(D, n, I * R, s0);
+  const View B = make_view(D, n, s0, J * R);
+  const View C = make_view(D, n, I * R, J * R);
+  int32_t acc[P::TM][P::TN];
+  P::load_c(C, acc);
+  P p;
+  p.run(s, A, B, S, acc);
+  P::store_c(C, acc);
+}
+
+// ---------------------------------------------------------------------------
 // host driver
 
 template <int T, bool VEC>
@@ -221,11 +208,13 @@ static int lead_round(int32_t* d, int32_t n, int32_t c, int32_t kb, int32_t B, i
   return FWX_OK;
 }
 
+// Rest tiles outside the R-tile band [s0/R, s0/R + excl): excl covers stage
+// s alone (plain schedule) or stage s and the next stage (look-ahead).
 template <int R, bool VEC>
-static int rest_pass(int32_t* d, int32_t n, int32_t s0, int32_t S, hipStream_t st) {
+static int rest_pass(int32_t* d, int32_t n, int32_t s0, int32_t S, int32_t excl, hipStream_t st) {
   const int nr = (n + R - 1) / R;
   const int sr0 = s0 / R;
-  const int srn = min((S + R - 1) / R, nr - sr0);
+  const int srn = min(excl, nr - sr0);
   const int m = nr - srn;
   if (m <= 0) return FWX_OK;
   const int64_t total = (int64_t)m * m;
@@ -235,6 +224,49 @@ static int rest_pass(int32_t* d, int32_t n, int32_t s0, int32_t S, hipStream_t s
   return FWX_OK;
 }
 
+template <int R, bool VEC>
+static int band_pass(int32_t* d, int32_t n, int32_t s0, int32_t S, int32_t en, int32_t fn_tiles,
+                     hipStream_t st) {
+  const int nr = (n + R - 1) / R;
+  const int e0 = s0 / R;
+  const int f0 = e0 + en;
+  const int fn = min(fn_tiles, nr - f0);
+  const int m1 = nr - en;
+  if (fn <= 0 || m1 <= 0) return FWX_OK;
+  const int cnt = fn * m1 + (m1 - fn) * fn;
+  if (cnt <= 0) return FWX_OK;
+  k_rest_band<R, VEC><<<cnt, 256, 0, st>>>(d, n, s0, S, e0, en, f0, fn, m1);
+  FWX_LAUNCHED("k_rest_band");
+  return FWX_OK;
+}
+
+template <bool VEC>
+static int rest_dispatch(int R, int32_t* d, int32_t n, int32_t s0, int32_t S, int32_t excl, hipStream_t st) {
+  if (R == 128) return rest_pass<128, VEC>(d, n, s0, S, excl, st);
+  if (R == 64) return rest_pass<64, VEC>(d, n, s0, S, excl, st);
+  return rest_pass<32, VEC>(d, n, s0, S, excl, st);
+}
+
+template <bool VEC>
+static int band_dispatch(int R, int32_t* d, int32_t n, int32_t s0, int32_t S, int32_t en, int32_t fn,
+                         hipStream_t st) {
+  if (R == 128) return band_pass<128, VEC>(d, n, s0, S, en, fn, st);
+  if (R == 64) return band_pass<64, VEC>(d, n, s0, S, en, fn, st);
+  return band_pass<32, VEC>(d, n, s0, S, en, fn, st);
+}
+
+// The lead rounds of one stage, bracketed for the profiler.
+template <int T, bool VEC>
+static int lead_stage(int32_t* d, int32_t n, int32_t kb, int32_t B, int32_t N, hipStream_t st) {
+  const double L = (double)min(B * T, n - kb * T), nn = (double)n;
+  hipEvent_t ev;
+  if (int rc = prof_open(st, &ev)) return rc;
+  for (int l = 0; l < B; ++l)
+    if (int rc = lead_round<T, VEC>(d, n, kb + l, kb, B, N, st)) return rc;
+  return prof_close(st, ev, FWX_PROF_LEAD, (nn * nn - (nn - L) * (nn - L)) * L);
+}
+
+// Plain schedule: lead(s), rest(s), lead(s+1), ... on one stream.
 template <int T, bool VEC>
 static int run_blocked(int32_t* d, int32_t n, int32_t kappa, hipStream_t st) {
   const int N = (n + T - 1) / T;
@@ -243,38 +275,112 @@ static int run_blocked(int32_t* d, int32_t n, int32_t kappa, hipStream_t st) {
   for (int kb = 0; kb < N; kb += kappa) {
     const int B = min(kappa, N - kb);
     const int s0 = kb * T, S = B * T;
-    // algorithmic relaxations: every cell x the stage's real k values
     const double L = (double)min(S, n - s0), nn = (double)n;
+    if (int rc = lead_stage<T, VEC>(d, n, kb, B, N, st)) return rc;
     hipEvent_t ev;
     if (int rc = prof_open(st, &ev)) return rc;
-    for (int l = 0; l < B; ++l) {
-      const int rc = lead_round<T, VEC>(d, n, kb + l, kb, B, N, st);
-      if (rc) return rc;
-    }
-    if (int rc = prof_close(st, ev, FWX_PROF_LEAD, (nn * nn - (nn - L) * (nn - L)) * L)) return rc;
-    if (int rc = prof_open(st, &ev)) return rc;
-    int rc;
-    if (R == 128)
-      rc = rest_pass<128, VEC>(d, n, s0, S, st);
-    else if (R == 64)
-      rc = rest_pass<64, VEC>(d, n, s0, S, st);
-    else
-      rc = rest_pass<32, VEC>(d, n, s0, S, st);
-    if (rc) return rc;
-    if (int rc2 = prof_close(st, ev, FWX_PROF_REST, (nn - L) * (nn - L) * L)) return rc2;
+    if (int rc = rest_dispatch<VEC>(R, d, n, s0, S, (S + R - 1) / R, st)) return rc;
+    if (int rc = prof_close(st, ev, FWX_PROF_REST, (nn - L) * (nn - L) * L)) return rc;
   }
   return FWX_OK;
+}
+
+// Look-ahead schedule.  Stage s+1's lead rounds only read and write tiles
+// with a row or column in band s+1; stage s's rest pass is split into the
+// tiles of that cross (rest_band, first, on the caller's stream) and the
+// others (after it).  A side stream runs lead(s+1) as soon as rest_band(s)
+// is done, concurrently with the other rest tiles of stage s, which touch
+// neither band s+1 nor anything lead(s+1) writes (lead(s+1) writes only band
+// s+1 rows/columns; rest(s) reads only band s rows/columns).  The caller's
+// stream waits for lead(s+1) before stage s+1's rest.
+template <int T, bool VEC>
+static int run_blocked_lookahead(int32_t* d, int32_t n, int32_t kappa, hipStream_t st, hipStream_t side,
+                                 hipEvent_t evA, hipEvent_t evB) {
+  const int N = (n + T - 1) / T;
+  const int span = kappa * T;
+  const int R = (span % 128 == 0) ? 128 : (span % 64 == 0) ? 64 : 32;
+  const double nn = (double)n;
+  if (int rc = lead_stage<T, VEC>(d, n, 0, min(kappa, N), N, st)) return rc;
+  for (int kb = 0; kb < N; kb += kappa) {
+    const int B = min(kappa, N - kb);
+    const int s0 = kb * T, S = B * T;
+    const double L = (double)min(S, n - s0);
+    const int en = (S + R - 1) / R;
+    const int kb2 = kb + kappa;
+    hipEvent_t ev;
+    if (kb2 < N) {
+      const int B2 = min(kappa, N - kb2);
+      const int fn = (B2 * T + R - 1) / R;
+      const double L2 = (double)min(B2 * T, n - kb2 * T);
+      if (int rc = prof_open(st, &ev)) return rc;
+      if (int rc = band_dispatch<VEC>(R, d, n, s0, S, en, fn, st)) return rc;
+      if (int rc = prof_close(st, ev, FWX_PROF_BAND, ((nn - L) * (nn - L) - (nn - L - L2) * (nn - L - L2)) * L))
+        return rc;
+      FWX_HIP(hipEventRecord(evA, st));
+      FWX_HIP(hipStreamWaitEvent(side, evA, 0));
+      if (int rc = lead_stage<T, VEC>(d, n, kb2, B2, N, side)) return rc;
+      FWX_HIP(hipEventRecord(evB, side));
+      if (int rc = prof_open(st, &ev)) return rc;
+      if (int rc = rest_dispatch<VEC>(R, d, n, s0, S, en + fn, st)) return rc;
+      if (int rc = prof_close(st, ev, FWX_PROF_REST, (nn - L - L2) * (nn - L - L2) * L)) return rc;
+      FWX_HIP(hipStreamWaitEvent(st, evB, 0));
+    } else {
+      if (int rc = prof_open(st, &ev)) return rc;
+      if (int rc = rest_dispatch<VEC>(R, d, n, s0, S, en, st)) return rc;
+      if (int rc = prof_close(st, ev, FWX_PROF_REST, (nn - L) * (nn - L) * L)) return rc;
+    }
+  }
+  return FWX_OK;
+}
+
+// One non-blocking side stream per device (default priority), created on
+// first use and kept for the life of the process.
+static int side_stream(hipStream_t* out) {
+  static std::mutex mu;
+  static hipStream_t streams[64] = {};
+  int dev = 0;
+  FWX_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 64) return fail(FWX_ERR_UNSUPPORTED, "device index above 63");
+  std::lock_guard<std::mutex> lk(mu);
+  if (!streams[dev]) FWX_HIP(hipStreamCreateWithFlags(&streams[dev], hipStreamNonBlocking));
+  *out = streams[dev];
+  return FWX_OK;
+}
+
+// FWX_LOOKAHEAD=0 selects the single-stream schedule (read on every call).
+static bool lookahead_enabled() {
+  const char* e = std::getenv("FWX_LOOKAHEAD");
+  return !(e && e[0] == '0');
+}
+
+template <int T, bool VEC>
+static int run_any(int32_t* d, int32_t n, int32_t kappa, hipStream_t st) {
+  const int N = (n + T - 1) / T;
+  if (!lookahead_enabled() || N <= kappa) return run_blocked<T, VEC>(d, n, kappa, st);
+  hipStream_t side;
+  if (int rc = side_stream(&side)) return rc;
+  hipEvent_t evA, evB;
+  FWX_HIP(hipEventCreateWithFlags(&evA, hipEventDisableTiming));
+  hipError_t e = hipEventCreateWithFlags(&evB, hipEventDisableTiming);
+  if (e != hipSuccess) {
+    (void)hipEventDestroy(evA);
+    return hip_fail(e, "hipEventCreateWithFlags", __FILE__, __LINE__);
+  }
+  const int rc = run_blocked_lookahead<T, VEC>(d, n, kappa, st, side, evA, evB);
+  (void)hipEventDestroy(evA);
+  (void)hipEventDestroy(evB);
+  return rc;
 }
 
 int enqueue_blocked(int32_t* d, int32_t n, int32_t tile, int32_t kappa, hipStream_t st) {
   const bool vec = (n % 4 == 0) && ((reinterpret_cast<uintptr_t>(d) & 15u) == 0);
   switch (tile) {
     case 32:
-      return vec ? run_blocked<32, true>(d, n, kappa, st) : run_blocked<32, false>(d, n, kappa, st);
+      return vec ? run_any<32, true>(d, n, kappa, st) : run_any<32, false>(d, n, kappa, st);
     case 64:
-      return vec ? run_blocked<64, true>(d, n, kappa, st) : run_blocked<64, false>(d, n, kappa, st);
+      return vec ? run_any<64, true>(d, n, kappa, st) : run_any<64, false>(d, n, kappa, st);
     case 128:
-      return vec ? run_blocked<128, true>(d, n, kappa, st) : run_blocked<128, false>(d, n, kappa, st);
+      return vec ? run_any<128, true>(d, n, kappa, st) : run_any<128, false>(d, n, kappa, st);
     default:
       return fail(FWX_ERR_INVALID_ARG, "tile must be 32, 64 or 128");
   }

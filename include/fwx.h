@@ -169,7 +169,8 @@ int fwx_diff_backward(const float* d_W, const float* d_D, const int32_t* d_P, co
 #define FWX_PROF_REST 0     /* one k_rest launch per stage */
 #define FWX_PROF_LEAD 1     /* the lead rounds of one stage */
 #define FWX_PROF_BASELINE 2 /* one per-k sweep */
-#define FWX_PROF_NCLASS 3
+#define FWX_PROF_BAND 3     /* look-ahead: the rest tiles the next stage's lead reads */
+#define FWX_PROF_NCLASS 4
 int fwx_profile_begin(void);
 int fwx_profile_end(double* ms, double* work, int64_t* groups);
 

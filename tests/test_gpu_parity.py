@@ -129,6 +129,17 @@ def test_large_values_take_exact_path():
         assert (gpu(g, algo, tile, kappa) == expect).all(), (algo, tile, kappa)
 
 
+@pytest.mark.parametrize("lookahead", ["0", "1"])
+def test_schedules_agree(monkeypatch, lookahead):
+    """The single-stream and the look-ahead (two-stream) schedules."""
+    monkeypatch.setenv("FWX_LOOKAHEAD", lookahead)
+    rng = np.random.default_rng(8)
+    for n, t, k in [(1000, 64, 2), (777, 64, 3), (1537, 128, 2), (900, 32, 4), (2048, 64, 4)]:
+        g = fx.generate_graph(n, float(rng.uniform(0.1, 0.9)), int(rng.integers(1, 2**31)), 1, 1000)
+        say("schedule", lookahead, n, t, k)
+        assert (gpu(g, "multi_layer_tiling_gpu", t, k) == po.fw_rowpar(g)).all(), (n, t, k)
+
+
 def test_config1_n1024():
     """n=1024, density 0.9, weights U[1,1000], seed 42: every algorithm."""
     g = fx.generate_graph(1024, 0.9, 42, 1, 1000)

@@ -136,3 +136,36 @@ def test_tests_runner_cpu_suite():
     assert r.returncode == 1 and "Unsupported CPU algorithm" in r.stderr
     r = run_cli("tests_runner")
     assert r.returncode == 1 and "Usage:" in r.stderr
+
+
+def test_batched_and_diff_argument_errors():
+    import ctypes
+    L = _lib.lib()
+    dummy = ctypes.c_void_p(64)
+    plan = ctypes.c_void_p()
+    sizes = np.array([4], dtype=np.int32)
+    offs = np.array([0], dtype=np.int64)
+    assert L.fwx_batch_plan_create(ctypes.byref(plan), _lib.ptr(sizes), _lib.ptr(offs), 0, dummy,
+                                   None) == _lib.ERR_INVALID_ARG
+    bad = np.array([0], dtype=np.int32)
+    assert L.fwx_batch_plan_create(ctypes.byref(plan), _lib.ptr(bad), _lib.ptr(offs), 1, dummy,
+                                   None) == _lib.ERR_INVALID_ARG
+    neg = np.array([-4], dtype=np.int64)
+    assert L.fwx_batch_plan_create(ctypes.byref(plan), _lib.ptr(sizes), _lib.ptr(neg), 1, dummy,
+                                   None) == _lib.ERR_INVALID_ARG
+    assert L.fwx_batch_plan_run(None, 1) == _lib.ERR_INVALID_ARG
+    assert L.fwx_batch_loop(_lib.ptr(sizes), _lib.ptr(offs), 0, dummy, None) == _lib.ERR_INVALID_ARG
+    assert L.fwx_diff_forward(dummy, dummy, None, 0, 1, None) == _lib.ERR_INVALID_ARG
+    assert L.fwx_diff_forward(dummy, dummy, None, 8193, 1, None) == _lib.ERR_INVALID_ARG
+    assert L.fwx_diff_forward(dummy, dummy, None, 8, 0, None) == _lib.ERR_INVALID_ARG
+    assert L.fwx_diff_backward(None, dummy, None, dummy, dummy, 8, 1, None) == _lib.ERR_INVALID_ARG
+    assert L.fwx_diff_backward(dummy, dummy, None, dummy, None, 8, 1, None) == _lib.ERR_INVALID_ARG
+
+
+def test_batch_layout_alignment():
+    offs, total = fx.batch_layout([3, 5, 1, 64])
+    assert list(offs) == [0, 12, 40, 44] and total == 44 + 4096
+    assert all(o % 4 == 0 for o in offs)
+    ns, dens, seeds = fx.batch_config4()
+    assert len(ns) == 512 and ns.min() >= 64 and ns.max() <= 2048
+    assert 0.05 <= dens.min() and dens.max() <= 0.9

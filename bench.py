@@ -40,7 +40,82 @@ def parse():
     ap.add_argument("--cpu-n", type=int, default=2048)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-compare", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-diff", action="store_true")
     return ap.parse_args()
+
+
+def bench_batched(steps):
+    """512 graphs, n log-uniform in [64, 2048] (seed 11), density U[0.05, 0.9],
+    weights U[1, 1000]: graphs/s of one hipGraph replay of the merged plan
+    vs. the per-graph launch loop (one plain solve per graph)."""
+    import numpy as np
+    import torch
+    import fw_cuda_amd as fx
+    from fw_cuda_amd import _lib
+    ns, dens, seeds = fx.batch_config4()
+    offs, total = fx.batch_layout(ns)
+    flat = np.zeros(total, dtype=np.int32)
+    for n, p, sd, o in zip(ns, dens, seeds, offs):
+        flat[o:o + int(n) * int(n)] = fx.generate_graph(int(n), float(p), int(sd), 1, 1000).ravel()
+    src = torch.from_numpy(flat).cuda()
+    work = torch.empty_like(src)
+    plan = fx.BatchPlan(ns, offs, _lib.ptr(work))
+    work.copy_(src)
+    plan.run(True)  # capture + first replay
+    torch.cuda.synchronize()
+    t = 0.0
+    for _ in range(steps):
+        work.copy_(src)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        plan.run(True)
+        t += time.perf_counter() - t0
+    graph_s = t / steps
+    once = work.clone()
+    work.copy_(src)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    fx.batch_loop(ns, offs, _lib.ptr(work))
+    loop_s = time.perf_counter() - t0
+    same = bool(torch.equal(once, work))
+    st = plan.stats()
+    plan.destroy()
+    relax = float((ns.astype(np.float64) ** 3).sum())
+    return {"graphs": int(len(ns)), "graphs_per_s": len(ns) / graph_s, "ms_per_batch": graph_s * 1e3,
+            "loop_graphs_per_s": len(ns) / loop_s, "loop_ms_per_batch": loop_s * 1e3,
+            "speedup_vs_loop": loop_s / graph_s, "n3_per_s": relax / graph_s,
+            "plan": st, "replay_equals_loop": same}
+
+
+def bench_diff(steps):
+    """Differentiable fp32 FW, weights U[0.5, 10], density 0.3, seed 3:
+    forward + backward of L = sum(G * D), batch of 64 graphs at n=512 and one
+    graph at n=4096."""
+    import numpy as np
+    import torch
+    from fw_cuda_amd.autograd import floyd_warshall_diff
+    out = {}
+    for name, (b, n) in {"b64_n512": (64, 512), "b1_n4096": (1, 4096)}.items():
+        rng = np.random.default_rng(3)
+        W = np.where(rng.random((b, n, n)) < 0.3, rng.uniform(0.5, 10.0, (b, n, n)), np.inf).astype(np.float32)
+        G = torch.from_numpy(rng.standard_normal((b, n, n)).astype(np.float32)).cuda()
+        Wt = torch.from_numpy(W).cuda()
+
+        def once():
+            w = Wt.clone().requires_grad_(True)
+            D = floyd_warshall_diff(w)
+            (D * G).sum().backward()
+            return w.grad
+
+        once()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            once()
+        torch.cuda.synchronize()
+        out[name] = {"fwd_bwd_ms": (time.perf_counter() - t0) * 1e3 / steps}
+    return out
 
 
 def cpu_baseline(n):
@@ -179,6 +254,10 @@ def main():
         "parity": check,
         "comparators": compare,
     }
+    if not a.no_batched:
+        out["batched"] = bench_batched(3)
+    if not a.no_diff:
+        out["differentiable"] = bench_diff(3)
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(a.cpu_n)
     print(json.dumps(out), flush=True)

@@ -72,10 +72,22 @@ PROF_REST, PROF_LEAD, PROF_BASELINE, PROF_BAND, PROF_NCLASS = 0, 1, 2, 3, 4
 _lib = None
 
 
+def _share_torch_runtime():
+    """A process must hold one HIP runtime.  PyTorch-ROCm wheels carry their
+    own libamdhip64; loading it before libfwx makes libfwx bind to that same
+    runtime instead of starting a second one from /opt/rocm (two runtimes in
+    one process do not share the device)."""
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+
+
 def lib():
     """Load libfwx.so once (raises OSError if it is not built)."""
     global _lib
     if _lib is None:
+        _share_torch_runtime()
         if not os.path.exists(LIB_PATH):
             raise OSError(f"{LIB_PATH} is not built: run `make -C fw_cuda_amd` "
                           "or __graft_entry__.build()")

@@ -52,10 +52,10 @@ __global__ __launch_bounds__(256) void k_batch_diag(int32_t* __restrict__ data,
   close_tile<T>(make_view(data + it.base, it.n, it.c0, it.c0));
 }
 
-template <int R, bool VEC>
+template <int R, bool VEC, bool PK>
 __device__ __forceinline__ void batch_prod_body(int32_t* data, const ProdItem& it,
-                                                typename TileProduct<R, R, 32, VEC>::Smem& s) {
-  using P = TileProduct<R, R, 32, VEC>;
+                                                typename TileProduct<R, R, 32, VEC, 16, PK>::Smem& s) {
+  using P = TileProduct<R, R, 32, VEC, 16, PK>;
   int32_t* g = data + it.base;
   const View A = make_view(g, it.n, it.ar, it.ac);
   const View B = make_view(g, it.n, it.br, it.bc);
@@ -69,18 +69,18 @@ __device__ __forceinline__ void batch_prod_body(int32_t* data, const ProdItem& i
 
 // One LDS allocation serves both instantiations (their Smem layouts are the
 // same; VEC only changes how global memory is read).
-template <int R>
+template <int R, bool PK>
 __global__ __launch_bounds__(256) void k_batch_prod(int32_t* __restrict__ data,
                                                     const ProdItem* __restrict__ items) {
-  using SV = typename TileProduct<R, R, 32, true>::Smem;
-  using SS = typename TileProduct<R, R, 32, false>::Smem;
+  using SV = typename TileProduct<R, R, 32, true, 16, PK>::Smem;
+  using SS = typename TileProduct<R, R, 32, false, 16, PK>::Smem;
   static_assert(sizeof(SV) == sizeof(SS), "staging layouts must match");
   __shared__ SV s;
   const ProdItem it = items[blockIdx.x];
   if (it.vec)
-    batch_prod_body<R, true>(data, it, s);
+    batch_prod_body<R, true, PK>(data, it, s);
   else
-    batch_prod_body<R, false>(data, it, *reinterpret_cast<SS*>(&s));
+    batch_prod_body<R, false, PK>(data, it, *reinterpret_cast<SS*>(&s));
 }
 
 // ---------------------------------------------------------------------------
@@ -167,23 +167,25 @@ struct fwx_batch_plan {
   std::vector<int32_t> step_of;       // step index of each launch
   fwx::ProdItem* d_prod[fwx::kKinds] = {};
   fwx::DiagItem* d_diag[fwx::kKinds] = {};
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t exec = nullptr;
+  hipGraph_t graph[2] = {};
+  hipGraphExec_t exec[2] = {};  // [0] min3 form, [1] pk64 form
   int64_t items = 0;
   int32_t steps = 0;
+  int64_t span = 0;  // elements from d_data covering every graph
 };
 
 namespace fwx {
 
+template <bool PK>
 static int enqueue_plan(fwx_batch_plan* p) {
   for (const Launch& L : p->launches) {
     switch (L.kind) {
       case kDiag32: k_batch_diag<32><<<L.count, 256, 0, p->stream>>>(p->data, p->d_diag[kDiag32] + L.first); break;
       case kDiag64: k_batch_diag<64><<<L.count, 256, 0, p->stream>>>(p->data, p->d_diag[kDiag64] + L.first); break;
       case kDiag128: k_batch_diag<128><<<L.count, 256, 0, p->stream>>>(p->data, p->d_diag[kDiag128] + L.first); break;
-      case kProd32: k_batch_prod<32><<<L.count, 256, 0, p->stream>>>(p->data, p->d_prod[kProd32] + L.first); break;
-      case kProd64: k_batch_prod<64><<<L.count, 256, 0, p->stream>>>(p->data, p->d_prod[kProd64] + L.first); break;
-      case kProd128: k_batch_prod<128><<<L.count, 256, 0, p->stream>>>(p->data, p->d_prod[kProd128] + L.first); break;
+      case kProd32: k_batch_prod<32, PK><<<L.count, 256, 0, p->stream>>>(p->data, p->d_prod[kProd32] + L.first); break;
+      case kProd64: k_batch_prod<64, PK><<<L.count, 256, 0, p->stream>>>(p->data, p->d_prod[kProd64] + L.first); break;
+      case kProd128: k_batch_prod<128, PK><<<L.count, 256, 0, p->stream>>>(p->data, p->d_prod[kProd128] + L.first); break;
       default: return fail(FWX_ERR_INVALID_ARG, "corrupt batch plan");
     }
     FWX_LAUNCHED("k_batch");
@@ -191,10 +193,14 @@ static int enqueue_plan(fwx_batch_plan* p) {
   return FWX_OK;
 }
 
+static int enqueue_mode(fwx_batch_plan* p, bool pk) { return pk ? enqueue_plan<true>(p) : enqueue_plan<false>(p); }
+
 static void destroy_plan(fwx_batch_plan* p) {
   if (!p) return;
-  if (p->exec) (void)hipGraphExecDestroy(p->exec);
-  if (p->graph) (void)hipGraphDestroy(p->graph);
+  for (int m = 0; m < 2; ++m) {
+    if (p->exec[m]) (void)hipGraphExecDestroy(p->exec[m]);
+    if (p->graph[m]) (void)hipGraphDestroy(p->graph[m]);
+  }
   for (int k = 0; k < kKinds; ++k) {
     if (p->d_prod[k]) (void)hipFree(p->d_prod[k]);
     if (p->d_diag[k]) (void)hipFree(p->d_diag[k]);
@@ -239,6 +245,8 @@ int fwx_batch_plan_create(fwx_batch_plan** out, const int32_t* sizes, const int6
 
   fwx_batch_plan* p = new fwx_batch_plan();
   p->data = d_data;
+  for (int32_t g = 0; g < count; ++g)
+    p->span = std::max<int64_t>(p->span, offsets[g] + (int64_t)sizes[g] * sizes[g]);
   p->stream = as_stream(stream);
   p->steps = steps;
   std::vector<ProdItem> prod[kKinds];
@@ -293,8 +301,12 @@ int fwx_batch_plan_create(fwx_batch_plan** out, const int32_t* sizes, const int6
 int fwx_batch_plan_run(fwx_batch_plan* p, int32_t use_graph) {
   clear_error();
   if (!p) return fail(FWX_ERR_INVALID_ARG, "null plan");
+  // one range check over the buffer span selects the relaxation form
+  bool pk = false;
+  if (int rc = values_in_pk64_range(p->data, p->span, p->stream, &pk)) return rc;
+  const int m = pk ? 1 : 0;
   if (use_graph) {
-    if (!p->exec) {
+    if (!p->exec[m]) {
       // a private capture stream: the caller's stream may be the legacy
       // default stream, which cannot be captured
       hipStream_t cap;
@@ -303,7 +315,7 @@ int fwx_batch_plan_run(fwx_batch_plan* p, int32_t use_graph) {
       p->stream = cap;
       hipError_t e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
       int rc = FWX_OK;
-      if (e == hipSuccess) rc = enqueue_plan(p);
+      if (e == hipSuccess) rc = enqueue_mode(p, pk);
       hipGraph_t g = nullptr;
       hipError_t e2 = hipStreamEndCapture(cap, &g);
       p->stream = saved;
@@ -314,12 +326,12 @@ int fwx_batch_plan_run(fwx_batch_plan* p, int32_t use_graph) {
         return rc;
       }
       if (e2 != hipSuccess) return hip_fail(e2, "hipStreamEndCapture", __FILE__, __LINE__);
-      p->graph = g;
-      FWX_HIP(hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0));
+      p->graph[m] = g;
+      FWX_HIP(hipGraphInstantiate(&p->exec[m], p->graph[m], nullptr, nullptr, 0));
     }
-    FWX_HIP(hipGraphLaunch(p->exec, p->stream));
+    FWX_HIP(hipGraphLaunch(p->exec[m], p->stream));
   } else {
-    if (int rc = enqueue_plan(p)) return rc;
+    if (int rc = enqueue_mode(p, pk)) return rc;
   }
   FWX_HIP(hipStreamSynchronize(p->stream));
   return FWX_OK;

@@ -52,10 +52,49 @@ __global__ __launch_bounds__(256) void k_diag(int32_t* __restrict__ D, int32_t n
 // panel: blocks [0, N-1) take row tile (c, j), blocks [N-1, 2N-2) column tile
 // (i, c).  The tile's own pre-update values are the second operand; it is
 // staged chunk by chunk before this block stores anything.
-template <int T, bool VEC>
+template <int T, bool VEC, bool PK>
 __global__ __launch_bounds__(256) void k_panel(int32_t* __restrict__ D, int32_t n, int32_t c,
                                                int32_t N) {
-  using P = TileProduct<T, T, kChunk, VEC>;
+  using P = TileProduct<T, T, kChunk, VEC, 16, PK>;
+  __shared__ typename P::Smem s;
+  const int b = blockIdx.x;
+  const int32_t base = c * T;
+  const View diag = make_view(D, n, base, base);
+  View A, B, C;
+  if (b < N - 1) {
+    const int j = b + (b >= c);
+    C = make_view(D, n, base, j * T);
+    A = diag;
+    B = C;
+  } else {
+    int i = b - (N - 1);
+    i += (i >= c);
+    C = make_view(D, n, i * T, base);
+    A = C;
+    B = diag;
+  }
+  int32_t acc[P::TM][P::TN];
+  P::load_c(C, acc);
+  P p;
+  p.run(s, A, B, T, acc);
+  P::store_c(C, acc);
+}
+
+// ---------------------------------------------------------------------------
+// dpanel: diag and panel fused into one launch.  Every block first closes the
+// diagonal tile itself and stores it (all blocks store the same bits), then
+// updates its row/column tile as k_panel does, reading the closed tile back.
+// A block may start from a partly overwritten diagonal tile; every cell then
+// lies between the closed value and the original one, and closing such a
+// tile yields the same closed tile, so the stored bits never change.
+// Saves one dependent launch per k-tile round at the price of 2(N-1)
+// redundant T^3 closures (negligible next to the round's products).
+template <int T, bool VEC, bool PK>
+__global__ __launch_bounds__(256) void k_dpanel(int32_t* __restrict__ D, int32_t n, int32_t c,
+                                                int32_t N) {
+  close_tile<T>(make_view(D, n, c * T, c * T));
+  __syncthreads();
+  using P = TileProduct<T, T, kChunk, VEC, 16, PK>;
   __shared__ typename P::Smem s;
   const int b = blockIdx.x;
   const int32_t base = c * T;
@@ -85,10 +124,10 @@ __global__ __launch_bounds__(256) void k_panel(int32_t* __restrict__ D, int32_t 
 // column c.  Region 1: stage rows (except c) x all columns (except c),
 // (B-1)(N-1) tiles.  Region 2: non-stage rows x stage columns (except c),
 // (N-B)(B-1) tiles.
-template <int T, bool VEC>
+template <int T, bool VEC, bool PK>
 __global__ __launch_bounds__(256) void k_cross(int32_t* __restrict__ D, int32_t n, int32_t c,
                                                int32_t kb, int32_t Bst, int32_t N) {
-  using P = TileProduct<T, T, kChunk, VEC>;
+  using P = TileProduct<T, T, kChunk, VEC, 16, PK>;
   __shared__ typename P::Smem s;
   int b = blockIdx.x;
   const int l = c - kb;
@@ -124,11 +163,41 @@ __global__ __launch_bounds__(256) void k_cross(int32_t* __restrict__ D, int32_t 
 // XCD (workgroups are dealt to the 8 XCDs round-robin), and each range is
 // walked in 8-row groups so concurrently resident tiles share A panels
 // (same rows) and B panels (nearby columns) in that XCD's L2.
-template <int R, bool VEC>
-__global__ __launch_bounds__(256) void k_rest(int32_t* __restrict__ D, int32_t n, int32_t s0,
-                                              int32_t S, int32_t sr0, int32_t srn, int32_t m,
-                                              int32_t per) {
-  using P = TileProduct<R, R, kChunk, VEC>;
+template <int R, bool VEC, bool PK, int KC = kChunk>
+__global__ __launch_bounds__(256, KC == 16 ? 3 : 2) void k_rest(int32_t* __restrict__ D, int32_t n,
+                                                                        int32_t s0, int32_t S, int32_t sr0,
+                                                                        int32_t srn, int32_t m, int32_t per) {
+  using P = TileProduct<R, R, KC, VEC, 16, PK>;
+  __shared__ typename P::Smem s;
+  const int b = blockIdx.x;
+  const int L = (b & 7) * per + (b >> 3);
+  if (L >= m * m) return;
+  constexpr int G = 8;
+  const int group = L / (G * m);
+  const int first = group * G;
+  const int rows = min(m - first, G);
+  const int within = L - group * G * m;
+  const int Ii = first + within % rows;
+  const int Jj = within / rows;
+  const int I = Ii + (Ii >= sr0 ? srn : 0);
+  const int J = Jj + (Jj >= sr0 ? srn : 0);
+  const View A = make_view(D, n, I * R, s0);
+  const View B = make_view(D, n, s0, J * R);
+  const View C = make_view(D, n, I * R, J * R);
+  int32_t acc[P::TM][P::TN];
+  P::load_c(C, acc);
+  P p;
+  p.run(s, A, B, S, acc);
+  P::store_c(C, acc);
+}
+
+// 512-thread shape, held to 128 VGPRs so that two blocks (4 waves/SIMD)
+// share a CU.
+template <int R, bool VEC, bool PK>
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_rest512(
+    int32_t* __restrict__ D, int32_t n, int32_t s0, int32_t S, int32_t sr0, int32_t srn, int32_t m,
+    int32_t per) {
+  using P = TileProduct<R, R, kChunk, VEC, 32, PK>;
   __shared__ typename P::Smem s;
   const int b = blockIdx.x;
   const int L = (b & 7) * per + (b >> 3);
@@ -158,11 +227,11 @@ __global__ __launch_bounds__(256) void k_rest(int32_t* __restrict__ D, int32_t n
 // column in the next stage's band [f0, f0+fn) (R-tile units, f0 = e0 + en).
 // m1 = tiles per side outside band e.  Region 1: band-f rows x every non-e
 // column (fn * m1 tiles); region 2: rows outside both bands x band-f columns.
-template <int R, bool VEC>
+template <int R, bool VEC, bool PK>
 __global__ __launch_bounds__(256) void k_rest_band(int32_t* __restrict__ D, int32_t n, int32_t s0,
                                                    int32_t S, int32_t e0, int32_t en, int32_t f0,
                                                    int32_t fn, int32_t m1) {
-  using P = TileProduct<R, R, kChunk, VEC>;
+  using P = TileProduct<R, R, kChunk, VEC, 16, PK>;
   __shared__ typename P::Smem s;
   int b = blockIdx.x;
   int I, J;
@@ -189,28 +258,48 @@ __global__ __launch_bounds__(256) void k_rest_band(int32_t* __restrict__ D, int3
 // ---------------------------------------------------------------------------
 // host driver
 
-template <int T, bool VEC>
+// FWX_DPANEL=0 keeps diag and panel as two launches.
+static bool fused_dpanel() {
+  const char* e = std::getenv("FWX_DPANEL");
+  return !(e && e[0] == '0');
+}
+
+template <int T, bool VEC, bool PK>
 static int lead_round(int32_t* d, int32_t n, int32_t c, int32_t kb, int32_t B, int32_t N,
                       hipStream_t st) {
-  k_diag<T><<<1, 256, 0, st>>>(d, n, c);
-  FWX_LAUNCHED("k_diag");
-  if (N > 1) {
-    k_panel<T, VEC><<<2 * (N - 1), 256, 0, st>>>(d, n, c, N);
-    FWX_LAUNCHED("k_panel");
+  if (N > 1 && fused_dpanel()) {
+    k_dpanel<T, VEC, PK><<<2 * (N - 1), 256, 0, st>>>(d, n, c, N);
+    FWX_LAUNCHED("k_dpanel");
+  } else {
+    k_diag<T><<<1, 256, 0, st>>>(d, n, c);
+    FWX_LAUNCHED("k_diag");
+    if (N > 1) {
+      k_panel<T, VEC, PK><<<2 * (N - 1), 256, 0, st>>>(d, n, c, N);
+      FWX_LAUNCHED("k_panel");
+    }
   }
   if (B > 1) {
     const int cnt = (B - 1) * (2 * N - B - 1);
     if (cnt > 0) {
-      k_cross<T, VEC><<<cnt, 256, 0, st>>>(d, n, c, kb, B, N);
+      k_cross<T, VEC, PK><<<cnt, 256, 0, st>>>(d, n, c, kb, B, N);
       FWX_LAUNCHED("k_cross");
     }
   }
   return FWX_OK;
 }
 
+// Shape of the 128-wide rest kernel (FWX_REST_VARIANT): 2 = 512 threads,
+// 8x4 cells each, 126 VGPRs, two blocks (4 waves/SIMD) per CU -- the default,
+// fastest measured; 0 = 256 threads, 8x8 cells, 32-deep chunks (2 waves/SIMD);
+// 1 = 256 threads with 16-deep chunks (3 waves/SIMD, twice the barriers).
+static int rest_variant() {
+  const char* e = std::getenv("FWX_REST_VARIANT");
+  return (e && e[0] >= '0' && e[0] <= '2') ? e[0] - '0' : 2;
+}
+
 // Rest tiles outside the R-tile band [s0/R, s0/R + excl): excl covers stage
 // s alone (plain schedule) or stage s and the next stage (look-ahead).
-template <int R, bool VEC>
+template <int R, bool VEC, bool PK>
 static int rest_pass(int32_t* d, int32_t n, int32_t s0, int32_t S, int32_t excl, hipStream_t st) {
   const int nr = (n + R - 1) / R;
   const int sr0 = s0 / R;
@@ -219,12 +308,18 @@ static int rest_pass(int32_t* d, int32_t n, int32_t s0, int32_t S, int32_t excl,
   if (m <= 0) return FWX_OK;
   const int64_t total = (int64_t)m * m;
   const int per = (int)((total + 7) / 8);
-  k_rest<R, VEC><<<8 * per, 256, 0, st>>>(d, n, s0, S, sr0, srn, m, per);
+  const int v = R == 128 ? rest_variant() : 0;
+  if (v == 1)
+    k_rest<R, VEC, PK, 16><<<8 * per, 256, 0, st>>>(d, n, s0, S, sr0, srn, m, per);
+  else if (v == 2)
+    k_rest512<R, VEC, PK><<<8 * per, 512, 0, st>>>(d, n, s0, S, sr0, srn, m, per);
+  else
+    k_rest<R, VEC, PK><<<8 * per, 256, 0, st>>>(d, n, s0, S, sr0, srn, m, per);
   FWX_LAUNCHED("k_rest");
   return FWX_OK;
 }
 
-template <int R, bool VEC>
+template <int R, bool VEC, bool PK>
 static int band_pass(int32_t* d, int32_t n, int32_t s0, int32_t S, int32_t en, int32_t fn_tiles,
                      hipStream_t st) {
   const int nr = (n + R - 1) / R;
@@ -235,39 +330,39 @@ static int band_pass(int32_t* d, int32_t n, int32_t s0, int32_t S, int32_t en, i
   if (fn <= 0 || m1 <= 0) return FWX_OK;
   const int cnt = fn * m1 + (m1 - fn) * fn;
   if (cnt <= 0) return FWX_OK;
-  k_rest_band<R, VEC><<<cnt, 256, 0, st>>>(d, n, s0, S, e0, en, f0, fn, m1);
+  k_rest_band<R, VEC, PK><<<cnt, 256, 0, st>>>(d, n, s0, S, e0, en, f0, fn, m1);
   FWX_LAUNCHED("k_rest_band");
   return FWX_OK;
 }
 
-template <bool VEC>
+template <bool VEC, bool PK>
 static int rest_dispatch(int R, int32_t* d, int32_t n, int32_t s0, int32_t S, int32_t excl, hipStream_t st) {
-  if (R == 128) return rest_pass<128, VEC>(d, n, s0, S, excl, st);
-  if (R == 64) return rest_pass<64, VEC>(d, n, s0, S, excl, st);
-  return rest_pass<32, VEC>(d, n, s0, S, excl, st);
+  if (R == 128) return rest_pass<128, VEC, PK>(d, n, s0, S, excl, st);
+  if (R == 64) return rest_pass<64, VEC, PK>(d, n, s0, S, excl, st);
+  return rest_pass<32, VEC, PK>(d, n, s0, S, excl, st);
 }
 
-template <bool VEC>
+template <bool VEC, bool PK>
 static int band_dispatch(int R, int32_t* d, int32_t n, int32_t s0, int32_t S, int32_t en, int32_t fn,
                          hipStream_t st) {
-  if (R == 128) return band_pass<128, VEC>(d, n, s0, S, en, fn, st);
-  if (R == 64) return band_pass<64, VEC>(d, n, s0, S, en, fn, st);
-  return band_pass<32, VEC>(d, n, s0, S, en, fn, st);
+  if (R == 128) return band_pass<128, VEC, PK>(d, n, s0, S, en, fn, st);
+  if (R == 64) return band_pass<64, VEC, PK>(d, n, s0, S, en, fn, st);
+  return band_pass<32, VEC, PK>(d, n, s0, S, en, fn, st);
 }
 
 // The lead rounds of one stage, bracketed for the profiler.
-template <int T, bool VEC>
+template <int T, bool VEC, bool PK>
 static int lead_stage(int32_t* d, int32_t n, int32_t kb, int32_t B, int32_t N, hipStream_t st) {
   const double L = (double)min(B * T, n - kb * T), nn = (double)n;
   hipEvent_t ev;
   if (int rc = prof_open(st, &ev)) return rc;
   for (int l = 0; l < B; ++l)
-    if (int rc = lead_round<T, VEC>(d, n, kb + l, kb, B, N, st)) return rc;
+    if (int rc = lead_round<T, VEC, PK>(d, n, kb + l, kb, B, N, st)) return rc;
   return prof_close(st, ev, FWX_PROF_LEAD, (nn * nn - (nn - L) * (nn - L)) * L);
 }
 
 // Plain schedule: lead(s), rest(s), lead(s+1), ... on one stream.
-template <int T, bool VEC>
+template <int T, bool VEC, bool PK>
 static int run_blocked(int32_t* d, int32_t n, int32_t kappa, hipStream_t st) {
   const int N = (n + T - 1) / T;
   const int span = kappa * T;
@@ -276,10 +371,10 @@ static int run_blocked(int32_t* d, int32_t n, int32_t kappa, hipStream_t st) {
     const int B = min(kappa, N - kb);
     const int s0 = kb * T, S = B * T;
     const double L = (double)min(S, n - s0), nn = (double)n;
-    if (int rc = lead_stage<T, VEC>(d, n, kb, B, N, st)) return rc;
+    if (int rc = lead_stage<T, VEC, PK>(d, n, kb, B, N, st)) return rc;
     hipEvent_t ev;
     if (int rc = prof_open(st, &ev)) return rc;
-    if (int rc = rest_dispatch<VEC>(R, d, n, s0, S, (S + R - 1) / R, st)) return rc;
+    if (int rc = rest_dispatch<VEC, PK>(R, d, n, s0, S, (S + R - 1) / R, st)) return rc;
     if (int rc = prof_close(st, ev, FWX_PROF_REST, (nn - L) * (nn - L) * L)) return rc;
   }
   return FWX_OK;
@@ -293,14 +388,14 @@ static int run_blocked(int32_t* d, int32_t n, int32_t kappa, hipStream_t st) {
 // neither band s+1 nor anything lead(s+1) writes (lead(s+1) writes only band
 // s+1 rows/columns; rest(s) reads only band s rows/columns).  The caller's
 // stream waits for lead(s+1) before stage s+1's rest.
-template <int T, bool VEC>
+template <int T, bool VEC, bool PK>
 static int run_blocked_lookahead(int32_t* d, int32_t n, int32_t kappa, hipStream_t st, hipStream_t side,
                                  hipEvent_t evA, hipEvent_t evB) {
   const int N = (n + T - 1) / T;
   const int span = kappa * T;
   const int R = (span % 128 == 0) ? 128 : (span % 64 == 0) ? 64 : 32;
   const double nn = (double)n;
-  if (int rc = lead_stage<T, VEC>(d, n, 0, min(kappa, N), N, st)) return rc;
+  if (int rc = lead_stage<T, VEC, PK>(d, n, 0, min(kappa, N), N, st)) return rc;
   for (int kb = 0; kb < N; kb += kappa) {
     const int B = min(kappa, N - kb);
     const int s0 = kb * T, S = B * T;
@@ -313,20 +408,20 @@ static int run_blocked_lookahead(int32_t* d, int32_t n, int32_t kappa, hipStream
       const int fn = (B2 * T + R - 1) / R;
       const double L2 = (double)min(B2 * T, n - kb2 * T);
       if (int rc = prof_open(st, &ev)) return rc;
-      if (int rc = band_dispatch<VEC>(R, d, n, s0, S, en, fn, st)) return rc;
+      if (int rc = band_dispatch<VEC, PK>(R, d, n, s0, S, en, fn, st)) return rc;
       if (int rc = prof_close(st, ev, FWX_PROF_BAND, ((nn - L) * (nn - L) - (nn - L - L2) * (nn - L - L2)) * L))
         return rc;
       FWX_HIP(hipEventRecord(evA, st));
       FWX_HIP(hipStreamWaitEvent(side, evA, 0));
-      if (int rc = lead_stage<T, VEC>(d, n, kb2, B2, N, side)) return rc;
+      if (int rc = lead_stage<T, VEC, PK>(d, n, kb2, B2, N, side)) return rc;
       FWX_HIP(hipEventRecord(evB, side));
       if (int rc = prof_open(st, &ev)) return rc;
-      if (int rc = rest_dispatch<VEC>(R, d, n, s0, S, en + fn, st)) return rc;
+      if (int rc = rest_dispatch<VEC, PK>(R, d, n, s0, S, en + fn, st)) return rc;
       if (int rc = prof_close(st, ev, FWX_PROF_REST, (nn - L - L2) * (nn - L - L2) * L)) return rc;
       FWX_HIP(hipStreamWaitEvent(st, evB, 0));
     } else {
       if (int rc = prof_open(st, &ev)) return rc;
-      if (int rc = rest_dispatch<VEC>(R, d, n, s0, S, en, st)) return rc;
+      if (int rc = rest_dispatch<VEC, PK>(R, d, n, s0, S, en, st)) return rc;
       if (int rc = prof_close(st, ev, FWX_PROF_REST, (nn - L) * (nn - L) * L)) return rc;
     }
   }
@@ -353,10 +448,10 @@ static bool lookahead_enabled() {
   return !(e && e[0] == '0');
 }
 
-template <int T, bool VEC>
+template <int T, bool VEC, bool PK>
 static int run_any(int32_t* d, int32_t n, int32_t kappa, hipStream_t st) {
   const int N = (n + T - 1) / T;
-  if (!lookahead_enabled() || N <= kappa) return run_blocked<T, VEC>(d, n, kappa, st);
+  if (!lookahead_enabled() || N <= kappa) return run_blocked<T, VEC, PK>(d, n, kappa, st);
   hipStream_t side;
   if (int rc = side_stream(&side)) return rc;
   hipEvent_t evA, evB;
@@ -366,24 +461,33 @@ static int run_any(int32_t* d, int32_t n, int32_t kappa, hipStream_t st) {
     (void)hipEventDestroy(evA);
     return hip_fail(e, "hipEventCreateWithFlags", __FILE__, __LINE__);
   }
-  const int rc = run_blocked_lookahead<T, VEC>(d, n, kappa, st, side, evA, evB);
+  const int rc = run_blocked_lookahead<T, VEC, PK>(d, n, kappa, st, side, evA, evB);
   (void)hipEventDestroy(evA);
   (void)hipEventDestroy(evB);
   return rc;
 }
 
-int enqueue_blocked(int32_t* d, int32_t n, int32_t tile, int32_t kappa, hipStream_t st) {
-  const bool vec = (n % 4 == 0) && ((reinterpret_cast<uintptr_t>(d) & 15u) == 0);
+template <bool VEC, bool PK>
+static int by_tile(int32_t* d, int32_t n, int32_t tile, int32_t kappa, hipStream_t st) {
   switch (tile) {
     case 32:
-      return vec ? run_any<32, true>(d, n, kappa, st) : run_any<32, false>(d, n, kappa, st);
+      return run_any<32, VEC, PK>(d, n, kappa, st);
     case 64:
-      return vec ? run_any<64, true>(d, n, kappa, st) : run_any<64, false>(d, n, kappa, st);
+      return run_any<64, VEC, PK>(d, n, kappa, st);
     case 128:
-      return vec ? run_any<128, true>(d, n, kappa, st) : run_any<128, false>(d, n, kappa, st);
+      return run_any<128, VEC, PK>(d, n, kappa, st);
     default:
       return fail(FWX_ERR_INVALID_ARG, "tile must be 32, 64 or 128");
   }
+}
+
+int enqueue_blocked(int32_t* d, int32_t n, int32_t tile, int32_t kappa, hipStream_t st) {
+  const bool vec = (n % 4 == 0) && ((reinterpret_cast<uintptr_t>(d) & 15u) == 0);
+  bool pk = false;
+  if (int rc = values_in_pk64_range(d, (int64_t)n * n, st, &pk)) return rc;
+  if (vec)
+    return pk ? by_tile<true, true>(d, n, tile, kappa, st) : by_tile<true, false>(d, n, tile, kappa, st);
+  return pk ? by_tile<false, true>(d, n, tile, kappa, st) : by_tile<false, false>(d, n, tile, kappa, st);
 }
 
 }  // namespace fwx

@@ -13,8 +13,11 @@
 //     exact when the low half cannot carry into the high half, i.e. when
 //     a0 + b0 < 2^32 as unsigned; every operand in [0, 2^30) guarantees it
 //     (and then no int32 sum overflows either, so the signed compare sees the
-//     same value as the reference).  Kernels check this per staged chunk of
-//     operands and take the min3 form otherwise.
+//     same value as the reference).  If every input cell lies in [0, 2^30),
+//     so does every value a Floyd-Warshall run produces (each is a sum of
+//     input cells along a walk, or an input cell, and only ever decreases),
+//     so one check of the input per call (fwx_range.hip) selects the form for
+//     the whole run.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -23,7 +26,7 @@
 namespace fwx {
 
 constexpr int32_t kInf = 1000000000;  // include/constants.h:3
-// Operand values with either of the top two bits set disable the pk64 form.
+// A value with either of the top two bits set rules out the pk64 form.
 constexpr uint32_t kPk64Mask = 0xC0000000u;
 
 __device__ __forceinline__ int32_t wadd(int32_t a, int32_t b) {
@@ -58,22 +61,6 @@ __device__ __forceinline__ int32_t relax2_min3(int32_t c, uint64_t a, uint64_t b
 
 __device__ __forceinline__ uint64_t pack2(int32_t lo, int32_t hi) {
   return (uint64_t)(uint32_t)lo | ((uint64_t)(uint32_t)hi << 32);
-}
-
-// Block-wide OR of a per-thread "operand outside [0,2^30)" flag, folded into
-// the barrier the caller needs anyway: each wave publishes one word.
-template <int NWAVES>
-__device__ __forceinline__ void publish_wave_flag(int* flags, bool bad) {
-  const unsigned long long any = __ballot(bad);
-  if ((threadIdx.x & 63) == 0) flags[threadIdx.x >> 6] = any != 0ull;
-}
-
-template <int NWAVES>
-__device__ __forceinline__ bool read_block_flag(const int* flags) {
-  int v = 0;
-#pragma unroll
-  for (int w = 0; w < NWAVES; ++w) v |= flags[w];
-  return v != 0;
 }
 
 }  // namespace fwx

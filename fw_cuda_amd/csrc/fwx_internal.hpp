@@ -42,6 +42,11 @@ bool profiling();
 int prof_open(hipStream_t st, hipEvent_t* a);
 int prof_close(hipStream_t st, hipEvent_t a, int cls, double work);
 
+// Range check of count cells at d (fwx_range.hip): *ok = every cell in
+// [0, 2^30), i.e. the pk64 relaxation form is exact for the whole run.
+// Synchronises `st`.
+int values_in_pk64_range(const int32_t* d, int64_t count, hipStream_t st, bool* ok);
+
 // Tuned (tile, kappa) for a plain run of size n.
 void tuned_config(int32_t n, int32_t* tile, int32_t* kappa);
 

@@ -1,0 +1,86 @@
+// fwx_range.hip -- per-call choice of the relaxation form.
+//
+// The pk64 form (fwx_device.hpp) is exact when every operand lies in
+// [0, 2^30).  Floyd-Warshall only ever stores min(old, a + b) where a, b are
+// earlier values, so if every input cell is in that range every value of the
+// run is too.  One pass over the input decides the form for the whole call.
+//
+// The flag lives in a file-scope device array (no allocation): a call takes
+// one of 64 slots round-robin, so up to 64 concurrent calls on one device do
+// not share a slot.
+#include <hip/hip_runtime.h>
+
+#include <atomic>
+#include <mutex>
+
+#include "fwx_device.hpp"
+#include "fwx_internal.hpp"
+
+namespace fwx {
+
+namespace {
+
+constexpr int kSlots = 64;
+__device__ int g_range_flags[kSlots];
+
+__global__ void k_flag_zero(int* f) { *f = 0; }
+
+__global__ __launch_bounds__(256) void k_range(const int32_t* __restrict__ d, int64_t count, int vec4,
+                                               int* __restrict__ f) {
+  uint32_t o = 0;
+  const int64_t tid = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int64_t stride = (int64_t)gridDim.x * 256;
+  int64_t done = 0;
+  if (vec4) {
+    const int4* d4 = reinterpret_cast<const int4*>(d);
+    const int64_t n4 = count / 4;
+    for (int64_t i = tid; i < n4; i += stride) {
+      const int4 v = d4[i];
+      o |= (uint32_t)v.x | (uint32_t)v.y | (uint32_t)v.z | (uint32_t)v.w;
+    }
+    done = n4 * 4;
+  }
+  for (int64_t i = done + tid; i < count; i += stride) o |= (uint32_t)d[i];
+  const bool bad = (o & kPk64Mask) != 0u;
+  if (__ballot(bad) != 0ull && (threadIdx.x & 63) == 0) atomicOr(f, 1);
+}
+
+std::mutex g_mu;
+int* g_flag_base[64] = {};
+std::atomic<unsigned> g_slot{0};
+
+int flag_base(int** out) {
+  int dev = 0;
+  FWX_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 64) return fail(FWX_ERR_UNSUPPORTED, "device index above 63");
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (!g_flag_base[dev]) {
+    void* p = nullptr;
+    FWX_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(g_range_flags)));
+    g_flag_base[dev] = static_cast<int*>(p);
+  }
+  *out = g_flag_base[dev];
+  return FWX_OK;
+}
+
+}  // namespace
+
+int values_in_pk64_range(const int32_t* d, int64_t count, hipStream_t st, bool* ok) {
+  int* base = nullptr;
+  if (int rc = flag_base(&base)) return rc;
+  int* f = base + (g_slot.fetch_add(1) % kSlots);
+  k_flag_zero<<<1, 1, 0, st>>>(f);
+  FWX_LAUNCHED("k_flag_zero");
+  const int64_t work = (count + 3) / 4;
+  const int blocks = (int)std::min<int64_t>((work + 255) / 256, 4096);
+  const int vec4 = (reinterpret_cast<uintptr_t>(d) & 15u) == 0;
+  k_range<<<blocks > 0 ? blocks : 1, 256, 0, st>>>(d, count, vec4, f);
+  FWX_LAUNCHED("k_range");
+  int h = 1;
+  FWX_HIP(hipMemcpyAsync(&h, f, sizeof(int), hipMemcpyDeviceToHost, st));
+  FWX_HIP(hipStreamSynchronize(st));
+  *ok = (h == 0);
+  return FWX_OK;
+}
+
+}  // namespace fwx

@@ -2,20 +2,20 @@
 //
 //   C[BM x BN] <- min(C, A[BM x K] (x) B[K x BN])      ((x) = min-plus product)
 //
-// One 256-thread workgroup (four wave64s, one per SIMD) owns the C block in
-// VGPRs: thread (tx, ty) = (tid % 16, tid / 16) holds cells
-// (ty + 16 r, tx + 16 c) for r < BM/16, c < BN/16.  The interleaved mapping
-// makes the LDS reads conflict-free: a wave's 16 tx lanes read 16
-// consecutive B pairs (128 B) and its 4 ty rows hit 4 distinct 16-byte bank
-// groups of A (row stride KC+4 dwords), the rest are broadcasts.
+// One workgroup of NTX x 16 threads (256 or 512: one or two wave64s per
+// SIMD) owns the C block in VGPRs: thread (tx, ty) = (tid % NTX, tid / NTX)
+// holds cells in rows ty + 16 r and in groups of up to four adjacent columns
+// (colmap below).  The LDS reads are conflict-free: a wave's tx lanes read
+// consecutive 64-bit B pairs and its ty rows hit distinct 16-byte bank groups
+// of A (row stride KC+4 dwords); the rest are broadcasts.
 //
 // K is walked in chunks of KC, double-buffered through LDS with one barrier
 // per chunk.  A is staged row-major (k contiguous), B pair-interleaved
 // (B[k][j], B[k+1][j] adjacent), so both operands of a k pair are one 64-bit
-// register each -- the layout the pk64 form needs (see fwx_device.hpp).
-// While staging, each wave ballots whether any operand of the chunk lies
-// outside [0, 2^30); the flag rides on the chunk barrier and selects the
-// pk64 or the min3 form for the whole chunk, uniformly across the block.
+// register each -- the layout the pk64 form needs (fwx_device.hpp).  The
+// arithmetic form (PK64 or min3) is a template parameter chosen per call by
+// the host from a range check of the input (fwx_range.hip): carrying both
+// forms in one kernel costs ~60 VGPRs of the min3 form's temporaries.
 //
 // Out-of-range operand cells (row/col/k >= n) are staged as INF and C cells
 // outside the matrix are never stored, so ragged n needs no padding buffer.
@@ -49,13 +49,24 @@ __device__ __forceinline__ View make_view(int32_t* D, int32_t n, int32_t r0, int
 
 // VEC: 16-byte global loads (needs ld % 4 == 0, 16-byte aligned base and
 // c0 % 4 == 0, so a 4-wide group is wholly in or wholly out of range).
-template <int BM, int BN, int KC, bool VEC>
+// NTX threads across columns x 16 across rows: 256 (NTX=16) or 512 (NTX=32).
+template <int BM, int BN, int KC, bool VEC, int NTX, bool PK64>
 struct TileProduct {
-  static constexpr int NT = 256;
+  static constexpr int NT = NTX * 16;
+  static constexpr int NW = NT / 64;  // waves
   static constexpr int TM = BM / 16;
-  static constexpr int TN = BN / 16;
+  static constexpr int TN = BN / NTX;
   static constexpr int LDA = KC + 4;
-  static_assert(BM % 16 == 0 && BN % 16 == 0, "block must be a multiple of 16");
+  // Column placement.  Thread tx owns groups of G = min(4, TN) adjacent
+  // columns: cell slot c is column colmap(tx, c) = G*(tx + NTX*(c/G)) + c%G.
+  // In LDS, B's pair row stores column j at pair slot bpos(j) =
+  // (j%G)*(BN/G) + j/G, so a compute read (fixed c, all tx) and a stash write
+  // (fixed column offset e, consecutive 4-column groups) both touch
+  // consecutive 8-byte slots across lanes -- conflict-free either way.
+  static constexpr int G = TN >= 4 ? 4 : TN;
+  __device__ __forceinline__ static int colmap(int tx, int c) { return G * (tx + NTX * (c / G)) + c % G; }
+  __device__ __forceinline__ static int bpos(int j) { return (j % G) * (BN / G) + j / G; }
+  static_assert(BM % 16 == 0 && BN % NTX == 0, "block must divide over the threads");
   static_assert(KC % 4 == 0, "chunk must hold whole k quads");
   // A staging: KC/4 int4 groups per row.
   static constexpr int AG = KC / 4;
@@ -68,90 +79,131 @@ struct TileProduct {
   struct Smem {
     int32_t a[2][BM * LDA];
     int32_t b[2][(KC / 2) * BN * 2];
-    int flag[2][4];
   };
 
   int4 ra[APASS];
   int4 rb0[BPASS], rb1[BPASS];
-  bool bad;
+  bool full_;
+  int32_t kc_;
 
-  __device__ __forceinline__ static int4 load4(const View& v, int32_t r, int32_t c) {
-    if (VEC) {
-      if (v.in(r, c)) return *reinterpret_cast<const int4*>(v.at(r, c));
-      return make_int4(kInf, kInf, kInf, kInf);
-    } else {
-      int4 x;
-      x.x = v.in(r, c + 0) ? *v.at(r, c + 0) : kInf;
-      x.y = v.in(r, c + 1) ? *v.at(r, c + 1) : kInf;
-      x.z = v.in(r, c + 2) ? *v.at(r, c + 2) : kInf;
-      x.w = v.in(r, c + 3) ? *v.at(r, c + 3) : kInf;
-      return x;
-    }
-  }
-
-  __device__ __forceinline__ static uint32_t orbits(int4 x) {
-    return (uint32_t)x.x | (uint32_t)x.y | (uint32_t)x.z | (uint32_t)x.w;
-  }
-
-  // global -> registers for the chunk starting at k offset kc0.  When the
-  // whole chunk is inside the matrix (the common case) every address is a
-  // uniform row base (SGPRs) plus one per-thread offset (one VGPR) plus an
-  // immediate; the bounds-checked path handles the ragged edge.
+  // global -> registers for the chunk starting at k offset kc0.  The loads
+  // are unconditional: out-of-range rows/columns are clamped into the matrix
+  // and replaced by INF in stash(), so no branch merges around the loads and
+  // they stay in flight while the previous chunk is computed.  When the chunk
+  // is wholly inside the matrix (the common case), every address is a
+  // uniform row base (SGPRs) plus one per-thread offset plus an immediate.
   __device__ __forceinline__ void fetch(const View& A, const View& B, int32_t kc0) {
     const int tid = threadIdx.x;
-    uint32_t o = 0;
     static_assert(NT % (BN / 4) == 0, "B staging rows must be uniform per pass");
     constexpr int BROWS = NT / (BN / 4);  // k pairs per B pass
-    const bool full = VEC && A.rlim >= BM && A.clim >= kc0 + KC && B.rlim >= kc0 + KC && B.clim >= BN;
-    if (full) {
-      const int32_t* pa = A.p + kc0;
-      const int32_t oa = (tid / AG) * A.ld + (tid % AG) * 4;
+    full_ = A.rlim >= BM && A.clim >= kc0 + KC && B.rlim >= kc0 + KC && B.clim >= BN;
+    kc_ = kc0;
+    if (VEC) {
+      const int32_t* qa[APASS];
+      const int32_t* qb[BPASS];
+      if (full_) {
+        const int32_t* pa = A.p + kc0;
+        const int32_t oa = (tid / AG) * A.ld + (tid % AG) * 4;
 #pragma unroll
-      for (int p = 0; p < APASS; ++p) {
-        if (BM % AROWS == 0 || p * AROWS + tid / AG < BM) {
-          ra[p] = *reinterpret_cast<const int4*>(pa + (p * AROWS) * A.ld + oa);
-          o |= orbits(ra[p]);
+        for (int p = 0; p < APASS; ++p) qa[p] = pa + (p * AROWS) * A.ld + oa;
+        const int32_t* pb = B.p + kc0 * B.ld;
+        const int32_t ob = (tid / (BN / 4)) * 2 * B.ld + (tid % (BN / 4)) * 4;
+#pragma unroll
+        for (int u = 0; u < BPASS; ++u) qb[u] = pb + (u * BROWS * 2) * B.ld + ob;
+      } else {
+#pragma unroll
+        for (int p = 0; p < APASS; ++p) {
+          const int r = min(p * AROWS + tid / AG, A.rlim - 1);
+          const int k = min(kc0 + (tid % AG) * 4, A.clim - 4);
+          qa[p] = A.at(r, k);
+        }
+#pragma unroll
+        for (int u = 0; u < BPASS; ++u) {
+          const int idx = u * NT + tid;
+          const int k = min(kc0 + 2 * (idx / (BN / 4)), B.rlim - 1);
+          const int j = min((idx % (BN / 4)) * 4, B.clim - 4);
+          qb[u] = B.at(k, j);
         }
       }
-      const int32_t* pb = B.p + kc0 * B.ld;
-      const int32_t ob = (tid / (BN / 4)) * 2 * B.ld + (tid % (BN / 4)) * 4;
+#pragma unroll
+      for (int p = 0; p < APASS; ++p)
+        if (BM % AROWS == 0 || p * AROWS + tid / AG < BM) ra[p] = *reinterpret_cast<const int4*>(qa[p]);
 #pragma unroll
       for (int u = 0; u < BPASS; ++u) {
         if (BU % NT == 0 || u * NT + tid < BU) {
-          const int32_t* q = pb + (u * BROWS * 2) * B.ld + ob;
-          rb0[u] = *reinterpret_cast<const int4*>(q);
-          rb1[u] = *reinterpret_cast<const int4*>(q + B.ld);
-          o |= orbits(rb0[u]) | orbits(rb1[u]);
+          rb0[u] = *reinterpret_cast<const int4*>(qb[u]);
+          const int idx = u * NT + tid;
+          const bool second = full_ || kc0 + 2 * (idx / (BN / 4)) + 1 < B.rlim;
+          rb1[u] = *reinterpret_cast<const int4*>(qb[u] + (second ? B.ld : 0));
         }
       }
     } else {
 #pragma unroll
       for (int p = 0; p < APASS; ++p) {
-        const int r = p * AROWS + tid / AG;
-        const int k = (tid % AG) * 4;
-        if (BM % AROWS == 0 || r < BM) {
-          ra[p] = load4(A, r, kc0 + k);
-          o |= orbits(ra[p]);
+        const int r = min(p * AROWS + tid / AG, A.rlim - 1);
+        const int k = kc0 + (tid % AG) * 4;
+        const int32_t* q = A.p + r * A.ld;
+        if (BM % AROWS == 0 || p * AROWS + tid / AG < BM) {
+          ra[p].x = q[min(k + 0, A.clim - 1)];
+          ra[p].y = q[min(k + 1, A.clim - 1)];
+          ra[p].z = q[min(k + 2, A.clim - 1)];
+          ra[p].w = q[min(k + 3, A.clim - 1)];
         }
       }
 #pragma unroll
       for (int u = 0; u < BPASS; ++u) {
         const int idx = u * NT + tid;
         if (BU % NT == 0 || idx < BU) {
-          const int pr = idx / (BN / 4);
+          const int k0 = min(kc0 + 2 * (idx / (BN / 4)), B.rlim - 1);
+          const int k1 = min(kc0 + 2 * (idx / (BN / 4)) + 1, B.rlim - 1);
           const int j = (idx % (BN / 4)) * 4;
-          rb0[u] = load4(B, kc0 + 2 * pr, j);
-          rb1[u] = load4(B, kc0 + 2 * pr + 1, j);
-          o |= orbits(rb0[u]) | orbits(rb1[u]);
+          const int32_t* q0 = B.p + k0 * B.ld;
+          const int32_t* q1 = B.p + k1 * B.ld;
+          rb0[u].x = q0[min(j + 0, B.clim - 1)];
+          rb0[u].y = q0[min(j + 1, B.clim - 1)];
+          rb0[u].z = q0[min(j + 2, B.clim - 1)];
+          rb0[u].w = q0[min(j + 3, B.clim - 1)];
+          rb1[u].x = q1[min(j + 0, B.clim - 1)];
+          rb1[u].y = q1[min(j + 1, B.clim - 1)];
+          rb1[u].z = q1[min(j + 2, B.clim - 1)];
+          rb1[u].w = q1[min(j + 3, B.clim - 1)];
         }
       }
     }
-    bad = (o & kPk64Mask) != 0u;
   }
 
-  // registers -> LDS buffer `buf`
-  __device__ __forceinline__ void stash(Smem& s, int buf) {
+  __device__ __forceinline__ static int32_t keep(bool ok, int32_t v) { return ok ? v : kInf; }
+
+  // registers -> LDS buffer `buf`.  Cells that fetch() clamped become INF.
+  __device__ __forceinline__ void stash(Smem& s, int buf, const View& A, const View& B) {
     const int tid = threadIdx.x;
+    if (!full_) {
+#pragma unroll
+      for (int p = 0; p < APASS; ++p) {
+        const int r = p * AROWS + tid / AG;
+        const int k = kc_ + (tid % AG) * 4;
+        const bool rok = r < A.rlim;
+        ra[p].x = keep(rok && k + 0 < A.clim, ra[p].x);
+        ra[p].y = keep(rok && k + 1 < A.clim, ra[p].y);
+        ra[p].z = keep(rok && k + 2 < A.clim, ra[p].z);
+        ra[p].w = keep(rok && k + 3 < A.clim, ra[p].w);
+      }
+#pragma unroll
+      for (int u = 0; u < BPASS; ++u) {
+        const int idx = u * NT + tid;
+        const int k0 = kc_ + 2 * (idx / (BN / 4));
+        const int j = (idx % (BN / 4)) * 4;
+        const bool r0 = k0 < B.rlim, r1 = k0 + 1 < B.rlim;
+        rb0[u].x = keep(r0 && j + 0 < B.clim, rb0[u].x);
+        rb0[u].y = keep(r0 && j + 1 < B.clim, rb0[u].y);
+        rb0[u].z = keep(r0 && j + 2 < B.clim, rb0[u].z);
+        rb0[u].w = keep(r0 && j + 3 < B.clim, rb0[u].w);
+        rb1[u].x = keep(r1 && j + 0 < B.clim, rb1[u].x);
+        rb1[u].y = keep(r1 && j + 1 < B.clim, rb1[u].y);
+        rb1[u].z = keep(r1 && j + 2 < B.clim, rb1[u].z);
+        rb1[u].w = keep(r1 && j + 3 < B.clim, rb1[u].w);
+      }
+    }
 #pragma unroll
     for (int p = 0; p < APASS; ++p) {
       const int r = p * AROWS + tid / AG;
@@ -165,18 +217,22 @@ struct TileProduct {
       if (BU % NT == 0 || idx < BU) {
         const int pr = idx / (BN / 4);
         const int j = (idx % (BN / 4)) * 4;
-        int32_t* dst = &s.b[buf][(pr * BN + j) * 2];
-        *reinterpret_cast<int4*>(dst) = make_int4(rb0[u].x, rb1[u].x, rb0[u].y, rb1[u].y);
-        *reinterpret_cast<int4*>(dst + 4) = make_int4(rb0[u].z, rb1[u].z, rb0[u].w, rb1[u].w);
+        int32_t* row = &s.b[buf][pr * BN * 2];
+        row[bpos(j + 0) * 2 + 0] = rb0[u].x;
+        row[bpos(j + 0) * 2 + 1] = rb1[u].x;
+        row[bpos(j + 1) * 2 + 0] = rb0[u].y;
+        row[bpos(j + 1) * 2 + 1] = rb1[u].y;
+        row[bpos(j + 2) * 2 + 0] = rb0[u].z;
+        row[bpos(j + 2) * 2 + 1] = rb1[u].z;
+        row[bpos(j + 3) * 2 + 0] = rb0[u].w;
+        row[bpos(j + 3) * 2 + 1] = rb1[u].w;
       }
     }
-    publish_wave_flag<4>(s.flag[buf], bad);
   }
 
-  template <bool PK64>
   __device__ __forceinline__ static void compute(const Smem& s, int buf, int32_t (&acc)[TM][TN]) {
-    const int tx = threadIdx.x & 15;
-    const int ty = threadIdx.x >> 4;
+    const int tx = threadIdx.x % NTX;
+    const int ty = threadIdx.x / NTX;
     const int32_t* As = s.a[buf];
     const uint64_t* Bs = reinterpret_cast<const uint64_t*>(s.b[buf]);
 #pragma unroll FWX_QUNROLL
@@ -188,8 +244,8 @@ struct TileProduct {
         a[r] = *reinterpret_cast<const int4*>(&As[(ty + 16 * r) * LDA + 4 * q]);
 #pragma unroll
       for (int c = 0; c < TN; ++c) {
-        b0[c] = Bs[(2 * q) * BN + tx + 16 * c];
-        b1[c] = Bs[(2 * q + 1) * BN + tx + 16 * c];
+        b0[c] = Bs[(2 * q) * BN + bpos(colmap(tx, c))];
+        b1[c] = Bs[(2 * q + 1) * BN + bpos(colmap(tx, c))];
       }
 #pragma unroll
       for (int r = 0; r < TM; ++r) {
@@ -214,16 +270,13 @@ struct TileProduct {
                                       int32_t (&acc)[TM][TN]) {
     const int nch = depth / KC;
     fetch(A, B, 0);
-    stash(s, 0);
+    stash(s, 0, A, B);
     __syncthreads();
     for (int ch = 0; ch < nch; ++ch) {
       const int buf = ch & 1;
       if (ch + 1 < nch) fetch(A, B, (ch + 1) * KC);
-      if (read_block_flag<4>(s.flag[buf]))
-        compute<false>(s, buf, acc);
-      else
-        compute<true>(s, buf, acc);
-      if (ch + 1 < nch) stash(s, buf ^ 1);
+      compute(s, buf, acc);
+      if (ch + 1 < nch) stash(s, buf ^ 1, A, B);
       __syncthreads();
     }
   }
@@ -233,42 +286,42 @@ struct TileProduct {
   }
 
   __device__ __forceinline__ static void load_c(const View& C, int32_t (&acc)[TM][TN]) {
-    const int tx = threadIdx.x & 15;
-    const int ty = threadIdx.x >> 4;
+    const int tx = threadIdx.x % NTX;
+    const int ty = threadIdx.x / NTX;
     if (full(C)) {
-      const int32_t oc = ty * C.ld + tx;
+      const int32_t oc = ty * C.ld + G * tx;
 #pragma unroll
       for (int r = 0; r < TM; ++r) {
         const int32_t* pr = C.p + (16 * r) * C.ld;
 #pragma unroll
-        for (int c = 0; c < TN; ++c) acc[r][c] = pr[oc + 16 * c];
+        for (int c = 0; c < TN; ++c) acc[r][c] = pr[oc + colmap(0, c)];
       }
     } else {
 #pragma unroll
       for (int r = 0; r < TM; ++r)
 #pragma unroll
         for (int c = 0; c < TN; ++c)
-          acc[r][c] = C.in(ty + 16 * r, tx + 16 * c) ? *C.at(ty + 16 * r, tx + 16 * c) : kInf;
+          acc[r][c] = C.in(ty + 16 * r, colmap(tx, c)) ? *C.at(ty + 16 * r, colmap(tx, c)) : kInf;
     }
   }
 
   __device__ __forceinline__ static void store_c(const View& C, const int32_t (&acc)[TM][TN]) {
-    const int tx = threadIdx.x & 15;
-    const int ty = threadIdx.x >> 4;
+    const int tx = threadIdx.x % NTX;
+    const int ty = threadIdx.x / NTX;
     if (full(C)) {
-      const int32_t oc = ty * C.ld + tx;
+      const int32_t oc = ty * C.ld + G * tx;
 #pragma unroll
       for (int r = 0; r < TM; ++r) {
         int32_t* pr = C.p + (16 * r) * C.ld;
 #pragma unroll
-        for (int c = 0; c < TN; ++c) pr[oc + 16 * c] = acc[r][c];
+        for (int c = 0; c < TN; ++c) pr[oc + colmap(0, c)] = acc[r][c];
       }
     } else {
 #pragma unroll
       for (int r = 0; r < TM; ++r)
 #pragma unroll
         for (int c = 0; c < TN; ++c)
-          if (C.in(ty + 16 * r, tx + 16 * c)) *C.at(ty + 16 * r, tx + 16 * c) = acc[r][c];
+          if (C.in(ty + 16 * r, colmap(tx, c))) *C.at(ty + 16 * r, colmap(tx, c)) = acc[r][c];
     }
   }
 };

@@ -32,6 +32,7 @@
 
 #include <cstdlib>
 #include <mutex>
+#include <vector>
 
 #include "fwx_device.hpp"
 #include "fwx_internal.hpp"
@@ -258,10 +259,11 @@ __global__ __launch_bounds__(256) void k_rest_band(int32_t* __restrict__ D, int3
 // ---------------------------------------------------------------------------
 // host driver
 
-// FWX_DPANEL=0 keeps diag and panel as two launches.
+// FWX_DPANEL=1 fuses diag and panel into one launch (measured slower on
+// MI355X: the redundant closures contend with the rest pass).
 static bool fused_dpanel() {
   const char* e = std::getenv("FWX_DPANEL");
-  return !(e && e[0] == '0');
+  return e && e[0] == '1';
 }
 
 template <int T, bool VEC, bool PK>
@@ -342,10 +344,21 @@ static int rest_dispatch(int R, int32_t* d, int32_t n, int32_t s0, int32_t S, in
   return rest_pass<32, VEC, PK>(d, n, s0, S, excl, st);
 }
 
+// The band is only ~2(kappa*T/n) of a stage's rest work, too few 128-wide
+// tiles to fill the chip (about one per CU at n=8192), so it runs on 64-wide
+// tiles (4x the blocks) unless FWX_BAND=128.
+static bool band_narrow() {
+  const char* e = std::getenv("FWX_BAND");
+  return !(e && e[0] == '1');
+}
+
 template <bool VEC, bool PK>
 static int band_dispatch(int R, int32_t* d, int32_t n, int32_t s0, int32_t S, int32_t en, int32_t fn,
                          hipStream_t st) {
-  if (R == 128) return band_pass<128, VEC, PK>(d, n, s0, S, en, fn, st);
+  if (R == 128) {
+    if (band_narrow()) return band_pass<64, VEC, PK>(d, n, s0, S, 2 * en, 2 * fn, st);
+    return band_pass<128, VEC, PK>(d, n, s0, S, en, fn, st);
+  }
   if (R == 64) return band_pass<64, VEC, PK>(d, n, s0, S, en, fn, st);
   return band_pass<32, VEC, PK>(d, n, s0, S, en, fn, st);
 }
@@ -481,13 +494,109 @@ static int by_tile(int32_t* d, int32_t n, int32_t tile, int32_t kappa, hipStream
   }
 }
 
+static int enqueue_direct(int32_t* d, int32_t n, int32_t tile, int32_t kappa, bool vec, bool pk,
+                          hipStream_t st) {
+  if (vec)
+    return pk ? by_tile<true, true>(d, n, tile, kappa, st) : by_tile<true, false>(d, n, tile, kappa, st);
+  return pk ? by_tile<false, true>(d, n, tile, kappa, st) : by_tile<false, false>(d, n, tile, kappa, st);
+}
+
+// ---------------------------------------------------------------------------
+// Replay cache.  A solve is a fixed sequence of a few hundred launches for a
+// given (buffer, n, tile, kappa, form, stream); the first call captures it
+// (both streams of the look-ahead schedule, joined by events) into a
+// hipGraph and later calls replay it, which removes most inter-launch gaps.
+// Entries are kept per process, at most kGraphCache, least recently used
+// first out.  FWX_GRAPH=0 disables it.  Used up to n = kGraphMaxN only:
+// measured on MI355X (tuned tile/kappa, best of 4-10 solves) the replay is
+// 10 % faster than direct launches at n=2048 and 2 % at n=4096, but 1.7 %
+// slower at n=8192, where launch gaps are negligible next to ~0.5 ms rest
+// kernels.  Profiled calls launch directly: events recorded by a graph's
+// event nodes cannot be timed with hipEventElapsedTime (the runtime reports
+// an invalid handle).
+
+namespace {
+
+struct GraphKey {
+  int32_t* d;
+  int32_t n, tile, kappa;
+  bool vec, pk, la;
+  int knobs;  // FWX_DPANEL / FWX_REST_VARIANT / FWX_BAND choices
+  hipStream_t st;
+  int dev;
+  bool operator==(const GraphKey& o) const {
+    return d == o.d && n == o.n && tile == o.tile && kappa == o.kappa && vec == o.vec && pk == o.pk &&
+           la == o.la && knobs == o.knobs && st == o.st && dev == o.dev;
+  }
+};
+
+struct GraphEntry {
+  GraphKey key;
+  hipGraph_t graph;
+  hipGraphExec_t exec;
+  uint64_t used;
+};
+
+constexpr size_t kGraphCache = 8;
+constexpr int32_t kGraphMaxN = 4096;
+std::mutex g_graph_mu;
+std::vector<GraphEntry> g_graphs;
+uint64_t g_graph_clock = 0;
+
+bool graphs_enabled() {
+  const char* e = std::getenv("FWX_GRAPH");
+  return !(e && e[0] == '0');
+}
+
+}  // namespace
+
 int enqueue_blocked(int32_t* d, int32_t n, int32_t tile, int32_t kappa, hipStream_t st) {
   const bool vec = (n % 4 == 0) && ((reinterpret_cast<uintptr_t>(d) & 15u) == 0);
   bool pk = false;
   if (int rc = values_in_pk64_range(d, (int64_t)n * n, st, &pk)) return rc;
-  if (vec)
-    return pk ? by_tile<true, true>(d, n, tile, kappa, st) : by_tile<true, false>(d, n, tile, kappa, st);
-  return pk ? by_tile<false, true>(d, n, tile, kappa, st) : by_tile<false, false>(d, n, tile, kappa, st);
+  if (n > kGraphMaxN || !graphs_enabled() || profiling()) return enqueue_direct(d, n, tile, kappa, vec, pk, st);
+  int dev = 0;
+  FWX_HIP(hipGetDevice(&dev));
+  const int knobs = (fused_dpanel() ? 1 : 0) | (rest_variant() << 1) | (band_narrow() ? 8 : 0);
+  const GraphKey key{d, n, tile, kappa, vec, pk, lookahead_enabled(), knobs, st, dev};
+  std::lock_guard<std::mutex> lk(g_graph_mu);
+  auto replay = [&](GraphEntry& e) -> int {
+    e.used = ++g_graph_clock;
+    FWX_HIP(hipGraphLaunch(e.exec, st));
+    return FWX_OK;
+  };
+  for (GraphEntry& e : g_graphs)
+    if (e.key == key) return replay(e);
+  // capture on a private stream: `st` may be the legacy default stream
+  hipStream_t cap;
+  FWX_HIP(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
+  hipError_t e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
+  int rc = FWX_OK;
+  if (e == hipSuccess) rc = enqueue_direct(d, n, tile, kappa, vec, pk, cap);
+  hipGraph_t graph = nullptr;
+  const hipError_t e2 = hipStreamEndCapture(cap, &graph);
+  (void)hipStreamDestroy(cap);
+  if (e != hipSuccess) return hip_fail(e, "hipStreamBeginCapture", __FILE__, __LINE__);
+  if (rc || e2 != hipSuccess) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return rc ? rc : hip_fail(e2, "hipStreamEndCapture", __FILE__, __LINE__);
+  }
+  hipGraphExec_t exec = nullptr;
+  e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    (void)hipGraphDestroy(graph);
+    return hip_fail(e, "hipGraphInstantiate", __FILE__, __LINE__);
+  }
+  if (g_graphs.size() >= kGraphCache) {
+    size_t lru = 0;
+    for (size_t i = 1; i < g_graphs.size(); ++i)
+      if (g_graphs[i].used < g_graphs[lru].used) lru = i;
+    (void)hipGraphExecDestroy(g_graphs[lru].exec);
+    (void)hipGraphDestroy(g_graphs[lru].graph);
+    g_graphs.erase(g_graphs.begin() + (long)lru);
+  }
+  g_graphs.push_back(GraphEntry{key, graph, exec, 0});
+  return replay(g_graphs.back());
 }
 
 }  // namespace fwx

@@ -5,6 +5,7 @@
 #include <stdint.h>
 
 #include <string>
+#include <vector>
 
 #include "../../include/fwx.h"
 
@@ -38,6 +39,7 @@ int enqueue_refshape(int32_t* d, int32_t n, hipStream_t st);
 
 // Event profiling (fwx_profile.hip): prof_open records a start event when
 // profiling is on (else leaves *a null); prof_close records the end event.
+// Profiled work must be launched directly, never captured into a graph.
 bool profiling();
 int prof_open(hipStream_t st, hipEvent_t* a);
 int prof_close(hipStream_t st, hipEvent_t a, int cls, double work);

@@ -2,9 +2,11 @@
 //
 // While enabled (fwx_profile_begin .. fwx_profile_end), the drivers bracket
 // each launch group of a class with a pair of events recorded on the stream
-// the kernels are launched on, and note the group's algorithmic work.  The
-// events are resolved at fwx_profile_end.  Off by default: a disabled
-// profiler costs one relaxed atomic load per launch group.
+// the kernels are launched on, and note the group's algorithmic work; the
+// pairs are resolved at fwx_profile_end.  Solves launch directly (no graph
+// replay) while profiling is on, so every event is a live stream event.
+// Off by default: a disabled profiler costs one relaxed atomic load per
+// launch group.
 #include <hip/hip_runtime.h>
 
 #include <atomic>
@@ -17,7 +19,7 @@ namespace fwx {
 
 namespace {
 
-struct Rec {
+struct ProfRec {
   int cls;
   hipEvent_t a, b;
   double work;
@@ -25,7 +27,18 @@ struct Rec {
 
 std::atomic<bool> g_on{false};
 std::mutex g_mu;
-std::vector<Rec> g_recs;
+std::vector<ProfRec> g_recs;
+double g_ms[FWX_PROF_NCLASS];
+double g_work[FWX_PROF_NCLASS];
+int64_t g_groups[FWX_PROF_NCLASS];
+
+void clear_totals() {
+  for (int c = 0; c < FWX_PROF_NCLASS; ++c) {
+    g_ms[c] = 0.0;
+    g_work[c] = 0.0;
+    g_groups[c] = 0;
+  }
+}
 
 }  // namespace
 
@@ -45,8 +58,16 @@ int prof_close(hipStream_t st, hipEvent_t a, int cls, double work) {
   FWX_HIP(hipEventCreate(&b));
   FWX_HIP(hipEventRecord(b, st));
   std::lock_guard<std::mutex> lk(g_mu);
-  g_recs.push_back(Rec{cls, a, b, work});
+  g_recs.push_back(ProfRec{cls, a, b, work});
   return FWX_OK;
+}
+
+static void prof_release(std::vector<ProfRec>& recs) {
+  for (ProfRec& r : recs) {
+    (void)hipEventDestroy(r.a);
+    (void)hipEventDestroy(r.b);
+  }
+  recs.clear();
 }
 
 }  // namespace fwx
@@ -58,11 +79,8 @@ extern "C" {
 int fwx_profile_begin(void) {
   clear_error();
   std::lock_guard<std::mutex> lk(g_mu);
-  for (Rec& r : g_recs) {
-    (void)hipEventDestroy(r.a);
-    (void)hipEventDestroy(r.b);
-  }
-  g_recs.clear();
+  prof_release(g_recs);
+  clear_totals();
   g_on.store(true);
   return FWX_OK;
 }
@@ -71,13 +89,8 @@ int fwx_profile_end(double* ms, double* work, int64_t* groups) {
   clear_error();
   g_on.store(false);
   std::lock_guard<std::mutex> lk(g_mu);
-  for (int c = 0; c < FWX_PROF_NCLASS; ++c) {
-    if (ms) ms[c] = 0.0;
-    if (work) work[c] = 0.0;
-    if (groups) groups[c] = 0;
-  }
   int rc = FWX_OK;
-  for (Rec& r : g_recs) {
+  for (ProfRec& r : g_recs) {
     float t = 0.f;
     if (rc == FWX_OK) {
       hipError_t e = hipEventSynchronize(r.b);
@@ -85,14 +98,18 @@ int fwx_profile_end(double* ms, double* work, int64_t* groups) {
       if (e != hipSuccess) rc = hip_fail(e, "hipEventElapsedTime", __FILE__, __LINE__);
     }
     if (r.cls >= 0 && r.cls < FWX_PROF_NCLASS) {
-      if (ms) ms[r.cls] += t;
-      if (work) work[r.cls] += r.work;
-      if (groups) groups[r.cls] += 1;
+      g_ms[r.cls] += t;
+      g_work[r.cls] += r.work;
+      g_groups[r.cls] += 1;
     }
-    (void)hipEventDestroy(r.a);
-    (void)hipEventDestroy(r.b);
   }
-  g_recs.clear();
+  prof_release(g_recs);
+  for (int c = 0; c < FWX_PROF_NCLASS; ++c) {
+    if (ms) ms[c] = g_ms[c];
+    if (work) work[c] = g_work[c];
+    if (groups) groups[c] = g_groups[c];
+  }
+  clear_totals();
   return rc;
 }
 

@@ -176,3 +176,61 @@ def test_config2_n8192_all_kappas():
         say("config2 tiling_gpu")
         fx.run_device("tiling_gpu", dm.ptr, 8192, 0, 0)
         assert (dm.download() == expect).all()
+
+
+@pytest.mark.slow
+def test_config3_n32768_spot_rows():
+    """n=32768 (4 GiB int32), density 0.1, weights U[1,1000], seed 7:
+    multi_layer_tiling_gpu (tuned) and tiling_gpu, 64 random rows bit-exact
+    against host Dijkstra over the INF-completed graph."""
+    n = 32768
+    g = fx.generate_graph(n, 0.1, 7, 1, 1000)
+    rows = np.random.default_rng(7).choice(n, 64, replace=False).astype(np.int32)
+    say("config3: dijkstra on", len(rows), "rows")
+    expect = po.dijkstra_rows(g, rows)
+    for algo, tile, kappa in (("multi_layer_tiling_gpu", 0, 0), ("tiling_gpu", 0, 0)):
+        with fx.DeviceMatrix(n) as dm:
+            dm.upload(g)
+            say("config3", algo)
+            fx.run_device(algo, dm.ptr, n, tile, kappa)
+            got = dm.download()
+        assert (got[rows] == expect).all(), algo
+        del got
+
+
+def test_graph_replay_cache(monkeypatch):
+    """Repeated solves on one buffer replay a captured hipGraph; every replay
+    must equal the oracle, whichever relaxation form (pk64 / min3) the input
+    selects, interleaved with direct (FWX_GRAPH=0) and profiled solves --
+    profiled solves launch directly and their totals add up to n^3 each."""
+    import ctypes
+    from fw_cuda_amd import _lib
+    n = 1000
+    g = fx.generate_graph(n, 0.5, 11, 1, 1000)
+    expect = po.fw_rowpar(g)
+    gneg = reweighted(np.random.default_rng(4), n, 0.5)
+    expect_neg = po.fw_rowpar(gneg)
+    with fx.DeviceMatrix(n) as dm:
+        def solve(src, exp, tag):
+            dm.upload(src)
+            fx.run_device("multi_layer_tiling_gpu", dm.ptr, n, 64, 2)
+            assert (dm.download() == exp).all(), tag
+
+        for graph in ("0", "1", "1", "1", "0", "1"):
+            monkeypatch.setenv("FWX_GRAPH", graph)
+            say("graph replay", graph)
+            solve(g, expect, "pk64 " + graph)
+            solve(gneg, expect_neg, "min3 " + graph)
+        L = _lib.lib()
+        _lib.check(L.fwx_profile_begin())
+        for _ in range(3):
+            solve(g, expect, "profiled")
+        ms = (ctypes.c_double * _lib.PROF_NCLASS)()
+        wk = (ctypes.c_double * _lib.PROF_NCLASS)()
+        grp = (ctypes.c_int64 * _lib.PROF_NCLASS)()
+        _lib.check(L.fwx_profile_end(ms, wk, grp))
+        # three solves' worth of rest + band + lead work = 3 n^3 relaxations
+        total = wk[_lib.PROF_REST] + wk[_lib.PROF_LEAD] + wk[_lib.PROF_BAND]
+        assert abs(total - 3 * float(n) ** 3) < 1e-6 * 3e9, total
+        assert ms[_lib.PROF_REST] > 0 and grp[_lib.PROF_REST] > 0
+        solve(g, expect, "replay after profiling")

@@ -52,35 +52,19 @@ __global__ __launch_bounds__(256) void k_batch_diag(int32_t* __restrict__ data,
   close_tile<T>(make_view(data + it.base, it.n, it.c0, it.c0));
 }
 
-template <int R, bool VEC, bool PK>
-__device__ __forceinline__ void batch_prod_body(int32_t* data, const ProdItem& it,
-                                                typename TileProduct<R, R, 32, VEC, 16, PK>::Smem& s) {
-  using P = TileProduct<R, R, 32, VEC, 16, PK>;
+template <int R, bool PK>
+__global__ __launch_bounds__(256) void k_batch_prod(int32_t* __restrict__ data,
+                                                    const ProdItem* __restrict__ items) {
+  __shared__ TileSmem<R, R, 32> s;
+  const ProdItem it = items[blockIdx.x];
   int32_t* g = data + it.base;
   const View A = make_view(g, it.n, it.ar, it.ac);
   const View B = make_view(g, it.n, it.br, it.bc);
   const View C = make_view(g, it.n, it.cr, it.cc);
-  int32_t acc[P::TM][P::TN];
-  P::load_c(C, acc);
-  P p;
-  p.run(s, A, B, it.depth, acc);
-  P::store_c(C, acc);
-}
-
-// One LDS allocation serves both instantiations (their Smem layouts are the
-// same; VEC only changes how global memory is read).
-template <int R, bool PK>
-__global__ __launch_bounds__(256) void k_batch_prod(int32_t* __restrict__ data,
-                                                    const ProdItem* __restrict__ items) {
-  using SV = typename TileProduct<R, R, 32, true, 16, PK>::Smem;
-  using SS = typename TileProduct<R, R, 32, false, 16, PK>::Smem;
-  static_assert(sizeof(SV) == sizeof(SS), "staging layouts must match");
-  __shared__ SV s;
-  const ProdItem it = items[blockIdx.x];
   if (it.vec)
-    batch_prod_body<R, true, PK>(data, it, s);
+    tile_update<R, R, 32, true, 16, PK>(s, A, B, C, it.depth);
   else
-    batch_prod_body<R, false, PK>(data, it, *reinterpret_cast<SS*>(&s));
+    tile_update<R, R, 32, false, 16, PK>(s, A, B, C, it.depth);
 }
 
 // ---------------------------------------------------------------------------

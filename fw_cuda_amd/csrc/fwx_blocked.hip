@@ -56,8 +56,7 @@ __global__ __launch_bounds__(256) void k_diag(int32_t* __restrict__ D, int32_t n
 template <int T, bool VEC, bool PK>
 __global__ __launch_bounds__(256) void k_panel(int32_t* __restrict__ D, int32_t n, int32_t c,
                                                int32_t N) {
-  using P = TileProduct<T, T, kChunk, VEC, 16, PK>;
-  __shared__ typename P::Smem s;
+  __shared__ TileSmem<T, T, kChunk> s;
   const int b = blockIdx.x;
   const int32_t base = c * T;
   const View diag = make_view(D, n, base, base);
@@ -74,11 +73,7 @@ __global__ __launch_bounds__(256) void k_panel(int32_t* __restrict__ D, int32_t 
     A = C;
     B = diag;
   }
-  int32_t acc[P::TM][P::TN];
-  P::load_c(C, acc);
-  P p;
-  p.run(s, A, B, T, acc);
-  P::store_c(C, acc);
+  tile_update<T, T, kChunk, VEC, 16, PK>(s, A, B, C, T);
 }
 
 // ---------------------------------------------------------------------------
@@ -95,8 +90,7 @@ __global__ __launch_bounds__(256) void k_dpanel(int32_t* __restrict__ D, int32_t
                                                 int32_t N) {
   close_tile<T>(make_view(D, n, c * T, c * T));
   __syncthreads();
-  using P = TileProduct<T, T, kChunk, VEC, 16, PK>;
-  __shared__ typename P::Smem s;
+  __shared__ TileSmem<T, T, kChunk> s;
   const int b = blockIdx.x;
   const int32_t base = c * T;
   const View diag = make_view(D, n, base, base);
@@ -113,11 +107,7 @@ __global__ __launch_bounds__(256) void k_dpanel(int32_t* __restrict__ D, int32_t
     A = C;
     B = diag;
   }
-  int32_t acc[P::TM][P::TN];
-  P::load_c(C, acc);
-  P p;
-  p.run(s, A, B, T, acc);
-  P::store_c(C, acc);
+  tile_update<T, T, kChunk, VEC, 16, PK>(s, A, B, C, T);
 }
 
 // ---------------------------------------------------------------------------
@@ -128,8 +118,7 @@ __global__ __launch_bounds__(256) void k_dpanel(int32_t* __restrict__ D, int32_t
 template <int T, bool VEC, bool PK>
 __global__ __launch_bounds__(256) void k_cross(int32_t* __restrict__ D, int32_t n, int32_t c,
                                                int32_t kb, int32_t Bst, int32_t N) {
-  using P = TileProduct<T, T, kChunk, VEC, 16, PK>;
-  __shared__ typename P::Smem s;
+  __shared__ TileSmem<T, T, kChunk> s;
   int b = blockIdx.x;
   const int l = c - kb;
   const int r1 = (Bst - 1) * (N - 1);
@@ -151,11 +140,7 @@ __global__ __launch_bounds__(256) void k_cross(int32_t* __restrict__ D, int32_t 
   const View A = make_view(D, n, i * T, c * T);
   const View B = make_view(D, n, c * T, j * T);
   const View C = make_view(D, n, i * T, j * T);
-  int32_t acc[P::TM][P::TN];
-  P::load_c(C, acc);
-  P p;
-  p.run(s, A, B, T, acc);
-  P::store_c(C, acc);
+  tile_update<T, T, kChunk, VEC, 16, PK>(s, A, B, C, T);
 }
 
 // ---------------------------------------------------------------------------
@@ -168,8 +153,7 @@ template <int R, bool VEC, bool PK, int KC = kChunk>
 __global__ __launch_bounds__(256, KC == 16 ? 3 : 2) void k_rest(int32_t* __restrict__ D, int32_t n,
                                                                         int32_t s0, int32_t S, int32_t sr0,
                                                                         int32_t srn, int32_t m, int32_t per) {
-  using P = TileProduct<R, R, KC, VEC, 16, PK>;
-  __shared__ typename P::Smem s;
+  __shared__ TileSmem<R, R, KC> s;
   const int b = blockIdx.x;
   const int L = (b & 7) * per + (b >> 3);
   if (L >= m * m) return;
@@ -185,11 +169,7 @@ __global__ __launch_bounds__(256, KC == 16 ? 3 : 2) void k_rest(int32_t* __restr
   const View A = make_view(D, n, I * R, s0);
   const View B = make_view(D, n, s0, J * R);
   const View C = make_view(D, n, I * R, J * R);
-  int32_t acc[P::TM][P::TN];
-  P::load_c(C, acc);
-  P p;
-  p.run(s, A, B, S, acc);
-  P::store_c(C, acc);
+  tile_update<R, R, KC, VEC, 16, PK>(s, A, B, C, S);
 }
 
 // 512-thread shape, held to 128 VGPRs so that two blocks (4 waves/SIMD)
@@ -198,8 +178,7 @@ template <int R, bool VEC, bool PK>
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_rest512(
     int32_t* __restrict__ D, int32_t n, int32_t s0, int32_t S, int32_t sr0, int32_t srn, int32_t m,
     int32_t per) {
-  using P = TileProduct<R, R, kChunk, VEC, 32, PK>;
-  __shared__ typename P::Smem s;
+  __shared__ TileSmem<R, R, kChunk> s;
   const int b = blockIdx.x;
   const int L = (b & 7) * per + (b >> 3);
   if (L >= m * m) return;
@@ -215,11 +194,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
   const View A = make_view(D, n, I * R, s0);
   const View B = make_view(D, n, s0, J * R);
   const View C = make_view(D, n, I * R, J * R);
-  int32_t acc[P::TM][P::TN];
-  P::load_c(C, acc);
-  P p;
-  p.run(s, A, B, S, acc);
-  P::store_c(C, acc);
+  tile_update<R, R, kChunk, VEC, 32, PK>(s, A, B, C, S);
 }
 
 // ---------------------------------------------------------------------------
@@ -232,8 +207,7 @@ template <int R, bool VEC, bool PK>
 __global__ __launch_bounds__(256) void k_rest_band(int32_t* __restrict__ D, int32_t n, int32_t s0,
                                                    int32_t S, int32_t e0, int32_t en, int32_t f0,
                                                    int32_t fn, int32_t m1) {
-  using P = TileProduct<R, R, kChunk, VEC, 16, PK>;
-  __shared__ typename P::Smem s;
+  __shared__ TileSmem<R, R, kChunk> s;
   int b = blockIdx.x;
   int I, J;
   if (b < fn * m1) {
@@ -249,11 +223,7 @@ __global__ __launch_bounds__(256) void k_rest_band(int32_t* __restrict__ D, int3
   const View A = make_view(D, n, I * R, s0);
   const View B = make_view(D, n, s0, J * R);
   const View C = make_view(D, n, I * R, J * R);
-  int32_t acc[P::TM][P::TN];
-  P::load_c(C, acc);
-  P p;
-  p.run(s, A, B, S, acc);
-  P::store_c(C, acc);
+  tile_update<R, R, kChunk, VEC, 16, PK>(s, A, B, C, S);
 }
 
 // ---------------------------------------------------------------------------

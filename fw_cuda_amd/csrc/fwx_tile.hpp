@@ -21,6 +21,8 @@
 // outside the matrix are never stored, so ragged n needs no padding buffer.
 #pragma once
 
+#include <type_traits>
+
 #include "fwx_device.hpp"
 
 // Unroll of the k-quad loop inside a chunk (register pressure vs. LDS
@@ -47,10 +49,32 @@ __device__ __forceinline__ View make_view(int32_t* D, int32_t n, int32_t r0, int
   return View{D + ((int64_t)r0 * n + c0), n, n - r0, n - c0};
 }
 
+// base + an unsigned 32-bit byte offset.  With a uniform base this is the
+// shape the global load/store "saddr" form takes (64-bit base in SGPRs, one
+// 32-bit VGPR offset), instead of a 64-bit VGPR address per access.  Every
+// per-thread offset used with it is below 2^32 bytes (a window row of at most
+// 32769 int32 cells times at most 128 rows).
+template <class T>
+__device__ __forceinline__ T* at_bytes(T* base, uint32_t bytes) {
+  using C = typename std::conditional<std::is_const<T>::value, const char, char>::type;
+  return reinterpret_cast<T*>(reinterpret_cast<C*>(base) + bytes);
+}
+
+// LDS staging of one product: two buffers of an A chunk (row-major, row
+// stride KC+4) and a B chunk (pair-interleaved).
+template <int BM, int BN, int KC>
+struct TileSmem {
+  int32_t a[2][BM * (KC + 4)];
+  int32_t b[2][(KC / 2) * BN * 2];
+};
+
 // VEC: 16-byte global loads (needs ld % 4 == 0, 16-byte aligned base and
 // c0 % 4 == 0, so a 4-wide group is wholly in or wholly out of range).
 // NTX threads across columns x 16 across rows: 256 (NTX=16) or 512 (NTX=32).
-template <int BM, int BN, int KC, bool VEC, int NTX, bool PK64>
+// EDGE=false: the caller guarantees that the C block and every operand cell
+// of the whole depth lie inside the matrix (tile_update checks it), so the
+// clamping and masking code is compiled out.
+template <int BM, int BN, int KC, bool VEC, int NTX, bool PK64, bool EDGE = true>
 struct TileProduct {
   static constexpr int NT = NTX * 16;
   static constexpr int NW = NT / 64;  // waves
@@ -76,10 +100,7 @@ struct TileProduct {
   static constexpr int BU = (KC / 2) * (BN / 4);
   static constexpr int BPASS = (BU + NT - 1) / NT;
 
-  struct Smem {
-    int32_t a[2][BM * LDA];
-    int32_t b[2][(KC / 2) * BN * 2];
-  };
+  using Smem = TileSmem<BM, BN, KC>;
 
   int4 ra[APASS];
   int4 rb0[BPASS], rb1[BPASS];
@@ -96,20 +117,27 @@ struct TileProduct {
     const int tid = threadIdx.x;
     static_assert(NT % (BN / 4) == 0, "B staging rows must be uniform per pass");
     constexpr int BROWS = NT / (BN / 4);  // k pairs per B pass
-    full_ = A.rlim >= BM && A.clim >= kc0 + KC && B.rlim >= kc0 + KC && B.clim >= BN;
+    full_ = !EDGE || (A.rlim >= BM && A.clim >= kc0 + KC && B.rlim >= kc0 + KC && B.clim >= BN);
     kc_ = kc0;
     if (VEC) {
       const int32_t* qa[APASS];
       const int32_t* qb[BPASS];
       if (full_) {
-        const int32_t* pa = A.p + kc0;
-        const int32_t oa = (tid / AG) * A.ld + (tid % AG) * 4;
+        const uint32_t oa = (uint32_t)((tid / AG) * A.ld + (tid % AG) * 4) * 4u;
 #pragma unroll
-        for (int p = 0; p < APASS; ++p) qa[p] = pa + (p * AROWS) * A.ld + oa;
-        const int32_t* pb = B.p + kc0 * B.ld;
-        const int32_t ob = (tid / (BN / 4)) * 2 * B.ld + (tid % (BN / 4)) * 4;
+        for (int p = 0; p < APASS; ++p)
+          if (BM % AROWS == 0 || p * AROWS + tid / AG < BM)
+            ra[p] = *at_bytes(reinterpret_cast<const int4*>(A.p + kc0 + (p * AROWS) * A.ld), oa);
+        const uint32_t ob = (uint32_t)((tid / (BN / 4)) * 2 * B.ld + (tid % (BN / 4)) * 4) * 4u;
 #pragma unroll
-        for (int u = 0; u < BPASS; ++u) qb[u] = pb + (u * BROWS * 2) * B.ld + ob;
+        for (int u = 0; u < BPASS; ++u) {
+          if (BU % NT == 0 || u * NT + tid < BU) {
+            const int32_t* pb = B.p + (kc0 + u * BROWS * 2) * B.ld;
+            rb0[u] = *at_bytes(reinterpret_cast<const int4*>(pb), ob);
+            rb1[u] = *at_bytes(reinterpret_cast<const int4*>(pb + B.ld), ob);
+          }
+        }
+        return;
       } else {
 #pragma unroll
         for (int p = 0; p < APASS; ++p) {
@@ -282,19 +310,19 @@ struct TileProduct {
   }
 
   __device__ __forceinline__ static bool full(const View& C) {
-    return C.rlim >= BM && C.clim >= BN;
+    return !EDGE || (C.rlim >= BM && C.clim >= BN);
   }
 
   __device__ __forceinline__ static void load_c(const View& C, int32_t (&acc)[TM][TN]) {
     const int tx = threadIdx.x % NTX;
     const int ty = threadIdx.x / NTX;
     if (full(C)) {
-      const int32_t oc = ty * C.ld + G * tx;
+      const uint32_t oc = (uint32_t)(ty * C.ld + G * tx) * 4u;
 #pragma unroll
       for (int r = 0; r < TM; ++r) {
         const int32_t* pr = C.p + (16 * r) * C.ld;
 #pragma unroll
-        for (int c = 0; c < TN; ++c) acc[r][c] = pr[oc + colmap(0, c)];
+        for (int c = 0; c < TN; ++c) acc[r][c] = *at_bytes(pr + colmap(0, c), oc);
       }
     } else {
 #pragma unroll
@@ -305,16 +333,22 @@ struct TileProduct {
     }
   }
 
+  // The thread index is passed through an empty asm so that the store
+  // addresses are recomputed here rather than shared with load_c: shared,
+  // they would hold one 64-bit VGPR address per row across the whole depth
+  // loop (16 VGPRs at 8 rows).
   __device__ __forceinline__ static void store_c(const View& C, const int32_t (&acc)[TM][TN]) {
-    const int tx = threadIdx.x % NTX;
-    const int ty = threadIdx.x / NTX;
+    uint32_t tid = threadIdx.x;
+    asm("" : "+v"(tid));
+    const int tx = tid % NTX;
+    const int ty = tid / NTX;
     if (full(C)) {
-      const int32_t oc = ty * C.ld + G * tx;
+      const uint32_t oc = (uint32_t)(ty * C.ld + G * tx) * 4u;
 #pragma unroll
       for (int r = 0; r < TM; ++r) {
         int32_t* pr = C.p + (16 * r) * C.ld;
 #pragma unroll
-        for (int c = 0; c < TN; ++c) pr[oc + colmap(0, c)] = acc[r][c];
+        for (int c = 0; c < TN; ++c) *at_bytes(pr + colmap(0, c), oc) = acc[r][c];
       }
     } else {
 #pragma unroll
@@ -325,6 +359,30 @@ struct TileProduct {
     }
   }
 };
+
+// C <- min(C, A (x) B) over k in [0, depth) by one workgroup.  Blocks whose
+// C block and operands lie wholly inside the matrix (every block of an
+// n % BM == 0 matrix) run the edge-free instantiation, the others the
+// general one; the branch is uniform across the block.
+template <int BM, int BN, int KC, bool VEC, int NTX, bool PK64>
+__device__ __forceinline__ void tile_update(TileSmem<BM, BN, KC>& s, const View& A, const View& B,
+                                            const View& C, int32_t depth) {
+  using PF = TileProduct<BM, BN, KC, VEC, NTX, PK64, false>;
+  using PE = TileProduct<BM, BN, KC, VEC, NTX, PK64, true>;
+  int32_t acc[PF::TM][PF::TN];
+  if (C.rlim >= BM && C.clim >= BN && A.rlim >= BM && A.clim >= depth && B.rlim >= depth &&
+      B.clim >= BN) {
+    PF::load_c(C, acc);
+    PF p;
+    p.run(s, A, B, depth, acc);
+    PF::store_c(C, acc);
+  } else {
+    PE::load_c(C, acc);
+    PE p;
+    p.run(s, A, B, depth, acc);
+    PE::store_c(C, acc);
+  }
+}
 
 // Close a T x T diagonal tile in place (Floyd-Warshall inside the tile), one
 // 256-thread block.  Thread (tx, ty) keeps cells (ty+16r, tx+16q) in VGPRs.

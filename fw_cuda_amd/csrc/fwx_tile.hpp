@@ -385,81 +385,92 @@ __device__ __forceinline__ void tile_update(TileSmem<BM, BN, KC>& s, const View&
 }
 
 // Close a T x T diagonal tile in place (Floyd-Warshall inside the tile), one
-// 256-thread block.  Thread (tx, ty) keeps cells (ty+16r, tx+16q) in VGPRs.
-// Two k steps per barrier: the owners of rows k, k+1 and columns k, k+1
-// publish them to a double-buffered LDS line; every thread then forms
+// workgroup of NTH threads (16 columns x RS = NTH/16 rows of threads).
+// Thread (tx, ty) keeps cells (ty + RS r, tx + 16 q) in VGPRs.  Two k steps
+// per barrier: the owners of rows k, k+1 and columns k, k+1 publish them to
+// a double-buffered LDS line; every thread then forms
 //   col'[i] = min(D[i][k+1], D[i][k] + D[k][k+1])   (column k+1 after step k)
 //   row'[j] = min(D[k+1][j], D[k+1][k] + D[k][j])   (row k+1 after step k)
 //   D[i][j] = min3(D[i][j], D[i][k] + D[k][j], col'[i] + row'[j])
 // which is exactly steps k and k+1 of the sequential loop.  Line buffer p&1
 // is rewritten at pair p+2, after every thread passed barrier p+1, so one
 // barrier per pair suffices.  Steps with k >= n (padding) are skipped, so
-// padded vertices are never intermediates.
-template <int T>
+// padded vertices are never intermediates.  More threads mean fewer cells
+// per thread and more waves per SIMD to hide the barrier and LDS latency of
+// the T dependent steps.
+template <int T, int NTH = 256>
 __device__ __forceinline__ void close_tile(const View V) {
-  constexpr int TM = T / 16;
+  constexpr int RS = NTH / 16;  // thread rows
+  constexpr int TR = T / RS;    // register rows per thread
+  constexpr int TQ = T / 16;    // register columns per thread
+  static_assert(NTH % 32 == 0 && T % RS == 0 && RS % 16 == 0, "layout must tile T");
   __shared__ int32_t rows[2][2][T];  // [buf][k, k+1][j]
   __shared__ int32_t cols[2][2][T];  // [buf][k, k+1][i]
   const int tx = threadIdx.x & 15;
   const int ty = threadIdx.x >> 4;
-  int32_t acc[TM][TM];
+  int32_t acc[TR][TQ];
 #pragma unroll
-  for (int r = 0; r < TM; ++r)
+  for (int r = 0; r < TR; ++r)
 #pragma unroll
-    for (int q = 0; q < TM; ++q)
-      acc[r][q] = V.in(ty + 16 * r, tx + 16 * q) ? *V.at(ty + 16 * r, tx + 16 * q) : kInf;
+    for (int q = 0; q < TQ; ++q)
+      acc[r][q] = V.in(ty + RS * r, tx + 16 * q) ? *V.at(ty + RS * r, tx + 16 * q) : kInf;
   const int kmax = min(T, V.rlim);
 #pragma unroll
-  for (int kk = 0; kk < TM; ++kk) {
+  for (int qk = 0; qk < TQ; ++qk) {
+    // steps k = 16 qk + k16: column k is register column qk of threads
+    // tx = k16, row k is register row rk of threads ty = k % RS (RS is a
+    // multiple of 16, so rk is the same for the 16 steps)
+    constexpr int RPQ = RS / 16;  // register-column blocks per register row
+    const int rk = qk / RPQ;
     for (int k16 = 0; k16 < 16; k16 += 2) {
-      const int k = kk * 16 + k16;
+      const int k = qk * 16 + k16;
       if (k >= kmax) break;
       const int b = (k >> 1) & 1;
-      // rows k and k+1 live in the same register row kk (k16 even)
-      if (ty == k16 || ty == k16 + 1) {
-        const int h = ty - k16;
+      const int tyk = k % RS;
+      if (ty == tyk || ty == tyk + 1) {
+        const int h = ty - tyk;
 #pragma unroll
-        for (int q = 0; q < TM; ++q) rows[b][h][tx + 16 * q] = acc[kk][q];
+        for (int q = 0; q < TQ; ++q) rows[b][h][tx + 16 * q] = acc[rk][q];
       }
       if (tx == k16 || tx == k16 + 1) {
         const int h = tx - k16;
 #pragma unroll
-        for (int r = 0; r < TM; ++r) cols[b][h][ty + 16 * r] = acc[r][kk];
+        for (int r = 0; r < TR; ++r) cols[b][h][ty + RS * r] = acc[r][qk];
       }
       __syncthreads();
       if (k + 1 < kmax) {
         const int32_t dkk1 = rows[b][0][k + 1];  // D[k][k+1]
         const int32_t dk1k = rows[b][1][k];      // D[k+1][k]
-        int32_t rk[TM], rk1[TM], ck[TM], ck1[TM];
+        int32_t rw[TQ], rw1[TQ], cl[TR], cl1[TR];
 #pragma unroll
-        for (int q = 0; q < TM; ++q) {
-          rk[q] = rows[b][0][tx + 16 * q];
-          rk1[q] = min(rows[b][1][tx + 16 * q], wadd(dk1k, rk[q]));
+        for (int q = 0; q < TQ; ++q) {
+          rw[q] = rows[b][0][tx + 16 * q];
+          rw1[q] = min(rows[b][1][tx + 16 * q], wadd(dk1k, rw[q]));
         }
 #pragma unroll
-        for (int r = 0; r < TM; ++r) {
-          ck[r] = cols[b][0][ty + 16 * r];
-          ck1[r] = min(cols[b][1][ty + 16 * r], wadd(ck[r], dkk1));
+        for (int r = 0; r < TR; ++r) {
+          cl[r] = cols[b][0][ty + RS * r];
+          cl1[r] = min(cols[b][1][ty + RS * r], wadd(cl[r], dkk1));
         }
 #pragma unroll
-        for (int r = 0; r < TM; ++r)
+        for (int r = 0; r < TR; ++r)
 #pragma unroll
-          for (int q = 0; q < TM; ++q) acc[r][q] = vmin3(acc[r][q], wadd(ck[r], rk[q]), wadd(ck1[r], rk1[q]));
+          for (int q = 0; q < TQ; ++q) acc[r][q] = vmin3(acc[r][q], wadd(cl[r], rw[q]), wadd(cl1[r], rw1[q]));
       } else {
 #pragma unroll
-        for (int r = 0; r < TM; ++r) {
-          const int32_t ckr = cols[b][0][ty + 16 * r];
+        for (int r = 0; r < TR; ++r) {
+          const int32_t clr = cols[b][0][ty + RS * r];
 #pragma unroll
-          for (int q = 0; q < TM; ++q) acc[r][q] = min(acc[r][q], wadd(ckr, rows[b][0][tx + 16 * q]));
+          for (int q = 0; q < TQ; ++q) acc[r][q] = min(acc[r][q], wadd(clr, rows[b][0][tx + 16 * q]));
         }
       }
     }
   }
 #pragma unroll
-  for (int r = 0; r < TM; ++r)
+  for (int r = 0; r < TR; ++r)
 #pragma unroll
-    for (int q = 0; q < TM; ++q)
-      if (V.in(ty + 16 * r, tx + 16 * q)) *V.at(ty + 16 * r, tx + 16 * q) = acc[r][q];
+    for (int q = 0; q < TQ; ++q)
+      if (V.in(ty + RS * r, tx + 16 * q)) *V.at(ty + RS * r, tx + 16 * q) = acc[r][q];
 }
 
 }  // namespace fwx

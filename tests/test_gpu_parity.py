@@ -140,6 +140,21 @@ def test_schedules_agree(monkeypatch, lookahead):
         assert (gpu(g, "multi_layer_tiling_gpu", t, k) == po.fw_rowpar(g)).all(), (n, t, k)
 
 
+@pytest.mark.parametrize("nth", ["256", "512", "1024"])
+def test_diag_block_sizes(monkeypatch, nth):
+    """The diagonal closure with 256, 512 and 1024 threads per block (tile
+    64 and 128), on ragged sizes and negative weights."""
+    monkeypatch.setenv("FWX_DIAG_NTH", nth)
+    rng = np.random.default_rng(21)
+    for n in (1, 37, 64, 127, 128, 200, 385):
+        g = fx.generate_graph(n, float(rng.uniform(0.1, 0.9)), int(rng.integers(1, 2**31)), 1, 1000)
+        gneg = reweighted(rng, n, 0.4)
+        for t, k in ((64, 1), (64, 3), (128, 1), (128, 2)):
+            say("diag threads", nth, n, t, k)
+            assert (gpu(g, "multi_layer_tiling_gpu", t, k) == po.fw_rowpar(g)).all(), (n, t, k)
+            assert (gpu(gneg, "multi_layer_tiling_gpu", t, k) == po.fw_rowpar(gneg)).all(), (n, t, k)
+
+
 def test_config1_n1024():
     """n=1024, density 0.9, weights U[1,1000], seed 42: every algorithm."""
     g = fx.generate_graph(1024, 0.9, 42, 1, 1000)

@@ -37,7 +37,7 @@ def parse():
     ap.add_argument("--algo", default="multi_layer_tiling_gpu")
     ap.add_argument("--tile", type=int, default=0)
     ap.add_argument("--kappa", type=int, default=0)
-    ap.add_argument("--cpu-n", type=int, default=2048)
+    ap.add_argument("--cpu-n", type=int, default=4096)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-compare", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
@@ -118,6 +118,32 @@ def bench_diff(steps):
     return out
 
 
+def bench_baseline(L, work, src, n, stream):
+    """baseline_gpu (one HBM sweep per k) once on the workload, each per-k
+    launch bracketed by HIP events: n^3/s and the HBM read rate of the sweep
+    kernel (4 n^2 bytes read per launch; improved cells are written on top)."""
+    import ctypes
+    import torch
+    import fw_cuda_amd as fx
+    from fw_cuda_amd import _lib
+    work.copy_(src)
+    torch.cuda.synchronize()
+    _lib.check(L.fwx_profile_begin())
+    c0 = time.perf_counter()
+    fx.run_device("baseline_gpu", _lib.ptr(work), n, 0, 0, stream)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - c0
+    ms = (ctypes.c_double * _lib.PROF_NCLASS)()
+    wk = (ctypes.c_double * _lib.PROF_NCLASS)()
+    grp = (ctypes.c_int64 * _lib.PROF_NCLASS)()
+    _lib.check(L.fwx_profile_end(ms, wk, grp))
+    k_ms, k_bytes, k_n = ms[_lib.PROF_BASELINE], wk[_lib.PROF_BASELINE], grp[_lib.PROF_BASELINE]
+    gbps = k_bytes / (k_ms * 1e-3) / 1e9 if k_ms > 0 else 0.0
+    return {"n3_per_s": float(n) ** 3 / dt, "seconds": dt, "kernel": "k_baseline_step",
+            "launches": int(k_n), "avg_launch_us": k_ms * 1e3 / max(1, k_n),
+            "hbm_read_GBps": gbps, "frac_of_8TBps": gbps / 8000.0}
+
+
 def cpu_baseline(n):
     """The reference's fwBaselineCPU (oracle/_ref, built from the reference's
     own sources) or, if that is absent, the oracle's restatement: one cold
@@ -146,12 +172,27 @@ def cpu_baseline(n):
                       f"{dt:.2f} s on 1 thread of {cpu} (nproc={os.cpu_count()})"}
 
 
-def traffic_from_profiles():
+def traffic_from_profiles(n, tile, kappa):
+    """HBM bytes per rest-kernel launch from the committed rocprofv3 FETCH_SIZE
+    / WRITE_SIZE runs (calibrated against a copy kernel), if they were taken
+    on this configuration; else None."""
     p = os.path.join(ROOT, "profiles", "rest_kernel_traffic.json")
     if not os.path.exists(p):
         return None
     with open(p) as f:
-        return json.load(f).get("bytes_per_launch")
+        t = json.load(f)
+    if (t.get("n"), t.get("tile"), t.get("kappa")) != (n, tile, kappa):
+        return None
+    return t.get("bytes_per_launch")
+
+
+def rest_kernel_name(tile, kappa):
+    """The rest-pass kernel the library launches for this configuration: block
+    R = 128 if the stage width kappa*tile is a multiple of 128, else 64, else
+    32; R = 128 runs the 512-thread kernel."""
+    span = tile * kappa
+    R = 128 if span % 128 == 0 else 64 if span % 64 == 0 else 32
+    return "k_rest512<128>" if R == 128 else f"k_rest<{R}>"
 
 
 def main():
@@ -225,6 +266,7 @@ def main():
             torch.cuda.synchronize()
             dt = time.perf_counter() - c0
             compare[name] = {"n3_per_s": float(n) ** 3 / dt, "seconds": dt}
+        compare["baseline_gpu"] = bench_baseline(L, work, src, n, stream)
 
     out = {
         "metric": METRIC,
@@ -243,9 +285,9 @@ def main():
                                f"{a.algo} tile={tile} kappa={kappa}",
                    "n": n, "tile": tile, "kappa": kappa, "algorithm": a.algo},
         "pct_valu_minplus_peak": 100.0 * value / PEAK_2OP,
-        "roofline": {"bound": "valu", "kernel": "k_rest", "achieved": achieved, "peak": PEAK_2OP,
-                     "unit": "relaxations/s", "frac": achieved / PEAK_2OP,
-                     "traffic": traffic_from_profiles(),
+        "roofline": {"bound": "valu", "kernel": rest_kernel_name(tile, kappa), "achieved": achieved,
+                     "peak": PEAK_2OP, "unit": "relaxations/s", "frac": achieved / PEAK_2OP,
+                     "traffic": traffic_from_profiles(n, tile, kappa),
                      "launches": int(rest_launches), "avg_launch_ms": rest_ms / max(1, rest_launches),
                      "work_per_step": rest_work / a.steps,
                      "share_of_step": rest_ms / a.steps / ms_per_step,

@@ -11,7 +11,10 @@ Writes
   fw_cuda_amd/csrc/fwx_tune_table.inc   -- the table compiled into libfwx
   profiles/autotune_mi355x.json          -- every timing behind it
 
-Run on the GPU, then rebuild:  python -m fw_cuda_amd.autotune [--max-n 8192]
+The last size class tuned also covers every larger n up to MAX_N = 32769
+(include/constants.h:4).
+
+Run on the GPU, then rebuild:  python -m fw_cuda_amd.autotune [--max-n 16384]
 """
 import argparse
 import json
@@ -25,7 +28,8 @@ from . import DeviceMatrix, generate_graph, run_device
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 
-CLASSES = [64, 128, 256, 512, 1024, 2048, 4096, 8192, 16384, 32768]
+CLASSES = [64, 128, 256, 512, 1024, 2048, 4096, 8192, 16384]
+MAX_N = 32769
 TILES = (32, 64, 128)
 KAPPAS = (1, 2, 3, 4, 6, 8, 12, 16)
 
@@ -55,7 +59,7 @@ def time_config(dm, g, n, t, k, reps):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--max-n", type=int, default=8192)
+    ap.add_argument("--max-n", type=int, default=16384)
     ap.add_argument("--out-inc", default=os.path.join(HERE, "csrc", "fwx_tune_table.inc"))
     ap.add_argument("--out-json", default=os.path.join(ROOT, "profiles", "autotune_mi355x.json"))
     a = ap.parse_args()
@@ -77,6 +81,8 @@ def main():
         table.append((n, t, k))
         record[str(n)] = {"best": best, "seconds": res, "n3_per_s": n ** 3 / res[best]}
         print(f"n<={n}: tile={t} kappa={k} ({n ** 3 / res[best]:.3e} n^3/s)", flush=True)
+    if table and table[-1][0] < MAX_N:
+        table.append((MAX_N, table[-1][1], table[-1][2]))
     with open(a.out_inc, "w") as f:
         f.write("// Generated by fw_cuda_amd/autotune.py on the target GPU: {n_max, tile, kappa}.\n")
         for n, t, k in table:

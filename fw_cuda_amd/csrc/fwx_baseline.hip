@@ -47,7 +47,7 @@ int enqueue_baseline(int32_t* d, int32_t n, hipStream_t st) {
   const bool vec = (n % 4 == 0) && ((reinterpret_cast<uintptr_t>(d) & 15u) == 0);
   const dim3 block(64, 4);
   const dim3 grid((n + 255) / 256, (n + 3) / 4);
-  const double bytes = 8.0 * (double)n * (double)n;
+  const double bytes = 4.0 * (double)n * (double)n;  // every cell is read once per k
   for (int32_t k = 0; k < n; ++k) {
     hipEvent_t ev;
     if (int rc = prof_open(st, &ev)) return rc;

@@ -162,8 +162,9 @@ int fwx_diff_backward(const float* d_W, const float* d_D, const int32_t* d_P, co
  * Between fwx_profile_begin() and fwx_profile_end(), every launch group is
  * bracketed by HIP events on the stream it runs on.  fwx_profile_end fills
  * arrays of FWX_PROF_NCLASS entries (any may be NULL): summed event time in
- * ms, summed algorithmic work (relaxations for REST/LEAD, bytes for
- * BASELINE: 8 per cell per k), and the number of groups.  The reference
+ * ms, summed algorithmic work (relaxations for REST/LEAD/BAND; bytes for
+ * BASELINE: the 4 n^2 bytes every per-k sweep must read -- improved cells
+ * are written on top of that), and the number of groups.  The reference
  * times with cudaEvent pairs around a whole call (src/timer.cu:48-63).
  */
 #define FWX_PROF_REST 0     /* one k_rest launch per stage */

@@ -183,6 +183,118 @@ int oracle_dijkstra_rows(const int32_t* W, int32_t n, const int32_t* src, int32_
 }
 
 /*
+ * oracle_certify_rows -- exact check of claimed distance rows without solving
+ * anything: an optimality certificate of single-source shortest paths on the
+ * INF-completed graph (every off-diagonal cell an edge, as in
+ * src/fw_baseline_cpu.cpp:9).  Requires a zero diagonal and every
+ * off-diagonal weight in [1, INF] (returns -1 otherwise).  Row q claims
+ * d = dist(s, .) for s = src[q]; it passes iff
+ *   (a) d[s] == 0 and every d[v] lies in [0, INF], and
+ *   (b) d[v] == min over u != v of (d[u] + W[u][v])  for every v != s.
+ * (b) gives d[v] <= d[u] + W[u][v] on every edge, hence d <= dist along any
+ * shortest path from d[s] = 0; and since weights are >= 1, the argmin u of
+ * (b) has d[u] < d[v], so following argmins from v strictly decreases d and
+ * can only stop at s: a real walk of weight d[v], hence d >= dist.  So a row
+ * passes exactly when it is the shortest-distance row.  Cost n^2 per row, as
+ * Dijkstra, but as blocked min-plus sweeps that vectorise; used for spot
+ * checks of many rows at the largest sizes.  Writes ok[q] = 1 / 0; returns
+ * the number of failing rows (-1: weights outside the domain, -2: no memory).
+ */
+#define CERT_SB 32   /* sources per block */
+#define CERT_VB 2048 /* columns per block */
+__attribute__((target_clones("avx2", "default")))
+static void cert_block(const int32_t* W, size_t N, const int32_t* const* d, int nq, size_t v0, size_t v1,
+                       int32_t* x /* nq x (v1 - v0) */) {
+  const size_t w = v1 - v0;
+  for (int q = 0; q < nq; ++q)
+    for (size_t v = 0; v < w; ++v) x[(size_t)q * w + v] = INT32_MAX;
+  for (size_t u = 0; u < N; ++u) {
+    const int32_t* wu = W + u * N + v0;
+    for (int q = 0; q < nq; ++q) {
+      const int32_t a = d[q][u];
+      int32_t* xq = x + (size_t)q * w;
+      /* u == v is excluded: split the column range around it */
+      const size_t cut = (u >= v0 && u < v1) ? u - v0 : w;
+      for (size_t v = 0; v < cut; ++v) {
+        const int32_t t = a + wu[v];
+        xq[v] = t < xq[v] ? t : xq[v];
+      }
+      for (size_t v = cut + 1; v < w; ++v) {
+        const int32_t t = a + wu[v];
+        xq[v] = t < xq[v] ? t : xq[v];
+      }
+    }
+  }
+}
+
+int oracle_certify_rows(const int32_t* W, int32_t n, const int32_t* src, int32_t nsrc,
+                        const int32_t* rows, int32_t* ok, int32_t threads) {
+  const size_t N = (size_t)n;
+  for (size_t u = 0; u < N; ++u)
+    for (size_t v = 0; v < N; ++v) {
+      const int32_t wv = W[u * N + v];
+      if (u == v ? wv != 0 : (wv < 1 || wv > ORACLE_INF)) return -1;
+    }
+  /* (a); rows failing it are excluded from the sweeps (their sums could wrap) */
+  for (int32_t q = 0; q < nsrc; ++q) {
+    const int32_t* d = rows + (size_t)q * N;
+    int good = d[src[q]] == 0;
+    for (size_t v = 0; v < N && good; ++v) good = d[v] >= 0 && d[v] <= ORACLE_INF;
+    ok[q] = good;
+  }
+  int nomem = 0;
+  const int nsb = (nsrc + CERT_SB - 1) / CERT_SB;
+  const int nvb = (int)((N + CERT_VB - 1) / CERT_VB);
+#ifdef _OPENMP
+  if (threads <= 0) threads = omp_get_max_threads();
+#else
+  (void)threads;
+#endif
+#pragma omp parallel num_threads(threads)
+  {
+    int32_t* x = (int32_t*)malloc((size_t)CERT_SB * CERT_VB * sizeof(int32_t));
+    const int32_t* d[CERT_SB];
+    if (!x) {
+#pragma omp atomic write
+      nomem = 1;
+    }
+#pragma omp for schedule(dynamic, 1) collapse(2)
+    for (int sb = 0; sb < nsb; ++sb)
+      for (int vb = 0; vb < nvb; ++vb) {
+        const int q0 = sb * CERT_SB;
+        const int nq = (nsrc - q0) < CERT_SB ? (nsrc - q0) : CERT_SB;
+        int any = 0;
+        for (int q = 0; q < nq; ++q) {
+          d[q] = rows + (size_t)(q0 + q) * N;
+          int okq;
+#pragma omp atomic read
+          okq = ok[q0 + q];
+          any |= okq;
+        }
+        if (!any || !x) continue;
+        const size_t v0 = (size_t)vb * CERT_VB;
+        const size_t v1 = v0 + CERT_VB < N ? v0 + CERT_VB : N;
+        cert_block(W, N, d, nq, v0, v1, x);
+        for (int q = 0; q < nq; ++q) {
+          const size_t s = (size_t)src[q0 + q];
+          int good = 1;
+          for (size_t v = v0; v < v1 && good; ++v)
+            if (v != s && x[(size_t)q * (v1 - v0) + (v - v0)] != d[q][v]) good = 0;
+          if (!good) {
+#pragma omp atomic write
+            ok[q0 + q] = 0;
+          }
+        }
+      }
+    free(x);
+  }
+  if (nomem) return -2;
+  int bad = 0;
+  for (int32_t q = 0; q < nsrc; ++q) bad += !ok[q];
+  return bad;
+}
+
+/*
  * oracle_brute_walks -- shortest walks of at most n-1 edges by repeated
  * min-plus squaring of (W with a zero diagonal) -- no FW ordering at all.
  * Used on tiny graphs only (O(n^3 log n)).  Valid when there is no negative

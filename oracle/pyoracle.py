@@ -42,6 +42,9 @@ def lib():
         _lib.oracle_dijkstra_rows.argtypes = [_i32p, ctypes.c_int32, _i32p, ctypes.c_int32,
                                               _i32p, ctypes.c_int32]
         _lib.oracle_dijkstra_rows.restype = ctypes.c_int
+        _lib.oracle_certify_rows.argtypes = [_i32p, ctypes.c_int32, _i32p, ctypes.c_int32, _i32p, _i32p,
+                                             ctypes.c_int32]
+        _lib.oracle_certify_rows.restype = ctypes.c_int
         _lib.oracle_brute_walks.argtypes = [_i32p, ctypes.c_int32, _i32p]
         _lib.oracle_fw_f32_pred.argtypes = [_f32p, _f32p, _i32p, ctypes.c_int32]
         _lib.oracle_grad_paths.argtypes = [_i32p, _f32p, _f32p, ctypes.c_int32]
@@ -103,6 +106,21 @@ def dijkstra_rows(W, rows, threads=0):
     if rc != 0:
         raise ValueError(f"oracle_dijkstra_rows failed ({rc}): negative weights?")
     return out
+
+
+def certify_rows(W, rows, D_rows, threads=0):
+    """oracle_certify_rows: per-row True iff D_rows[q] is exactly the shortest
+    distance row from source rows[q] (weights must be in [1, INF], diagonal 0)."""
+    W, n = _sq(W)
+    rows = np.ascontiguousarray(rows, dtype=np.int32)
+    D_rows = np.ascontiguousarray(D_rows, dtype=np.int32)
+    assert D_rows.shape == (rows.size, n)
+    ok = np.zeros(rows.size, dtype=np.int32)
+    rc = lib().oracle_certify_rows(_ptr(W, _i32p), n, _ptr(rows, _i32p), rows.size, _ptr(D_rows, _i32p),
+                                   _ptr(ok, _i32p), int(threads))
+    if rc < 0:
+        raise ValueError(f"oracle_certify_rows failed ({rc}): weights outside [1, INF] or no memory")
+    return ok.astype(bool)
 
 
 def brute_walks(W):

@@ -176,41 +176,50 @@ def test_idempotent_on_solved_matrix():
 @pytest.mark.slow
 def test_config2_n8192_all_kappas():
     """n=8192, density 0.9, U[1,1000], seed 42: multi_layer_tiling_gpu with
-    kappa in {1,2,4,6,8} bit-exact against the row-parallel oracle."""
+    the autotuned tile and kappa in {1,2,4,6,8}, tile 64 with the tuned
+    kappa's neighbour, and tiling_gpu, bit-exact against the row-parallel
+    oracle (the restatement of fwBaselineCPU)."""
     g = fx.generate_graph(8192, 0.9, 42, 1, 1000)
     say("config2: oracle start")
     expect = po.fw_rowpar(g)
     say("config2: oracle done")
+    tile, _ = fx.tuned_config(8192)
+    runs = [("multi_layer_tiling_gpu", tile, k) for k in (1, 2, 4, 6, 8)]
+    runs += [("multi_layer_tiling_gpu", 64, 3), ("tiling_gpu", 0, 0)]
     with fx.DeviceMatrix(8192) as dm:
-        for kappa in (1, 2, 4, 6, 8):
+        for algo, t, k in runs:
             dm.upload(g)
-            say("config2 kappa", kappa)
-            fx.run_device("multi_layer_tiling_gpu", dm.ptr, 8192, 64, kappa)
-            assert (dm.download() == expect).all(), kappa
-        dm.upload(g)
-        say("config2 tiling_gpu")
-        fx.run_device("tiling_gpu", dm.ptr, 8192, 0, 0)
-        assert (dm.download() == expect).all()
+            say("config2", algo, t, k)
+            fx.run_device(algo, dm.ptr, 8192, t, k)
+            assert (dm.download() == expect).all(), (algo, t, k)
 
 
 @pytest.mark.slow
 def test_config3_n32768_spot_rows():
     """n=32768 (4 GiB int32), density 0.1, weights U[1,1000], seed 7:
-    multi_layer_tiling_gpu (tuned) and tiling_gpu, 64 random rows bit-exact
-    against host Dijkstra over the INF-completed graph."""
+    multi_layer_tiling_gpu (tuned) certified exact on 4096 random rows by the
+    shortest-path optimality certificate, 64 of them also equal to host
+    Dijkstra over the INF-completed graph; tiling_gpu bit-identical to it on
+    every cell."""
     n = 32768
     g = fx.generate_graph(n, 0.1, 7, 1, 1000)
-    rows = np.random.default_rng(7).choice(n, 64, replace=False).astype(np.int32)
-    say("config3: dijkstra on", len(rows), "rows")
-    expect = po.dijkstra_rows(g, rows)
-    for algo, tile, kappa in (("multi_layer_tiling_gpu", 0, 0), ("tiling_gpu", 0, 0)):
-        with fx.DeviceMatrix(n) as dm:
-            dm.upload(g)
-            say("config3", algo)
-            fx.run_device(algo, dm.ptr, n, tile, kappa)
-            got = dm.download()
-        assert (got[rows] == expect).all(), algo
-        del got
+    rows = np.sort(np.random.default_rng(7).choice(n, 4096, replace=False)).astype(np.int32)
+    say("config3: dijkstra on 64 rows")
+    dij = po.dijkstra_rows(g, rows[::64])
+    with fx.DeviceMatrix(n) as dm:
+        dm.upload(g)
+        say("config3 multi_layer_tiling_gpu")
+        fx.run_device("multi_layer_tiling_gpu", dm.ptr, n, 0, 0)
+        ml = dm.download()
+        dm.upload(g)
+        say("config3 tiling_gpu")
+        fx.run_device("tiling_gpu", dm.ptr, n, 0, 0)
+        same = bool((dm.download() == ml).all())
+    assert same, "tiling_gpu and multi_layer_tiling_gpu differ"
+    assert (ml[rows[::64]] == dij).all()
+    say("config3: certificate on", rows.size, "rows")
+    ok = po.certify_rows(g, rows, ml[rows])
+    assert ok.all(), rows[~ok][:10]
 
 
 def test_graph_replay_cache(monkeypatch):

@@ -105,6 +105,54 @@ def test_planted_mistake_is_caught():
     assert (po.fw_serial(W) == po.brute_walks(W)).all()
 
 
+def test_certificate_accepts_exact_rows_only():
+    """oracle_certify_rows accepts exactly the shortest-distance rows: it
+    passes FW/Dijkstra rows (ragged n, sparse to dense, INF edges), and fails
+    every planted mistake -- one cell too low or too high, a row of another
+    source, a source cell that is not 0, the k-innermost wrong loop order,
+    an unreachable (INF) pair claimed reachable."""
+    rng = np.random.default_rng(19)
+    for n, p in [(2, 1.0), (3, 0.0), (33, 0.1), (130, 0.5), (301, 0.9), (2500, 0.05)]:
+        W = random_graph(rng, n, p)
+        rows = np.unique(rng.integers(0, n, min(n, 70))).astype(np.int32)
+        D = po.fw_rowpar(W)
+        assert po.certify_rows(W, rows, D[rows]).all(), n
+        assert po.certify_rows(W, rows, po.dijkstra_rows(W, rows)).all(), n
+        if n < 3:
+            continue
+        bad = D[rows].copy()
+        q, v = 0, (int(rows[0]) + 1) % n
+        lo = bad.copy(); lo[q, v] -= 1
+        hi = bad.copy(); hi[q, v] += 1
+        other = bad.copy(); other[q] = D[(int(rows[0]) + 1) % n]
+        src = bad.copy(); src[q, rows[0]] = 1
+        for planted in (lo, hi, other, src):
+            ok = po.certify_rows(W, rows, planted)
+            assert not ok[0] and ok[1:].all(), n
+    # the k-innermost loop order leaves some rows unfinished; all of them fail
+    W = random_graph(rng, 40, 0.08)
+    bad = W.astype(np.int64).copy()
+    for i in range(40):
+        for j in range(40):
+            for k in range(40):
+                bad[i, j] = min(bad[i, j], bad[i, k] + bad[k, j])
+    bad = bad.astype(np.int32)
+    D = po.fw_serial(W)
+    rows = np.arange(40, dtype=np.int32)
+    wrong = np.where((bad != D).any(axis=1))[0]
+    assert wrong.size > 0
+    ok = po.certify_rows(W, rows, bad)
+    assert (~ok[wrong]).all() and ok[np.setdiff1d(rows, wrong)].all()
+    # an unreachable pair (INF) claimed one lower is caught too
+    W = random_graph(rng, 50, 0.02)
+    D = po.fw_serial(W)
+    rows = np.arange(50, dtype=np.int32)
+    i, j = np.argwhere(D == INF)[0]
+    bad = D.copy(); bad[i, j] = INF - 1
+    ok = po.certify_rows(W, rows, bad)
+    assert not ok[i] and ok[np.setdiff1d(rows, [i])].all()
+
+
 def test_f32_pred_matches_integer_oracle():
     rng = np.random.default_rng(19)
     for n, p in [(1, 0.5), (40, 0.2), (150, 0.6)]:

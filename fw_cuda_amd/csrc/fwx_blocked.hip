@@ -146,18 +146,14 @@ __global__ __launch_bounds__(256) void k_cross(int32_t* __restrict__ D, int32_t 
 
 // ---------------------------------------------------------------------------
 // rest: R x R tiles outside the stage rows/columns, depth S = B*T.
-// The (m x m) reduced tile grid is split into 8 contiguous ranges, one per
-// XCD (workgroups are dealt to the 8 XCDs round-robin), and each range is
-// walked in 8-row groups so concurrently resident tiles share A panels
-// (same rows) and B panels (nearby columns) in that XCD's L2.
-template <int R, bool VEC, bool PK, int KC = kChunk>
-__global__ __launch_bounds__(256, KC == 16 ? 3 : 2) void k_rest(int32_t* __restrict__ D, int32_t n,
-                                                                        int32_t s0, int32_t S, int32_t sr0,
-                                                                        int32_t srn, int32_t m, int32_t per) {
-  __shared__ TileSmem<R, R, KC> s;
+// The (m x m) reduced tile grid is split into 8 contiguous ranges of `per`
+// tiles, one per XCD (workgroups are dealt to the 8 XCDs round-robin), and
+// each range is walked in 8-row groups so concurrently resident tiles share
+// A panels (same rows) and B panels (nearby columns) in that XCD's L2.
+__device__ __forceinline__ bool rest_tile_of_block(int m, int per, int sr0, int srn, int* I, int* J) {
   const int b = blockIdx.x;
   const int L = (b & 7) * per + (b >> 3);
-  if (L >= m * m) return;
+  if (L >= m * m) return false;
   constexpr int G = 8;
   const int group = L / (G * m);
   const int first = group * G;
@@ -165,8 +161,18 @@ __global__ __launch_bounds__(256, KC == 16 ? 3 : 2) void k_rest(int32_t* __restr
   const int within = L - group * G * m;
   const int Ii = first + within % rows;
   const int Jj = within / rows;
-  const int I = Ii + (Ii >= sr0 ? srn : 0);
-  const int J = Jj + (Jj >= sr0 ? srn : 0);
+  *I = Ii + (Ii >= sr0 ? srn : 0);
+  *J = Jj + (Jj >= sr0 ? srn : 0);
+  return true;
+}
+
+template <int R, bool VEC, bool PK, int KC = kChunk>
+__global__ __launch_bounds__(256, KC == 16 ? 3 : 2) void k_rest(int32_t* __restrict__ D, int32_t n,
+                                                                        int32_t s0, int32_t S, int32_t sr0,
+                                                                        int32_t srn, int32_t m, int32_t per) {
+  __shared__ TileSmem<R, R, KC> s;
+  int I, J;
+  if (!rest_tile_of_block(m, per, sr0, srn, &I, &J)) return;
   const View A = make_view(D, n, I * R, s0);
   const View B = make_view(D, n, s0, J * R);
   const View C = make_view(D, n, I * R, J * R);
@@ -180,18 +186,8 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
     int32_t* __restrict__ D, int32_t n, int32_t s0, int32_t S, int32_t sr0, int32_t srn, int32_t m,
     int32_t per) {
   __shared__ TileSmem<R, R, kChunk> s;
-  const int b = blockIdx.x;
-  const int L = (b & 7) * per + (b >> 3);
-  if (L >= m * m) return;
-  constexpr int G = 8;
-  const int group = L / (G * m);
-  const int first = group * G;
-  const int rows = min(m - first, G);
-  const int within = L - group * G * m;
-  const int Ii = first + within % rows;
-  const int Jj = within / rows;
-  const int I = Ii + (Ii >= sr0 ? srn : 0);
-  const int J = Jj + (Jj >= sr0 ? srn : 0);
+  int I, J;
+  if (!rest_tile_of_block(m, per, sr0, srn, &I, &J)) return;
   const View A = make_view(D, n, I * R, s0);
   const View B = make_view(D, n, s0, J * R);
   const View C = make_view(D, n, I * R, J * R);

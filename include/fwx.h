@@ -163,8 +163,8 @@ int fwx_diff_backward(const float* d_W, const float* d_D, const int32_t* d_P, co
  * bracketed by HIP events on the stream it runs on.  fwx_profile_end fills
  * arrays of FWX_PROF_NCLASS entries (any may be NULL): summed event time in
  * ms, summed algorithmic work (relaxations for REST/LEAD/BAND; bytes for
- * BASELINE: the 4 n^2 bytes every per-k sweep must read -- improved cells
- * are written on top of that), and the number of groups.  The reference
+ * BASELINE: the 4 n^2 bytes every per-k sweep must read -- it writes only
+ * the cells that improve), and the number of groups.  The reference
  * times with cudaEvent pairs around a whole call (src/timer.cu:48-63).
  */
 #define FWX_PROF_REST 0     /* one k_rest launch per stage */

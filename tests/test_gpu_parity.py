@@ -140,6 +140,22 @@ def test_schedules_agree(monkeypatch, lookahead):
         assert (gpu(g, "multi_layer_tiling_gpu", t, k) == po.fw_rowpar(g)).all(), (n, t, k)
 
 
+@pytest.mark.parametrize("variant", ["0", "1", "2"])
+def test_rest_kernel_variants(monkeypatch, variant):
+    """Every shape of the 128-wide rest kernel (FWX_REST_VARIANT), ragged and
+    whole-tile sizes, both relaxation forms."""
+    monkeypatch.setenv("FWX_REST_VARIANT", variant)
+    rng = np.random.default_rng(31)
+    for n in (128, 256, 384, 640, 1024, 300, 1000):
+        g = fx.generate_graph(n, float(rng.uniform(0.1, 0.9)), int(rng.integers(1, 2**31)), 1, 1000)
+        gneg = reweighted(rng, n, 0.4)
+        expect, expect_neg = po.fw_rowpar(g), po.fw_rowpar(gneg)
+        for t, k in ((128, 1), (128, 2), (64, 2), (64, 4), (32, 4), (128, 3)):
+            say("rest variant", variant, n, t, k)
+            assert (gpu(g, "multi_layer_tiling_gpu", t, k) == expect).all(), (n, t, k)
+            assert (gpu(gneg, "multi_layer_tiling_gpu", t, k) == expect_neg).all(), (n, t, k)
+
+
 @pytest.mark.parametrize("nth", ["256", "512", "1024"])
 def test_diag_block_sizes(monkeypatch, nth):
     """The diagonal closure with 256, 512 and 1024 threads per block (tile

@@ -110,12 +110,18 @@ def floyd_warshall_(t, algo="multi_layer_tiling_gpu", tile=0, kappa=0):
     return t
 
 
+BATCH_MODES = {"direct": 0, "graph": 1, "persistent": 2, "persistent_graph": 3}
+
+
 class BatchPlan:
     """Many graphs of uneven size solved together (fwx_batch_plan_*).
 
     `sizes[g]` and element `offsets[g]` locate graph g inside the int32 device
-    buffer at address `d_data`.  run(use_graph=True) replays the captured
-    hipGraph; run(False) enqueues the same launches directly."""
+    buffer at address `d_data`.  run(mode) with mode one of BATCH_MODES:
+    "graph" replays the captured hipGraph of the step launches, "direct"
+    enqueues them, "persistent" runs the whole batch as one persistent kernel
+    and "persistent_graph" replays that launch from a hipGraph.  A bool is
+    accepted for the first two (True = "graph")."""
 
     def __init__(self, sizes, offsets, d_data, stream=None):
         self.sizes = np.ascontiguousarray(sizes, dtype=np.int32)
@@ -126,8 +132,19 @@ class BatchPlan:
                                           len(self.sizes), d_data, s))
         self.plan = p
 
-    def run(self, use_graph=True):
-        check(lib().fwx_batch_plan_run(self.plan, 1 if use_graph else 0))
+    def run(self, mode="graph"):
+        if isinstance(mode, bool):
+            mode = "graph" if mode else "direct"
+        check(lib().fwx_batch_plan_run(self.plan, BATCH_MODES[mode]))
+
+    def set_wait_limit(self, seconds):
+        """Limit of one device-side dependency wait of the persistent mode."""
+        check(lib().fwx_batch_plan_set_wait_limit(self.plan, float(seconds)))
+
+    def queues(self):
+        q, g, t = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int64()
+        check(lib().fwx_batch_plan_queues(self.plan, ctypes.byref(q), ctypes.byref(g), ctypes.byref(t)))
+        return {"queues": q.value, "grid": g.value, "tasks": t.value}
 
     def stats(self):
         st, la, it = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int64()

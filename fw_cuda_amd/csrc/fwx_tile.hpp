@@ -398,14 +398,16 @@ __device__ __forceinline__ void tile_update(TileSmem<BM, BN, KC>& s, const View&
 // padded vertices are never intermediates.  More threads mean fewer cells
 // per thread and more waves per SIMD to hide the barrier and LDS latency of
 // the T dependent steps.
+// `lines` is LDS scratch of 8 T ints (the caller's, so a kernel that also
+// runs tile products can lend their staging buffer).
 template <int T, int NTH = 256>
-__device__ __forceinline__ void close_tile(const View V) {
+__device__ __forceinline__ void close_tile_in(const View V, int32_t* lines) {
   constexpr int RS = NTH / 16;  // thread rows
   constexpr int TR = T / RS;    // register rows per thread
   constexpr int TQ = T / 16;    // register columns per thread
   static_assert(NTH % 32 == 0 && T % RS == 0 && RS % 16 == 0, "layout must tile T");
-  __shared__ int32_t rows[2][2][T];  // [buf][k, k+1][j]
-  __shared__ int32_t cols[2][2][T];  // [buf][k, k+1][i]
+  int32_t(*rows)[2][T] = reinterpret_cast<int32_t(*)[2][T]>(lines);          // [buf][k, k+1][j]
+  int32_t(*cols)[2][T] = reinterpret_cast<int32_t(*)[2][T]>(lines + 4 * T);  // [buf][k, k+1][i]
   const int tx = threadIdx.x & 15;
   const int ty = threadIdx.x >> 4;
   int32_t acc[TR][TQ];
@@ -471,6 +473,12 @@ __device__ __forceinline__ void close_tile(const View V) {
 #pragma unroll
     for (int q = 0; q < TQ; ++q)
       if (V.in(ty + RS * r, tx + 16 * q)) *V.at(ty + RS * r, tx + 16 * q) = acc[r][q];
+}
+
+template <int T, int NTH = 256>
+__device__ __forceinline__ void close_tile(const View V) {
+  __shared__ int32_t lines[8 * T];
+  close_tile_in<T, NTH>(V, lines);
 }
 
 }  // namespace fwx

@@ -31,7 +31,8 @@ typedef enum fwx_status {
   FWX_ERR_UNKNOWN_ALGO = 2, /* name not in the registry (reference throws std::out_of_range) */
   FWX_ERR_HIP = 3,          /* a HIP runtime call or launch failed */
   FWX_ERR_OOM = 4,          /* a library-owned workspace could not be allocated */
-  FWX_ERR_UNSUPPORTED = 5   /* valid request this build does not serve */
+  FWX_ERR_UNSUPPORTED = 5,  /* valid request this build does not serve */
+  FWX_ERR_TIMEOUT = 6       /* a device-side wait exceeded its limit / work left unrun */
 } fwx_status;
 
 /* Message of the last failing call on this host thread ("" if none). */
@@ -119,15 +120,35 @@ int fwx_has_negative_cycle(const int32_t* h_D, int32_t n);
  * The plan builds every graph's multi-layer schedule (tile and kappa from
  * the tuned table per size class), merges step j of all graphs into one
  * launch per work kind, largest graphs first, and uploads the work lists.
- * fwx_batch_plan_run(plan, 1) captures the launch sequence into a hipGraph
- * on first use and replays it; with 0 it enqueues the launches directly.
- * Results equal a plain fwx_multi_layer_tiling_gpu call per graph.  The plan
- * owns device memory (its work lists) until fwx_batch_plan_destroy.
+ * Run modes:
+ *   FWX_BATCH_DIRECT (0)   the step launches enqueued directly;
+ *   FWX_BATCH_GRAPH (1)    the step launches captured into a hipGraph on first
+ *                          use and replayed;
+ *   FWX_BATCH_PERSISTENT (2) ONE persistent kernel: per-XCD task lists in
+ *                          dependency order, per-graph completion counters,
+ *                          every wait bounded (fwx_batch_plan_set_wait_limit,
+ *                          default 10 s; exceeding it returns FWX_ERR_TIMEOUT
+ *                          and leaves the batch's contents undefined);
+ *   FWX_BATCH_PERSISTENT_GRAPH (3) the persistent launch (plus its counter
+ *                          reset) captured into a hipGraph and replayed.
+ * The two persistent modes are switched off in this build and return
+ * FWX_ERR_UNSUPPORTED (their first GPU runs did not complete).
+ * Results equal a plain fwx_multi_layer_tiling_gpu call per graph in every
+ * mode.  The plan owns device memory (its work lists) until
+ * fwx_batch_plan_destroy.
  */
 typedef struct fwx_batch_plan fwx_batch_plan;
+#define FWX_BATCH_DIRECT 0
+#define FWX_BATCH_GRAPH 1
+#define FWX_BATCH_PERSISTENT 2
+#define FWX_BATCH_PERSISTENT_GRAPH 3
 int fwx_batch_plan_create(fwx_batch_plan** plan, const int32_t* sizes, const int64_t* offsets,
                           int32_t count, int32_t* d_data, void* stream);
-int fwx_batch_plan_run(fwx_batch_plan* plan, int32_t use_graph);
+int fwx_batch_plan_run(fwx_batch_plan* plan, int32_t mode);
+/* Limit of one device-side dependency wait of the persistent mode. */
+int fwx_batch_plan_set_wait_limit(fwx_batch_plan* plan, double seconds);
+/* Persistent mode: task queues (one per XCD the probe saw), grid, tasks. */
+int fwx_batch_plan_queues(const fwx_batch_plan* plan, int32_t* queues, int32_t* grid, int64_t* tasks);
 /* Dependent steps, kernel launches per run, and work items of a plan. */
 int fwx_batch_plan_stats(const fwx_batch_plan* plan, int32_t* steps, int32_t* launches, int64_t* items);
 int fwx_batch_plan_destroy(fwx_batch_plan* plan);

@@ -154,6 +154,9 @@ def test_batched_and_diff_argument_errors():
     assert L.fwx_batch_plan_create(ctypes.byref(plan), _lib.ptr(sizes), _lib.ptr(neg), 1, dummy,
                                    None) == _lib.ERR_INVALID_ARG
     assert L.fwx_batch_plan_run(None, 1) == _lib.ERR_INVALID_ARG
+    assert L.fwx_batch_plan_run(None, 3) == _lib.ERR_INVALID_ARG
+    assert L.fwx_batch_plan_set_wait_limit(None, 1.0) == _lib.ERR_INVALID_ARG
+    assert L.fwx_batch_plan_queues(None, None, None, None) == _lib.ERR_INVALID_ARG
     assert L.fwx_batch_loop(_lib.ptr(sizes), _lib.ptr(offs), 0, dummy, None) == _lib.ERR_INVALID_ARG
     assert L.fwx_diff_forward(dummy, dummy, None, 0, 1, None) == _lib.ERR_INVALID_ARG
     assert L.fwx_diff_forward(dummy, dummy, None, 8193, 1, None) == _lib.ERR_INVALID_ARG

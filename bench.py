@@ -43,6 +43,7 @@ def parse():
     ap.add_argument("--no-compare", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-diff", action="store_true")
+    ap.add_argument("--no-pipeline", action="store_true")
     return ap.parse_args()
 
 
@@ -92,6 +93,47 @@ def bench_batched(steps):
             "loop_graphs_per_s": len(ns) / loop_s, "loop_ms_per_batch": loop_s * 1e3,
             "speedup_vs_loop": loop_s / steps_s, "n3_per_s": relax / steps_s,
             "plan": st, "replay_equals_loop": same}
+
+
+def bench_pipeline(n):
+    """Input pipeline at the workload's n: the reference-compatible host
+    generator (std::mt19937, one thread) vs the Philox device generator, and
+    a pageable vs a pinned host-to-device copy of the n x n matrix."""
+    import ctypes
+    import numpy as np
+    import fw_cuda_amd as fx
+    from fw_cuda_amd import _lib
+    L = _lib.lib()
+    out = {"n": n}
+    t0 = time.perf_counter()
+    h = fx.generate_graph(n, 0.9, 42, 1, 1000)
+    out["host_generator_s"] = time.perf_counter() - t0
+    with fx.DeviceMatrix(n) as dm:
+        fx.generate_graph_device(dm.ptr, n, 0.9, 42)  # warm
+        t0 = time.perf_counter()
+        fx.generate_graph_device(dm.ptr, n, 0.9, 42)
+        out["device_generator_s"] = time.perf_counter() - t0
+        nbytes = 4.0 * n * n
+        t0 = time.perf_counter()
+        _lib.check(L.fwx_copy_to_device(dm.ptr, _lib.ptr(h), n))
+        dt = time.perf_counter() - t0
+        out["pageable_h2d_GBps"] = nbytes / dt / 1e9
+        hp = ctypes.c_void_p()
+        _lib.check(L.fwx_host_alloc_pinned(ctypes.byref(hp), n))
+        try:
+            buf = np.ctypeslib.as_array(ctypes.cast(hp, ctypes.POINTER(ctypes.c_int32)), shape=(n, n))
+            buf[:] = h
+            _lib.check(L.fwx_copy_to_device_async(dm.ptr, hp, n, None))
+            _lib.check(L.fwx_stream_synchronize(None))
+            t0 = time.perf_counter()
+            _lib.check(L.fwx_copy_to_device_async(dm.ptr, hp, n, None))
+            _lib.check(L.fwx_stream_synchronize(None))
+            dt = time.perf_counter() - t0
+            out["pinned_h2d_GBps"] = nbytes / dt / 1e9
+        finally:
+            _lib.check(L.fwx_host_free_pinned(hp))
+    out["generator_speedup"] = out["host_generator_s"] / out["device_generator_s"]
+    return out
 
 
 def bench_diff(steps):
@@ -316,6 +358,8 @@ def main():
         out["batched"] = bench_batched(3)
     if not a.no_diff:
         out["differentiable"] = bench_diff(3)
+    if not a.no_pipeline:
+        out["input_pipeline"] = bench_pipeline(n)
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(a.cpu_n, a.cpu_n_warm)
     print(json.dumps(out), flush=True)

@@ -48,6 +48,22 @@ def has_negative_cycle(D):
     return bool(lib().fwx_has_negative_cycle(ptr(D), int(D.shape[0])))
 
 
+def generate_graph_device(d_ptr, n, density, seed, wmin=1, wmax=1000, stream=None):
+    """fwx_generate_graph_device: Philox-based graph generated in place on the
+    GPU (same distributions as generate_graph, a different stream)."""
+    s = ctypes.c_void_p(stream) if stream else None
+    check(lib().fwx_generate_graph_device(d_ptr, int(n), float(density), int(seed) & (2**64 - 1),
+                                          int(wmin), int(wmax), s))
+
+
+def has_negative_cycle_device(d_ptr, n, stream=None):
+    """fwx_has_negative_cycle_device: the diagonal check on a device matrix."""
+    r = ctypes.c_int32()
+    s = ctypes.c_void_p(stream) if stream else None
+    check(lib().fwx_has_negative_cycle_device(d_ptr, int(n), ctypes.byref(r), s))
+    return bool(r.value)
+
+
 class DeviceMatrix:
     """n x n int32 device buffer owned by the caller (reference GPUMemory,
     src/gpu_memory.cu:6-27): allocate, copy in, run in place, copy out, free."""

@@ -57,6 +57,13 @@ SIGNATURES = {
     "fwx_warmup": (ctypes.c_int, []),
     "fwx_generate_graph": (ctypes.c_int, [_vp, _i32, ctypes.c_double, ctypes.c_uint32, _i32, _i32]),
     "fwx_has_negative_cycle": (ctypes.c_int, [_vp, _i32]),
+    "fwx_has_negative_cycle_device": (ctypes.c_int, [_vp, _i32, _i32p, _vp]),
+    "fwx_generate_graph_device": (ctypes.c_int, [_vp, _i32, ctypes.c_double, ctypes.c_uint64, _i32, _i32, _vp]),
+    "fwx_host_alloc_pinned": (ctypes.c_int, [ctypes.POINTER(_vp), _i32]),
+    "fwx_host_free_pinned": (ctypes.c_int, [_vp]),
+    "fwx_copy_to_device_async": (ctypes.c_int, [_vp, _vp, _i32, _vp]),
+    "fwx_copy_to_host_async": (ctypes.c_int, [_vp, _vp, _i32, _vp]),
+    "fwx_stream_synchronize": (ctypes.c_int, [_vp]),
     "fwx_batch_plan_create": (ctypes.c_int, [ctypes.POINTER(_vp), _vp, _vp, _i32, _vp, _vp]),
     "fwx_batch_plan_run": (ctypes.c_int, [_vp, _i32]),
     "fwx_batch_plan_stats": (ctypes.c_int, [_vp, _i32p, _i32p, ctypes.POINTER(_i64)]),

@@ -209,4 +209,14 @@ int fwx_has_negative_cycle(const int32_t* h_D, int32_t n) {
   return 0;
 }
 
+int fwx_has_negative_cycle_device(const int32_t* d_D, int32_t n, int32_t* result, void* stream) {
+  clear_error();
+  if (!d_D || !result || n <= 0 || n > FWX_MAX_N)
+    return fail(FWX_ERR_INVALID_ARG, "fwx_has_negative_cycle_device: null pointer or n out of range");
+  bool neg = false;
+  if (int rc = device_negative_cycle(d_D, n, as_stream(stream), &neg)) return rc;
+  *result = neg ? 1 : 0;
+  return FWX_OK;
+}
+
 }  // extern "C"

@@ -83,4 +83,31 @@ int values_in_pk64_range(const int32_t* d, int64_t count, hipStream_t st, bool* 
   return FWX_OK;
 }
 
+namespace {
+
+// Any D[i][i] < 0 -> *f = 1.  One diagonal cell per thread.
+__global__ __launch_bounds__(256) void k_negative_diagonal(const int32_t* __restrict__ d, int32_t n,
+                                                            int* __restrict__ f) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const bool neg = i < n && d[i * n + i] < 0;
+  if (__ballot(neg) != 0ull && (threadIdx.x & 63) == 0) atomicOr(f, 1);
+}
+
+}  // namespace
+
+int device_negative_cycle(const int32_t* d, int32_t n, hipStream_t st, bool* neg) {
+  int* base = nullptr;
+  if (int rc = flag_base(&base)) return rc;
+  int* f = base + (g_slot.fetch_add(1) % kSlots);
+  k_flag_zero<<<1, 1, 0, st>>>(f);
+  FWX_LAUNCHED("k_flag_zero");
+  k_negative_diagonal<<<(n + 255) / 256, 256, 0, st>>>(d, n, f);
+  FWX_LAUNCHED("k_negative_diagonal");
+  int h = 0;
+  FWX_HIP(hipMemcpyAsync(&h, f, sizeof(int), hipMemcpyDeviceToHost, st));
+  FWX_HIP(hipStreamSynchronize(st));
+  *neg = (h != 0);
+  return FWX_OK;
+}
+
 }  // namespace fwx

@@ -111,6 +111,35 @@ int fwx_generate_graph(int32_t* h_D, int32_t n, double density, uint32_t seed,
                        int32_t wmin, int32_t wmax);
 /* 1 if any D[i][i] < 0 (reference hasNegativeCycle, src/utils.cpp:5-12). */
 int fwx_has_negative_cycle(const int32_t* h_D, int32_t n);
+/* The same check on a device matrix (no copy of the n x n result to the
+ * host): *result = 1 if any d_D[i][i] < 0, else 0.  Reads the n diagonal
+ * cells on `stream` and synchronises it.  Replaces the host-side
+ * hasNegativeCycle call after a GPU solve, which needs the whole matrix
+ * copied back first (src/tests_gpu.cu:41-53,139-141). */
+int fwx_has_negative_cycle_device(const int32_t* d_D, int32_t n, int32_t* result, void* stream);
+
+/* ---------------- input pipeline (no reference counterpart) ------------- */
+/*
+ * Graph generated on the GPU into a caller-owned device matrix: Philox4x32-10
+ * counter-based draws, one per cell (counter = cell index i*n+j, key = seed),
+ * edge with probability `density`, weight uniform in [wmin, wmax], diagonal
+ * 0, INF where absent -- the distributions of generateGraph1D
+ * (src/graph_generator.cpp:25-43) but NOT its mt19937 stream: a seed gives a
+ * different matrix than fwx_generate_graph.  Identical output for any launch
+ * shape.  Synchronises `stream`.
+ */
+int fwx_generate_graph_device(int32_t* d_D, int32_t n, double density, uint64_t seed, int32_t wmin,
+                              int32_t wmax, void* stream);
+/* Pinned host matrices and asynchronous copies on `stream` (the reference's
+ * GPUMemory::copyToDevice/copyToHost are synchronous pageable cudaMemcpy,
+ * src/gpu_memory.cu:13-21).  The async copies return at once; pair them with
+ * fwx_stream_synchronize.  Pageable host pointers work too, at pageable
+ * speed. */
+int fwx_host_alloc_pinned(int32_t** h_D, int32_t n);
+int fwx_host_free_pinned(int32_t* h_D);
+int fwx_copy_to_device_async(int32_t* d_D, const int32_t* h_D, int32_t n, void* stream);
+int fwx_copy_to_host_async(int32_t* h_D, const int32_t* d_D, int32_t n, void* stream);
+int fwx_stream_synchronize(void* stream);
 
 /* ---------------- batched FW (no reference counterpart) ----------------- */
 /*

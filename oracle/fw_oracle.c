@@ -324,6 +324,63 @@ void oracle_brute_walks(const int32_t* W, int32_t n, int32_t* out) {
   free(B);
 }
 
+/*
+ * oracle_philox4x32_10 / oracle_generate_philox -- restatement of the device
+ * generator's definition (no reference counterpart: the reference generates
+ * with std::mt19937 on the host).  Philox4x32 with 10 rounds as published by
+ * Salmon, Moraes, Dror and Shaw ("Parallel random numbers: as easy as 1, 2,
+ * 3", SC'11): each round multiplies counter words 0 and 2 by 0xD2511F53 and
+ * 0xCD9E8D57, mixes the high halves with words 1 / 3 and the key, and the
+ * key advances by 0x9E3779B9 / 0xBB67AE85 between rounds.  Pinned by the
+ * known-answer vectors of the Random123 distribution (tests/test_oracle.py).
+ * Cell (i, j): Philox(counter = (cell lo, cell hi, 0, 0), key = (seed lo,
+ * seed hi)) = (c0, c1, ..); diagonal 0; edge iff c0 < floor(density 2^32)
+ * (always for density 1), weight wmin + floor(c1 (wmax - wmin + 1) / 2^32);
+ * else INF.
+ */
+void oracle_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+  uint32_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3], k0 = key[0], k1 = key[1];
+  for (int r = 0; r < 10; ++r) {
+    if (r) {
+      k0 += 0x9E3779B9u;
+      k1 += 0xBB67AE85u;
+    }
+    const uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+    const uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+    const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
+    const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+    c1 = (uint32_t)p1;
+    c3 = (uint32_t)p0;
+    c0 = n0;
+    c2 = n2;
+  }
+  out[0] = c0;
+  out[1] = c1;
+  out[2] = c2;
+  out[3] = c3;
+}
+
+void oracle_generate_philox(int32_t* out, int32_t n, double density, uint64_t seed, int32_t wmin, int32_t wmax) {
+  const uint64_t thresh = density >= 1.0 ? (1ull << 32) : (uint64_t)(density * 4294967296.0);
+  const uint64_t range = (uint64_t)((int64_t)wmax - (int64_t)wmin + 1);
+  const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+  const size_t N = (size_t)n;
+  for (size_t i = 0; i < N; ++i)
+    for (size_t j = 0; j < N; ++j) {
+      const uint64_t cell = (uint64_t)(i * N + j);
+      int32_t v;
+      if (i == j) {
+        v = 0;
+      } else {
+        const uint32_t ctr[4] = {(uint32_t)cell, (uint32_t)(cell >> 32), 0u, 0u};
+        uint32_t r[4];
+        oracle_philox4x32_10(ctr, key, r);
+        v = (uint64_t)r[0] < thresh ? wmin + (int32_t)(((uint64_t)r[1] * range) >> 32) : ORACLE_INF;
+      }
+      out[i * N + j] = v;
+    }
+}
+
 /* ------------------------------------------------------------------------ */
 /* Differentiable (fp32) variant.  No reference code exists for it; these   */
 /* restate the textbook definitions the product's kernels implement.        */

@@ -46,6 +46,10 @@ def lib():
                                              ctypes.c_int32]
         _lib.oracle_certify_rows.restype = ctypes.c_int
         _lib.oracle_brute_walks.argtypes = [_i32p, ctypes.c_int32, _i32p]
+        _u32p = ctypes.POINTER(ctypes.c_uint32)
+        _lib.oracle_philox4x32_10.argtypes = [_u32p, _u32p, _u32p]
+        _lib.oracle_generate_philox.argtypes = [_i32p, ctypes.c_int32, ctypes.c_double, ctypes.c_uint64,
+                                                ctypes.c_int32, ctypes.c_int32]
         _lib.oracle_fw_f32_pred.argtypes = [_f32p, _f32p, _i32p, ctypes.c_int32]
         _lib.oracle_grad_paths.argtypes = [_i32p, _f32p, _f32p, ctypes.c_int32]
         _lib.oracle_grad_paths.restype = ctypes.c_int
@@ -121,6 +125,24 @@ def certify_rows(W, rows, D_rows, threads=0):
     if rc < 0:
         raise ValueError(f"oracle_certify_rows failed ({rc}): weights outside [1, INF] or no memory")
     return ok.astype(bool)
+
+
+def philox4x32_10(ctr, key):
+    """oracle_philox4x32_10: one Philox4x32-10 block."""
+    c = np.ascontiguousarray(ctr, dtype=np.uint32)
+    k = np.ascontiguousarray(key, dtype=np.uint32)
+    out = np.zeros(4, dtype=np.uint32)
+    u = ctypes.POINTER(ctypes.c_uint32)
+    lib().oracle_philox4x32_10(c.ctypes.data_as(u), k.ctypes.data_as(u), out.ctypes.data_as(u))
+    return out
+
+
+def generate_philox(n, density, seed, wmin=1, wmax=1000):
+    """oracle_generate_philox: the device generator's matrix, on the host."""
+    out = np.empty((n, n), dtype=np.int32)
+    lib().oracle_generate_philox(_ptr(out, _i32p), int(n), float(density), int(seed) & (2**64 - 1),
+                                 int(wmin), int(wmax))
+    return out
 
 
 def brute_walks(W):

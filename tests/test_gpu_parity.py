@@ -112,6 +112,34 @@ def test_negative_weights(algo, tile, kappa):
         assert (gpu(g, algo, tile, kappa) == po.fw_rowpar(g)).all(), n
 
 
+def test_negative_cycle_check_on_device(kats):
+    """fwx_has_negative_cycle_device agrees with the host check (reference
+    hasNegativeCycle) on the KAT with a negative cycle after a GPU solve,
+    on solved graphs without one, and on planted negative diagonals at
+    the first, last and a middle position of ragged sizes."""
+    k = [k for k in kats if k.get("check") == "negative_cycle"][0]
+    n = k["n"]
+    D = np.array(k["input"], dtype=np.int32).reshape(n, n)
+    with fx.DeviceMatrix(n) as dm:
+        dm.upload(D)
+        fx.run_device("multi_layer_tiling_gpu", dm.ptr, n, 0, 0)
+        assert fx.has_negative_cycle_device(dm.ptr, n)
+        assert fx.has_negative_cycle(dm.download())
+    rng = np.random.default_rng(41)
+    for n in (1, 2, 255, 256, 257, 1000):
+        g = fx.generate_graph(n, 0.5, int(rng.integers(1, 2**31)), 1, 1000)
+        with fx.DeviceMatrix(n) as dm:
+            dm.upload(g)
+            fx.run_device("multi_layer_tiling_gpu", dm.ptr, n, 0, 0)
+            assert not fx.has_negative_cycle_device(dm.ptr, n)
+            for pos in {0, n // 2, n - 1}:
+                h = g.copy()
+                h[pos, pos] = -1
+                dm.upload(h)
+                say("device neg-cycle", n, pos)
+                assert fx.has_negative_cycle_device(dm.ptr, n), (n, pos)
+
+
 def test_large_values_take_exact_path():
     """Operands >= 2^30 (above INF) must take the exact min3 form.  They are
     placed in one column only, so no relaxation adds two of them and no sum

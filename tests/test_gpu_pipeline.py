@@ -1,0 +1,64 @@
+"""The input pipeline around the solve: the Philox device generator (bit-exact
+against the oracle's restatement of the same definition) and pinned host
+matrices with asynchronous copies, end to end through a solve."""
+import ctypes
+import sys
+
+import numpy as np
+import pytest
+
+import fw_cuda_amd as fx
+from fw_cuda_amd import _lib
+from oracle import pyoracle as po
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.mark.parametrize("n,density,seed,wmin,wmax", [
+    (1, 0.5, 1, 1, 1000), (2, 1.0, 2, 1, 1000), (3, 0.0, 3, 1, 1000), (37, 0.3, 4, 1, 1000),
+    (64, 0.9, 5, 1, 25000), (257, 0.1, 2**40 + 17, 0, 50), (1000, 0.9, 42, 1, 1000),
+    (1024, 0.5, 2**63 + 1, -5, 5), (2047, 0.05, 99, 7, 7)])
+def test_device_generator_matches_oracle(n, density, seed, wmin, wmax):
+    with fx.DeviceMatrix(n) as dm:
+        say("generate", n, density, seed)
+        fx.generate_graph_device(dm.ptr, n, density, seed, wmin, wmax)
+        got = dm.download()
+    assert (got == po.generate_philox(n, density, seed, wmin, wmax)).all()
+
+
+def test_generated_graph_solves_exactly():
+    """Generate on the device, solve in place, compare with the oracle solve
+    of the oracle-generated matrix (no host generation on the product side)."""
+    n = 1500
+    with fx.DeviceMatrix(n) as dm:
+        fx.generate_graph_device(dm.ptr, n, 0.4, 2024)
+        fx.run_device("multi_layer_tiling_gpu", dm.ptr, n, 0, 0)
+        got = dm.download()
+    assert (got == po.fw_rowpar(po.generate_philox(n, 0.4, 2024))).all()
+
+
+def test_pinned_async_round_trip():
+    """Pinned host matrix -> async H2D -> solve -> async D2H -> sync."""
+    L = _lib.lib()
+    n = 777
+    g = fx.generate_graph(n, 0.5, 31, 1, 1000)
+    expect = po.fw_rowpar(g)
+    h = ctypes.c_void_p()
+    _lib.check(L.fwx_host_alloc_pinned(ctypes.byref(h), n))
+    try:
+        buf = np.ctypeslib.as_array(ctypes.cast(h, ctypes.POINTER(ctypes.c_int32)), shape=(n, n))
+        buf[:] = g
+        with fx.DeviceMatrix(n) as dm:
+            _lib.check(L.fwx_copy_to_device_async(dm.ptr, h, n, None))
+            fx.run_device("multi_layer_tiling_gpu", dm.ptr, n, 0, 0)
+            buf[:] = 0
+            _lib.check(L.fwx_copy_to_host_async(h, dm.ptr, n, None))
+            _lib.check(L.fwx_stream_synchronize(None))
+        assert (buf == expect).all()
+    finally:
+        _lib.check(L.fwx_host_free_pinned(h))

@@ -154,6 +154,13 @@ def test_batched_and_diff_argument_errors():
     assert L.fwx_batch_plan_create(ctypes.byref(plan), _lib.ptr(sizes), _lib.ptr(neg), 1, dummy,
                                    None) == _lib.ERR_INVALID_ARG
     assert L.fwx_batch_plan_run(None, 1) == _lib.ERR_INVALID_ARG
+    r = ctypes.c_int32()
+    assert L.fwx_has_negative_cycle_device(None, 4, ctypes.byref(r), None) == _lib.ERR_INVALID_ARG
+    assert L.fwx_generate_graph_device(None, 4, 0.5, 1, 1, 10, None) == _lib.ERR_INVALID_ARG
+    assert L.fwx_generate_graph_device(dummy, 4, 1.5, 1, 1, 10, None) == _lib.ERR_INVALID_ARG
+    assert L.fwx_generate_graph_device(dummy, 4, 0.5, 1, 10, 1, None) == _lib.ERR_INVALID_ARG
+    assert L.fwx_host_alloc_pinned(None, 4) == _lib.ERR_INVALID_ARG
+    assert L.fwx_copy_to_device_async(None, None, 4, None) == _lib.ERR_INVALID_ARG
     assert L.fwx_batch_plan_run(None, 3) == _lib.ERR_INVALID_ARG
     assert L.fwx_batch_plan_set_wait_limit(None, 1.0) == _lib.ERR_INVALID_ARG
     assert L.fwx_batch_plan_queues(None, None, None, None) == _lib.ERR_INVALID_ARG

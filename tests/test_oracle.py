@@ -153,6 +153,42 @@ def test_certificate_accepts_exact_rows_only():
     assert not ok[i] and ok[np.setdiff1d(rows, [i])].all()
 
 
+def test_philox_known_answers():
+    """Philox4x32-10 against known-answer vectors of the Random123
+    distribution (kat_vectors: zero, all-ones and the digits of pi)."""
+    kat = [([0, 0, 0, 0], [0, 0], [0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8]),
+           ([0xFFFFFFFF] * 4, [0xFFFFFFFF] * 2, [0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD]),
+           ([0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344], [0xA4093822, 0x299F31D0],
+            [0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1])]
+    for ctr, key, expect in kat:
+        assert list(po.philox4x32_10(ctr, key)) == expect
+
+
+def test_philox_generator_definition():
+    """The generator's cell rule on top of Philox: diagonal 0, INF where the
+    first word is above the density threshold, weight from the second word;
+    density 0 / 1 give no / every edge; the edge fraction and the weight
+    range match; the cell depends on (seed, i*n+j) only."""
+    g = po.generate_philox(300, 0.3, 123456789, 5, 9)
+    assert (np.diag(g) == 0).all()
+    off = g[~np.eye(300, dtype=bool)]
+    present = off != INF
+    assert abs(present.mean() - 0.3) < 0.01
+    assert off[present].min() == 5 and off[present].max() == 9
+    for w in range(5, 10):  # uniform over the 5 weights
+        assert abs((off[present] == w).mean() - 0.2) < 0.01
+    assert (po.generate_philox(50, 0.0, 7)[~np.eye(50, dtype=bool)] == INF).all()
+    assert (po.generate_philox(50, 1.0, 7)[~np.eye(50, dtype=bool)] != INF).all()
+    # one cell by hand: (i, j) = (3, 17) of n = 40, seed 2^40 + 5
+    n, seed, i, j = 40, (1 << 40) + 5, 3, 17
+    r = po.philox4x32_10([i * n + j, 0, 0, 0], [5, 1 << 8])
+    expect = (1 + ((int(r[1]) * 1000) >> 32)) if int(r[0]) < int(0.5 * 2**32) else INF
+    assert po.generate_philox(n, 0.5, seed)[i, j] == expect
+    # a different seed gives a different matrix; the same seed the same one
+    assert (po.generate_philox(64, 0.5, 1) != po.generate_philox(64, 0.5, 2)).any()
+    assert (po.generate_philox(64, 0.5, 1) == po.generate_philox(64, 0.5, 1)).all()
+
+
 def test_f32_pred_matches_integer_oracle():
     rng = np.random.default_rng(19)
     for n, p in [(1, 0.5), (40, 0.2), (150, 0.6)]:

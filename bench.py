@@ -133,6 +133,17 @@ def bench_pipeline(n):
         finally:
             _lib.check(L.fwx_host_free_pinned(hp))
     out["generator_speedup"] = out["host_generator_s"] / out["device_generator_s"]
+    # predecessor matrix of the workload (one extra n^3 pass after the solve)
+    with fx.DeviceMatrix(n) as dw, fx.DeviceMatrix(n) as dd, fx.DeviceMatrix(n) as dp:
+        dw.upload(h)
+        dd.upload(h)
+        fx.run_device("multi_layer_tiling_gpu", dd.ptr, n, 0, 0)
+        _lib.check(L.fwx_predecessors(dw.ptr, dd.ptr, dp.ptr, n, None))  # warm
+        t0 = time.perf_counter()
+        _lib.check(L.fwx_predecessors(dw.ptr, dd.ptr, dp.ptr, n, None))
+        dt = time.perf_counter() - t0
+        out["predecessors_s"] = dt
+        out["predecessors_n3_per_s"] = float(n) ** 3 / dt
     return out
 
 

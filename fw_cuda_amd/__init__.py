@@ -56,6 +56,20 @@ def generate_graph_device(d_ptr, n, density, seed, wmin=1, wmax=1000, stream=Non
                                           int(wmin), int(wmax), s))
 
 
+def floyd_warshall_with_pred(W, algo="multi_layer_tiling_gpu", tile=0, kappa=0):
+    """Distances and the predecessor matrix (fwx_predecessors) of host matrix
+    W: P[i][j] is the vertex before j on a shortest i->j path (-1 on the
+    diagonal)."""
+    W = np.ascontiguousarray(W, dtype=np.int32)
+    n = W.shape[0]
+    with DeviceMatrix(n) as dw, DeviceMatrix(n) as dd, DeviceMatrix(n) as dp:
+        dw.upload(W)
+        dd.upload(W)
+        run_device(algo, dd.ptr, n, tile, kappa)
+        check(lib().fwx_predecessors(dw.ptr, dd.ptr, dp.ptr, n, None))
+        return dd.download(), dp.download()
+
+
 def has_negative_cycle_device(d_ptr, n, stream=None):
     """fwx_has_negative_cycle_device: the diagonal check on a device matrix."""
     r = ctypes.c_int32()

@@ -141,6 +141,18 @@ int fwx_copy_to_device_async(int32_t* d_D, const int32_t* h_D, int32_t n, void* 
 int fwx_copy_to_host_async(int32_t* h_D, const int32_t* d_D, int32_t n, void* stream);
 int fwx_stream_synchronize(void* stream);
 
+/* ---------------- predecessors (no reference counterpart) --------------- */
+/*
+ * Path reconstruction for the int32 engine.  d_W: the input weights (the
+ * solve overwrites its buffer, so the caller keeps a copy), d_D: the solved
+ * distances, d_P: n x n int32 output.  P[i][j] = the smallest u != j with
+ * D[i][u] + W[u][j] == D[i][j] (the vertex before j on a shortest i->j
+ * path), P[i][i] = -1, -1 where no u matches.  When every cycle has
+ * positive weight, following P from j reaches i along a shortest path.  One
+ * n^3 pass on `stream`; synchronises it.
+ */
+int fwx_predecessors(const int32_t* d_W, const int32_t* d_D, int32_t* d_P, int32_t n, void* stream);
+
 /* ---------------- batched FW (no reference counterpart) ----------------- */
 /*
  * Many independent graphs solved together.  Graph g is an n_g x n_g row-major

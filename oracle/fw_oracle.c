@@ -381,6 +381,33 @@ void oracle_generate_philox(int32_t* out, int32_t n, double density, uint64_t se
     }
 }
 
+/*
+ * oracle_predecessors -- restatement of the int32 predecessor output (no
+ * reference counterpart): P[i][j] = the smallest u != j with
+ * D[i][u] + W[u][j] == D[i][j] (sums in int64), P[i][i] = -1, -1 if none.
+ * Pinned by a hand-worked graph with ties and by the path invariant (walking
+ * P from j reaches i and the weights along the walk sum to D[i][j]) on
+ * random graphs with positive cycles (tests/test_oracle.py).
+ */
+void oracle_predecessors(const int32_t* W, const int32_t* D, int32_t* P, int32_t n) {
+  const size_t N = (size_t)n;
+  for (size_t i = 0; i < N; ++i)
+    for (size_t j = 0; j < N; ++j) {
+      int32_t best = -1;
+      if (i != j) {
+        const int64_t d = D[i * N + j];
+        for (size_t u = 0; u < N; ++u) {
+          if (u == j) continue;
+          if ((int64_t)D[i * N + u] + (int64_t)W[u * N + j] == d) {
+            best = (int32_t)u;
+            break;
+          }
+        }
+      }
+      P[i * N + j] = best;
+    }
+}
+
 /* ------------------------------------------------------------------------ */
 /* Differentiable (fp32) variant.  No reference code exists for it; these   */
 /* restate the textbook definitions the product's kernels implement.        */

@@ -46,6 +46,7 @@ def lib():
                                              ctypes.c_int32]
         _lib.oracle_certify_rows.restype = ctypes.c_int
         _lib.oracle_brute_walks.argtypes = [_i32p, ctypes.c_int32, _i32p]
+        _lib.oracle_predecessors.argtypes = [_i32p, _i32p, _i32p, ctypes.c_int32]
         _u32p = ctypes.POINTER(ctypes.c_uint32)
         _lib.oracle_philox4x32_10.argtypes = [_u32p, _u32p, _u32p]
         _lib.oracle_generate_philox.argtypes = [_i32p, ctypes.c_int32, ctypes.c_double, ctypes.c_uint64,
@@ -125,6 +126,15 @@ def certify_rows(W, rows, D_rows, threads=0):
     if rc < 0:
         raise ValueError(f"oracle_certify_rows failed ({rc}): weights outside [1, INF] or no memory")
     return ok.astype(bool)
+
+
+def predecessors(W, D):
+    """oracle_predecessors: smallest tight predecessor u != j of every (i, j)."""
+    W, n = _sq(W)
+    D, _ = _sq(D)
+    P = np.empty((n, n), dtype=np.int32)
+    lib().oracle_predecessors(_ptr(W, _i32p), _ptr(D, _i32p), _ptr(P, _i32p), n)
+    return P
 
 
 def philox4x32_10(ctr, key):

@@ -62,3 +62,46 @@ def test_pinned_async_round_trip():
         assert (buf == expect).all()
     finally:
         _lib.check(L.fwx_host_free_pinned(h))
+
+
+def positive_cycle_graph(rng, n, p):
+    """Negative edges, every cycle positive: w(u,v) + h(u) - h(v), w >= 1."""
+    W = fx.generate_graph(n, p, int(rng.integers(1, 2**31)), 1, 50).astype(np.int64)
+    h = rng.integers(0, 40, n)
+    W2 = np.where(W != fx.INF, W + h[:, None] - h[None, :], fx.INF).astype(np.int32)
+    np.fill_diagonal(W2, 0)
+    return W2
+
+
+@pytest.mark.parametrize("n", [1, 2, 63, 64, 65, 200, 511, 1000])
+def test_predecessors_match_oracle(n):
+    """fwx_predecessors bit-exact against the oracle's restatement, on
+    generated graphs and on graphs with negative edges (ragged sizes)."""
+    rng = np.random.default_rng(n)
+    for W in (fx.generate_graph(n, float(rng.uniform(0.05, 0.9)), int(rng.integers(1, 2**31)), 1, 1000),
+              positive_cycle_graph(rng, n, 0.3)):
+        say("predecessors", n)
+        D, P = fx.floyd_warshall_with_pred(W)
+        assert (D == po.fw_rowpar(W)).all()
+        assert (P == po.predecessors(W, D)).all()
+
+
+def test_predecessors_walks_at_4096():
+    """n=4096: walking P from j along 64 sampled rows reaches i with the
+    weights summing to D[i][j] (every cell of those rows)."""
+    n = 4096
+    W = fx.generate_graph(n, 0.3, 77, 1, 1000)
+    D, P = fx.floyd_warshall_with_pred(W)
+    rows = np.random.default_rng(1).choice(n, 64, replace=False)
+    for i in rows:
+        for j in range(n):
+            if j == i:
+                assert P[i, j] == -1
+                continue
+            total, v, steps = 0, j, 0
+            while v != i:
+                u = int(P[i, v])
+                assert 0 <= u < n and steps <= n, (i, j)
+                total += int(W[u, v])
+                v, steps = u, steps + 1
+            assert total == int(D[i, j]), (i, j)

@@ -160,6 +160,8 @@ def test_batched_and_diff_argument_errors():
     assert L.fwx_generate_graph_device(dummy, 4, 1.5, 1, 1, 10, None) == _lib.ERR_INVALID_ARG
     assert L.fwx_generate_graph_device(dummy, 4, 0.5, 1, 10, 1, None) == _lib.ERR_INVALID_ARG
     assert L.fwx_host_alloc_pinned(None, 4) == _lib.ERR_INVALID_ARG
+    assert L.fwx_predecessors(None, dummy, dummy, 4, None) == _lib.ERR_INVALID_ARG
+    assert L.fwx_predecessors(dummy, dummy, dummy, 0, None) == _lib.ERR_INVALID_ARG
     assert L.fwx_copy_to_device_async(None, None, 4, None) == _lib.ERR_INVALID_ARG
     assert L.fwx_batch_plan_run(None, 3) == _lib.ERR_INVALID_ARG
     assert L.fwx_batch_plan_set_wait_limit(None, 1.0) == _lib.ERR_INVALID_ARG

@@ -153,6 +153,65 @@ def test_certificate_accepts_exact_rows_only():
     assert not ok[i] and ok[np.setdiff1d(rows, [i])].all()
 
 
+def walk_ok(W, D, P, i, j):
+    """Walking P from j reaches i; the weights along the walk sum to D[i][j]."""
+    n = W.shape[0]
+    total, v, steps = 0, j, 0
+    while v != i:
+        u = int(P[i, v])
+        if u < 0 or steps > n:
+            return False
+        total += int(W[u, v])
+        v, steps = u, steps + 1
+    return total == int(D[i, j])
+
+
+def test_predecessors_hand_worked():
+    """A 5-vertex graph worked by hand: two equal-length routes 0->1->3 and
+    0->2->3 (the tie goes to the smaller vertex, 1), an unreachable vertex 4
+    (reached only by the INF edge itself: predecessor 0), a direct edge
+    beaten by a detour (0->3 of weight 9 vs 2+2)."""
+    W = np.array([[0, 2, 2, 9, INF],
+                  [INF, 0, INF, 2, INF],
+                  [INF, INF, 0, 2, INF],
+                  [INF, INF, INF, 0, INF],
+                  [INF, INF, INF, INF, 0]], dtype=np.int32)
+    D = po.fw_serial(W)
+    P = po.predecessors(W, D)
+    assert list(P[0]) == [-1, 0, 0, 1, 0]
+    assert P[1, 3] == 1 and P[2, 3] == 2
+    assert (np.diag(P) == -1).all()
+
+
+def test_predecessors_walk_to_shortest_paths():
+    """On random graphs with positive weights, and with negative edges but
+    positive cycles, every predecessor walk is a shortest path."""
+    rng = np.random.default_rng(23)
+
+    def positive_cycles_reweighted(n, p):
+        # w(u,v) + h(u) - h(v) with w >= 1: every cycle keeps its positive weight
+        W = random_graph(rng, n, p, 1, 50).astype(np.int64)
+        h = rng.integers(0, 40, n)
+        W2 = np.where(W != INF, W + h[:, None] - h[None, :], INF).astype(np.int32)
+        np.fill_diagonal(W2, 0)
+        return W2
+
+    for n, p in [(2, 1.0), (17, 0.3), (60, 0.1), (90, 0.6)]:
+        for W in (random_graph(rng, n, p), positive_cycles_reweighted(n, p)):
+            D = po.fw_serial(W)
+            P = po.predecessors(W, D)
+            for i in range(n):
+                for j in range(n):
+                    if i != j:
+                        assert walk_ok(W, D, P, i, j), (n, i, j)
+    # a planted mistake -- predecessors of the input graph instead of the
+    # closure -- breaks the walk
+    W = random_graph(rng, 30, 0.2)
+    P_bad = po.predecessors(W, W)
+    D = po.fw_serial(W)
+    assert not all(walk_ok(W, D, P_bad, i, j) for i in range(30) for j in range(30) if i != j)
+
+
 def test_philox_known_answers():
     """Philox4x32-10 against known-answer vectors of the Random123
     distribution (kat_vectors: zero, all-ones and the digits of pi)."""

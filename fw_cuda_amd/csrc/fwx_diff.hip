@@ -10,11 +10,11 @@
 //
 // Backward of L = sum(G * D): each D[s][t] is the sum of the weights on the
 // chosen s->t path, so dL/dW[u][v] = sum of G[s][t] over the (s,t) whose path
-// uses edge (u,v).  Per source s (one workgroup): sort the vertices by
-// D[s][.] descending (bitonic sort in LDS), then walk them far to near: a
-// vertex's accumulated gradient g[t] = G[s][t] + sum of its tree children's
-// g is final when it is reached (children are strictly farther, weights > 0),
-// is added to dW[P[s][t]][t] and pushed to its parent.
+// uses edge (u,v).  Per source s (one workgroup) the predecessor row is a
+// tree rooted at s; a vertex's accumulated gradient g[t] = G[s][t] + sum of
+// its children's g is added to dW[P[s][t]][t].  The tree is processed level
+// by level from the deepest (depths by pointer jumping), so all children of
+// a vertex are final before it is pushed to its parent.
 //
 // Without a stored P, the backward recomputes each source block's
 // predecessors as P[s][t] = argmin_u D[s][u] + W[u][t] (u != t): the last
@@ -298,67 +298,131 @@ __global__ __launch_bounds__(256) void k_ffill(float* __restrict__ x, int64_t to
 
 // Backward for sources [s0, s0 + rows) of every graph.  Pr points at the
 // predecessor rows of those sources (row stride n, graph stride pstride).
-// Dynamic LDS: key[n2] float, idx[n2] int, g[n] float, par[n] int.
-__global__ __launch_bounds__(256) void k_fback(const float* __restrict__ D, const int32_t* __restrict__ Pr,
-                                               int64_t pstride, const float* __restrict__ G,
-                                               float* __restrict__ dW, int32_t n, int32_t n2, int32_t s0) {
-  extern __shared__ unsigned char smem[];
-  float* key = reinterpret_cast<float*>(smem);
-  int32_t* idx = reinterpret_cast<int32_t*>(key + n2);
-  float* gacc = reinterpret_cast<float*>(idx + n2);
-  int32_t* par = reinterpret_cast<int32_t*>(gacc + n);
-  const int64_t g = blockIdx.y;
+//
+// Level-synchronous tree accumulation, one 1024-thread workgroup per source
+// s.  The predecessor row is a tree rooted at s; a vertex's gradient
+// g[t] = G[s][t] + sum of its children's g is final once every deeper level
+// is done, so:
+//   1. depths by pointer jumping (ceil(log2 n) rounds: dep += dep[anc],
+//      anc = anc[anc]);
+//   2. vertices bucketed by depth (counting sort in LDS);
+//   3. from the deepest level up: each vertex adds g[t] to its parent's g
+//      (LDS float atomic) and to dW[P[s][t]][t] (global atomic).
+// Work O(n log n) per source with (log n + height) barriers, instead of a
+// sort of all n keys followed by one lane walking n vertices.
+// Dynamic LDS, 14 bytes per vertex + 8: par, dep, aux (int16: vertex ids
+// and depths < 2^15), g (float), off (int32, n + 2).
+constexpr int kBackThreads = 1024;
+
+__global__ __launch_bounds__(kBackThreads) void k_fback(const float* __restrict__ D, const int32_t* __restrict__ Pr,
+                                                        int64_t pstride, const float* __restrict__ G,
+                                                        float* __restrict__ dW, int32_t n, int32_t s0) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  float* g = reinterpret_cast<float*>(smem);
+  int32_t* off = reinterpret_cast<int32_t*>(g + n);
+  int16_t* par = reinterpret_cast<int16_t*>(off + n + 2);
+  int16_t* dep = par + n;
+  int16_t* aux = dep + n;
+  __shared__ int32_t s_h, s_changed[2];
+  const int64_t gi = blockIdx.y;
   const int32_t lr = blockIdx.x;
   const int32_t src = s0 + lr;
   const int64_t nn = (int64_t)n * n;
-  const float* Drow = D + g * nn + (int64_t)src * n;
-  const float* Grow = G + g * nn + (int64_t)src * n;
-  const int32_t* Prow = Pr + g * pstride + (int64_t)lr * n;
-  float* dWg = dW + g * nn;
+  const float* Drow = D + gi * nn + (int64_t)src * n;
+  const float* Grow = G + gi * nn + (int64_t)src * n;
+  const int32_t* Prow = Pr + gi * pstride + (int64_t)lr * n;
+  float* dWg = dW + gi * nn;
   const int tid = threadIdx.x;
-  for (int t = tid; t < n2; t += 256) {
-    if (t < n) {
-      const float d = Drow[t];
-      const int32_t p = Prow[t];
-      key[t] = (t != src && p >= 0 && d < kFInf) ? d : -kFInf;
-      gacc[t] = Grow[t];
-      par[t] = p;
-    } else {
-      key[t] = -kFInf;
-    }
-    idx[t] = t;
+  if (tid == 0) {
+    s_h = 0;
+    s_changed[0] = 0;
+    s_changed[1] = 0;
   }
+  for (int t = tid; t < n; t += kBackThreads) {
+    const int32_t p = Prow[t];
+    const bool reach = t != src && p >= 0 && Drow[t] < kFInf;
+    par[t] = (int16_t)(reach ? p : -1);
+    dep[t] = (int16_t)(t == src ? 0 : (reach ? 1 : -1));
+    aux[t] = (int16_t)(reach ? p : -1);
+    g[t] = Grow[t];
+  }
+  for (int d = tid; d < n + 2; d += kBackThreads) off[d] = 0;
   __syncthreads();
-  // bitonic sort, descending by key
-  for (int k = 2; k <= n2; k <<= 1) {
-    for (int j = k >> 1; j > 0; j >>= 1) {
-      for (int i = tid; i < n2; i += 256) {
-        const int l = i ^ j;
-        if (l > i) {
-          const bool desc = (i & k) == 0;
-          const float ki = key[i], kl = key[l];
-          if (desc ? (ki < kl) : (ki > kl)) {
-            key[i] = kl;
-            key[l] = ki;
-            const int32_t t = idx[i];
-            idx[i] = idx[l];
-            idx[l] = t;
-          }
+  // 1. pointer jumping: after it, aux[t] == src and dep[t] = hops for every
+  // reachable t.  Rounds are synchronous (all reads, barrier, all writes,
+  // barrier); the "anything moved" flag of round r lives in slot r & 1 and
+  // is cleared for round r + 1 only after every thread has passed round r's
+  // first barrier, i.e. finished reading the slot of round r - 1.
+  constexpr int kPer = 8;  // n <= 8192 = 8 * 1024
+  for (int round = 0; round < 16; ++round) {
+    int16_t nd[kPer], na[kPer];
+    int changed = 0;
+#pragma unroll
+    for (int e = 0; e < kPer; ++e) {
+      const int t = tid + e * kBackThreads;
+      nd[e] = 0;
+      na[e] = -1;
+      if (t < n) {
+        const int a = aux[t];
+        if (a >= 0 && a != src) {
+          nd[e] = dep[a];
+          na[e] = aux[a];
+          changed = 1;
         }
       }
-      __syncthreads();
+    }
+    __syncthreads();
+    if (changed) s_changed[round & 1] = 1;
+#pragma unroll
+    for (int e = 0; e < kPer; ++e) {
+      const int t = tid + e * kBackThreads;
+      if (t < n && na[e] >= 0) {
+        dep[t] = (int16_t)(dep[t] + nd[e]);
+        aux[t] = na[e];
+      }
+    }
+    __syncthreads();
+    const int any = s_changed[round & 1];
+    if (tid == 0) s_changed[(round + 1) & 1] = 0;
+    if (!any) break;
+  }
+  // 2. counting sort by depth: off[d] counts level d, then exclusive prefix
+  // sums (off[0] = 0 stays), then scatter into aux (order by level)
+  for (int t = tid; t < n; t += kBackThreads) {
+    const int d = dep[t];
+    if (d > 0 && d <= n) {
+      atomicAdd(&off[d], 1);
+      atomicMax(&s_h, d);
     }
   }
-  // far-to-near accumulation: one lane walks the order
+  __syncthreads();
+  const int H = s_h;
   if (tid == 0) {
-    for (int q = 0; q < n; ++q) {
-      if (!(key[q] > -kFInf)) break;
-      const int32_t t = idx[q];
-      const float gt = gacc[t];
-      const int32_t p = par[t];
-      gacc[p] += gt;
+    int run = 0;
+    for (int d = 1; d <= H; ++d) {  // start of level d; serial: H is the tree height
+      const int c = off[d];
+      off[d] = run;
+      run += c;
+    }
+  }
+  __syncthreads();
+  for (int t = tid; t < n; t += kBackThreads) {
+    const int d = dep[t];
+    if (d > 0 && d <= n) aux[atomicAdd(&off[d], 1)] = (int16_t)t;
+  }
+  __syncthreads();
+  // now level d occupies [off[d - 1], off[d]) of aux (off[0] = 0)
+  // 3. deepest level first
+  for (int d = H; d >= 1; --d) {
+    const int q0 = off[d - 1], q1 = off[d];
+    for (int q = q0 + tid; q < q1; q += kBackThreads) {
+      const int t = aux[q];
+      const int p = par[t];
+      const float gt = g[t];
+      atomicAdd(&g[p], gt);
       atomicAdd(&dWg[(int64_t)p * n + t], gt);
     }
+    __syncthreads();
   }
 }
 
@@ -382,7 +446,7 @@ int forward_impl(const float* W, float* D, int32_t* P, int32_t n, int32_t batch,
   return FWX_OK;
 }
 
-constexpr int32_t kDiffMaxN = 8192;  // backward keeps 4 words per vertex in LDS
+constexpr int32_t kDiffMaxN = 8192;  // backward keeps 14 bytes per vertex in LDS
 
 int check_diff(const void* a, const void* b, int32_t n, int32_t batch) {
   if (!a || !b) return fail(FWX_ERR_INVALID_ARG, "null matrix pointer");
@@ -420,14 +484,12 @@ int fwx_diff_backward(const float* d_W, const float* d_D, const int32_t* d_P, co
   const int64_t total = (int64_t)n * n * batch;
   k_ffill<<<(int)std::min<int64_t>((total + 255) / 256, 65536), 256, 0, st>>>(d_dW, total);
   FWX_LAUNCHED("k_ffill");
-  int32_t n2 = 1;
-  while (n2 < n) n2 <<= 1;
-  const size_t lds = (size_t)n2 * 8 + (size_t)n * 8;
+  const size_t lds = (size_t)n * 14 + 8;
   if (lds > 65536)
     FWX_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fback),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   if (d_P) {
-    k_fback<<<dim3(n, batch), 256, lds, st>>>(d_D, d_P, (int64_t)n * n, d_G, d_dW, n, n2, 0);
+    k_fback<<<dim3(n, batch), kBackThreads, lds, st>>>(d_D, d_P, (int64_t)n * n, d_G, d_dW, n, 0);
     FWX_LAUNCHED("k_fback");
   } else {
     // source blocks of 256 rows; workspace 256 x n x batch int32
@@ -441,7 +503,7 @@ int fwx_diff_backward(const float* d_W, const float* d_D, const int32_t* d_P, co
       k_fpred<<<dim3(((r + FT - 1) / FT) * nt, batch), 256, 0, st>>>(d_W, d_D, ws, n, s0, r);
       hipError_t e = hipGetLastError();
       if (e == hipSuccess) {
-        k_fback<<<dim3(r, batch), 256, lds, st>>>(d_D, ws, (int64_t)r * n, d_G, d_dW, n, n2, s0);
+        k_fback<<<dim3(r, batch), kBackThreads, lds, st>>>(d_D, ws, (int64_t)r * n, d_G, d_dW, n, s0);
         e = hipGetLastError();
       }
       if (e != hipSuccess) rc = hip_fail(e, "k_fpred/k_fback", __FILE__, __LINE__);

@@ -68,6 +68,39 @@ def test_forward_and_backward_small():
     assert (dW[~np.isfinite(W)] == 0).all()
 
 
+def test_backward_deep_trees():
+    """Shortest-path trees hundreds of levels deep (a ring with a few light
+    chords, and a bare chain): the level-by-level accumulation must give the
+    path-enumeration gradient of the GPU's own tree."""
+    import torch
+    from fw_cuda_amd.autograd import forward_with_pred
+    rng = np.random.default_rng(12)
+    n = 600
+    W = np.full((2, n, n), np.inf, dtype=np.float32)
+    for b in range(2):
+        np.fill_diagonal(W[b], 0.0)
+        for v in range(n - 1):
+            W[b, v, v + 1] = rng.uniform(0.5, 1.5)
+        if b == 0:  # ring + sparse chords
+            W[b, n - 1, 0] = 1.0
+            for _ in range(20):
+                u, v = rng.integers(0, n, 2)
+                if u != v:
+                    W[b, u, v] = rng.uniform(5.0, 10.0)
+    G = rng.standard_normal((2, n, n)).astype(np.float32)
+    _, Pg = forward_with_pred(torch.from_numpy(W).cuda())
+    Pg = Pg.cpu().numpy()
+    _, dW = gpu_fwd_bwd(W, G)
+    for b in range(2):
+        ref = po.grad_paths(Pg[b], G[b])
+        # fp32 sums over subtrees of up to n vertices in different orders:
+        # the error scales with the partial sums, so compare globally
+        err = float(np.linalg.norm(dW[b] - ref) / np.linalg.norm(ref))
+        mx = float(np.abs(dW[b] - ref).max() / np.abs(ref).max())
+        say("deep tree", b, "rel L2", err, "max rel", mx)
+        assert err < 1e-5 and mx < 1e-4
+
+
 def test_backward_recomputed_predecessors_match_stored():
     W, G = graphs(3, 300, 0.3, 2)
     _, g1 = gpu_fwd_bwd(W, G, keep_pred=True)

@@ -433,7 +433,7 @@ static int run_blocked_lookahead(int32_t* d, int32_t n, int32_t kappa, hipStream
 
 // One non-blocking side stream per device (default priority), created on
 // first use and kept for the life of the process.
-static int side_stream(hipStream_t* out) {
+int side_stream(hipStream_t* out) {
   static std::mutex mu;
   static hipStream_t streams[64] = {};
   int dev = 0;

@@ -24,6 +24,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 
 #include "fwx_internal.hpp"
 
@@ -234,16 +235,40 @@ __global__ __launch_bounds__(256) void k_fpanel(float* __restrict__ D, int32_t* 
   fstore<PRED>(C, acc, pacc);
 }
 
+// Rest tiles of stage c.  mode 0: every (i, j) with i, j != c, (N-1)^2
+// blocks.  Look-ahead split (c + 1 < N): mode 1, the band that stage c+1's
+// closure and panels read -- row c+1 (j != c, N-1 blocks) and column c+1
+// (i != c, c+1, N-2 blocks); mode 2, the others (i, j not in {c, c+1}),
+// (N-2)^2 blocks.
 template <bool PRED>
 __global__ __launch_bounds__(256) void k_frest(float* __restrict__ D, int32_t* __restrict__ P, int32_t n,
-                                               int32_t c, int32_t N) {
+                                               int32_t c, int32_t N, int32_t mode) {
   __shared__ FSmem s;
   const int64_t g = blockIdx.y;
   float* Dg = D + g * (int64_t)n * n;
   int32_t* Pg = PRED ? P + g * (int64_t)n * n : nullptr;
-  int i = blockIdx.x / (N - 1), j = blockIdx.x % (N - 1);
-  i += (i >= c);
-  j += (j >= c);
+  const int b = blockIdx.x;
+  int i, j;
+  if (mode == 0) {
+    i = b / (N - 1);
+    j = b % (N - 1);
+    i += (i >= c);
+    j += (j >= c);
+  } else if (mode == 1) {
+    if (b < N - 1) {
+      i = c + 1;
+      j = b + (b >= c);
+    } else {
+      i = b - (N - 1);
+      i = i < c ? i : i + 2;
+      j = c + 1;
+    }
+  } else {
+    i = b / (N - 2);
+    j = b % (N - 2);
+    i = i < c ? i : i + 2;
+    j = j < c ? j : j + 2;
+  }
   const FView A = fview(Dg, Pg, n, i * FT, c * FT);
   const FView B = fview(Dg, Pg, n, c * FT, j * FT);
   const FView C = fview(Dg, Pg, n, i * FT, j * FT);
@@ -426,6 +451,29 @@ __global__ __launch_bounds__(kBackThreads) void k_fback(const float* __restrict_
   }
 }
 
+// FWX_LOOKAHEAD=0 selects the single-stream schedule (read on every call).
+static bool diff_lookahead() {
+  const char* e = std::getenv("FWX_LOOKAHEAD");
+  return !(e && e[0] == '0');
+}
+
+template <bool PRED>
+int lead_round(float* D, int32_t* P, int32_t n, int32_t c, int32_t N, int32_t batch, hipStream_t st) {
+  k_fdiag<PRED><<<dim3(1, batch), 256, 0, st>>>(D, P, n, c);
+  FWX_LAUNCHED("k_fdiag");
+  if (N > 1) {
+    k_fpanel<PRED><<<dim3(2 * (N - 1), batch), 256, 0, st>>>(D, P, n, c, N);
+    FWX_LAUNCHED("k_fpanel");
+  }
+  return FWX_OK;
+}
+
+// Three-phase blocked FW over 64-tiles.  With the look-ahead (two streams),
+// stage c's rest pass first updates the band stage c+1's closure and panels
+// read, then the side stream runs stage c+1's closure and panels while the
+// main stream updates the other tiles of stage c; the main stream waits for
+// them before stage c+1's band.  Every tile sees the stages in order, so the
+// result is the same bits as the single-stream schedule.
 template <bool PRED>
 int forward_impl(const float* W, float* D, int32_t* P, int32_t n, int32_t batch, hipStream_t st) {
   const int64_t total = (int64_t)n * n * batch;
@@ -433,17 +481,53 @@ int forward_impl(const float* W, float* D, int32_t* P, int32_t n, int32_t batch,
   k_finit<PRED><<<fillblocks, 256, 0, st>>>(W, D, P, n, total);
   FWX_LAUNCHED("k_finit");
   const int N = (n + FT - 1) / FT;
-  for (int c = 0; c < N; ++c) {
-    k_fdiag<PRED><<<dim3(1, batch), 256, 0, st>>>(D, P, n, c);
-    FWX_LAUNCHED("k_fdiag");
-    if (N > 1) {
-      k_fpanel<PRED><<<dim3(2 * (N - 1), batch), 256, 0, st>>>(D, P, n, c, N);
-      FWX_LAUNCHED("k_fpanel");
-      k_frest<PRED><<<dim3((N - 1) * (N - 1), batch), 256, 0, st>>>(D, P, n, c, N);
-      FWX_LAUNCHED("k_frest");
+  if (N < 3 || !diff_lookahead()) {
+    for (int c = 0; c < N; ++c) {
+      if (int rc = lead_round<PRED>(D, P, n, c, N, batch, st)) return rc;
+      if (N > 1) {
+        k_frest<PRED><<<dim3((N - 1) * (N - 1), batch), 256, 0, st>>>(D, P, n, c, N, 0);
+        FWX_LAUNCHED("k_frest");
+      }
+    }
+    return FWX_OK;
+  }
+  hipStream_t side;
+  if (int rc = side_stream(&side)) return rc;
+  hipEvent_t evA, evB;
+  FWX_HIP(hipEventCreateWithFlags(&evA, hipEventDisableTiming));
+  hipError_t e = hipEventCreateWithFlags(&evB, hipEventDisableTiming);
+  if (e != hipSuccess) {
+    (void)hipEventDestroy(evA);
+    return hip_fail(e, "hipEventCreateWithFlags", __FILE__, __LINE__);
+  }
+  int rc = lead_round<PRED>(D, P, n, 0, N, batch, st);
+  for (int c = 0; c < N && rc == FWX_OK; ++c) {
+    if (c + 1 < N) {
+      k_frest<PRED><<<dim3(2 * N - 3, batch), 256, 0, st>>>(D, P, n, c, N, 1);
+      e = hipGetLastError();
+      if (e == hipSuccess) e = hipEventRecord(evA, st);
+      if (e == hipSuccess) e = hipStreamWaitEvent(side, evA, 0);
+      if (e != hipSuccess) {
+        rc = hip_fail(e, "diff look-ahead band", __FILE__, __LINE__);
+        break;
+      }
+      if ((rc = lead_round<PRED>(D, P, n, c + 1, N, batch, side))) break;
+      e = hipEventRecord(evB, side);
+      if (e == hipSuccess) {
+        k_frest<PRED><<<dim3((N - 2) * (N - 2), batch), 256, 0, st>>>(D, P, n, c, N, 2);
+        e = hipGetLastError();
+      }
+      if (e == hipSuccess) e = hipStreamWaitEvent(st, evB, 0);
+      if (e != hipSuccess) rc = hip_fail(e, "diff look-ahead rest", __FILE__, __LINE__);
+    } else {
+      k_frest<PRED><<<dim3((N - 1) * (N - 1), batch), 256, 0, st>>>(D, P, n, c, N, 0);
+      e = hipGetLastError();
+      if (e != hipSuccess) rc = hip_fail(e, "k_frest", __FILE__, __LINE__);
     }
   }
-  return FWX_OK;
+  (void)hipEventDestroy(evA);
+  (void)hipEventDestroy(evB);
+  return rc;
 }
 
 constexpr int32_t kDiffMaxN = 8192;  // backward keeps 14 bytes per vertex in LDS

@@ -48,6 +48,8 @@ int prof_close(hipStream_t st, hipEvent_t a, int cls, double work);
 // [0, 2^30), i.e. the pk64 relaxation form is exact for the whole run.
 // Synchronises `st`.
 int values_in_pk64_range(const int32_t* d, int64_t count, hipStream_t st, bool* ok);
+// The per-device second stream of the look-ahead schedules (fwx_blocked.hip).
+int side_stream(hipStream_t* out);
 // Any D[i][i] < 0 on a device matrix (fwx_range.hip); synchronises `st`.
 int device_negative_cycle(const int32_t* d, int32_t n, hipStream_t st, bool* neg);
 

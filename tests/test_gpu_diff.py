@@ -68,6 +68,26 @@ def test_forward_and_backward_small():
     assert (dW[~np.isfinite(W)] == 0).all()
 
 
+@pytest.mark.parametrize("batch,n", [(3, 129), (2, 300), (1, 640), (4, 200)])
+def test_forward_schedules_agree(monkeypatch, batch, n):
+    """The look-ahead (two-stream) forward and the single-stream forward give
+    the same bits for D and P."""
+    import torch
+    from fw_cuda_amd.autograd import forward_with_pred
+    W, _ = graphs(batch, n, 0.3, n + batch)
+    Wt = torch.from_numpy(W).cuda()
+    out = {}
+    for la in ("0", "1"):
+        monkeypatch.setenv("FWX_LOOKAHEAD", la)
+        D, P = forward_with_pred(Wt)
+        out[la] = (D.cpu().numpy(), P.cpu().numpy())
+    assert np.array_equal(out["0"][0], out["1"][0], equal_nan=True)
+    assert (out["0"][1] == out["1"][1]).all()
+    Do, _ = po.fw_f32_pred(W[0])
+    fin = np.isfinite(Do)
+    assert np.allclose(out["1"][0][0][fin], Do[fin], rtol=2e-6, atol=0)
+
+
 def test_backward_deep_trees():
     """Shortest-path trees hundreds of levels deep (a ring with a few light
     chords, and a bare chain): the level-by-level accumulation must give the

@@ -48,15 +48,17 @@ int enqueue_baseline(int32_t* d, int32_t n, hipStream_t st) {
   const dim3 block(64, 4);
   const dim3 grid((n + 255) / 256, (n + 3) / 4);
   const double bytes = 4.0 * (double)n * (double)n;  // every cell is read once per k
+  // While profiling, one event per sweep: each ends the previous sweep's
+  // group and starts the next (the group time is launch to launch).
+  hipEvent_t ev;
+  if (int rc = prof_open(st, &ev)) return rc;
   for (int32_t k = 0; k < n; ++k) {
-    hipEvent_t ev;
-    if (int rc = prof_open(st, &ev)) return rc;
     if (vec)
       k_baseline_step<true><<<grid, block, 0, st>>>(d, n, k);
     else
       k_baseline_step<false><<<grid, block, 0, st>>>(d, n, k);
     FWX_LAUNCHED("k_baseline_step");
-    if (int rc = prof_close(st, ev, FWX_PROF_BASELINE, bytes)) return rc;
+    if (int rc = prof_chain(st, &ev, FWX_PROF_BASELINE, bytes)) return rc;
   }
   return FWX_OK;
 }

@@ -43,10 +43,17 @@ namespace fwx {
 constexpr int kChunk = 32;
 constexpr int kDiagThreads = 512;
 
+// The lead kernels (diag, panel, cross) form the look-ahead schedule's
+// dependent chain and share CUs with the concurrent rest pass; raising their
+// waves' issue priority lets them take the SIMDs first where both are
+// resident.  A wave's own priority, set by the wave; it ends with the wave.
+__device__ __forceinline__ void lead_wave_priority() { __builtin_amdgcn_s_setprio(3); }
+
 // ---------------------------------------------------------------------------
 // diag: close tile (c, c).
 template <int T, int NTH>
-__global__ __launch_bounds__(NTH) void k_diag(int32_t* __restrict__ D, int32_t n, int32_t c) {
+__global__ __launch_bounds__(NTH) void k_diag(int32_t* __restrict__ D, int32_t n, int32_t c, int32_t prio) {
+  if (prio) lead_wave_priority();
   close_tile<T, NTH>(make_view(D, n, c * T, c * T));
 }
 
@@ -56,7 +63,8 @@ __global__ __launch_bounds__(NTH) void k_diag(int32_t* __restrict__ D, int32_t n
 // staged chunk by chunk before this block stores anything.
 template <int T, bool VEC, bool PK>
 __global__ __launch_bounds__(256) void k_panel(int32_t* __restrict__ D, int32_t n, int32_t c,
-                                               int32_t N) {
+                                               int32_t N, int32_t prio) {
+  if (prio) lead_wave_priority();
   __shared__ TileSmem<T, T, kChunk> s;
   const int b = blockIdx.x;
   const int32_t base = c * T;
@@ -88,7 +96,8 @@ __global__ __launch_bounds__(256) void k_panel(int32_t* __restrict__ D, int32_t 
 // redundant T^3 closures (negligible next to the round's products).
 template <int T, bool VEC, bool PK>
 __global__ __launch_bounds__(256) void k_dpanel(int32_t* __restrict__ D, int32_t n, int32_t c,
-                                                int32_t N) {
+                                                int32_t N, int32_t prio) {
+  if (prio) lead_wave_priority();
   close_tile<T>(make_view(D, n, c * T, c * T));
   __syncthreads();
   __shared__ TileSmem<T, T, kChunk> s;
@@ -118,7 +127,8 @@ __global__ __launch_bounds__(256) void k_dpanel(int32_t* __restrict__ D, int32_t
 // (N-B)(B-1) tiles.
 template <int T, bool VEC, bool PK>
 __global__ __launch_bounds__(256) void k_cross(int32_t* __restrict__ D, int32_t n, int32_t c,
-                                               int32_t kb, int32_t Bst, int32_t N) {
+                                               int32_t kb, int32_t Bst, int32_t N, int32_t prio) {
+  if (prio) lead_wave_priority();
   __shared__ TileSmem<T, T, kChunk> s;
   int b = blockIdx.x;
   const int l = c - kb;
@@ -245,35 +255,35 @@ static int diag_threads() {
 }
 
 template <int T>
-static int diag_launch(int32_t* d, int32_t n, int32_t c, hipStream_t st) {
+static int diag_launch(int32_t* d, int32_t n, int32_t c, int prio, hipStream_t st) {
   const int nth = T == 32 ? 256 : diag_threads();
   if (nth == 1024)
-    k_diag<T, T == 32 ? 256 : 1024><<<1, T == 32 ? 256 : 1024, 0, st>>>(d, n, c);
+    k_diag<T, T == 32 ? 256 : 1024><<<1, T == 32 ? 256 : 1024, 0, st>>>(d, n, c, prio);
   else if (nth == 512)
-    k_diag<T, T == 32 ? 256 : 512><<<1, T == 32 ? 256 : 512, 0, st>>>(d, n, c);
+    k_diag<T, T == 32 ? 256 : 512><<<1, T == 32 ? 256 : 512, 0, st>>>(d, n, c, prio);
   else
-    k_diag<T, 256><<<1, 256, 0, st>>>(d, n, c);
+    k_diag<T, 256><<<1, 256, 0, st>>>(d, n, c, prio);
   FWX_LAUNCHED("k_diag");
   return FWX_OK;
 }
 
 template <int T, bool VEC, bool PK>
-static int lead_round(int32_t* d, int32_t n, int32_t c, int32_t kb, int32_t B, int32_t N,
+static int lead_round(int32_t* d, int32_t n, int32_t c, int32_t kb, int32_t B, int32_t N, int prio,
                       hipStream_t st) {
   if (N > 1 && fused_dpanel()) {
-    k_dpanel<T, VEC, PK><<<2 * (N - 1), 256, 0, st>>>(d, n, c, N);
+    k_dpanel<T, VEC, PK><<<2 * (N - 1), 256, 0, st>>>(d, n, c, N, prio);
     FWX_LAUNCHED("k_dpanel");
   } else {
-    if (int rc = diag_launch<T>(d, n, c, st)) return rc;
+    if (int rc = diag_launch<T>(d, n, c, prio, st)) return rc;
     if (N > 1) {
-      k_panel<T, VEC, PK><<<2 * (N - 1), 256, 0, st>>>(d, n, c, N);
+      k_panel<T, VEC, PK><<<2 * (N - 1), 256, 0, st>>>(d, n, c, N, prio);
       FWX_LAUNCHED("k_panel");
     }
   }
   if (B > 1) {
     const int cnt = (B - 1) * (2 * N - B - 1);
     if (cnt > 0) {
-      k_cross<T, VEC, PK><<<cnt, 256, 0, st>>>(d, n, c, kb, B, N);
+      k_cross<T, VEC, PK><<<cnt, 256, 0, st>>>(d, n, c, kb, B, N, prio);
       FWX_LAUNCHED("k_cross");
     }
   }
@@ -353,34 +363,54 @@ static int band_dispatch(int R, int32_t* d, int32_t n, int32_t s0, int32_t S, in
   return band_pass<32, VEC, PK>(d, n, s0, S, en, fn, st);
 }
 
-// The lead rounds of one stage, bracketed for the profiler.
+// The lead rounds of one stage.
 template <int T, bool VEC, bool PK>
-static int lead_stage(int32_t* d, int32_t n, int32_t kb, int32_t B, int32_t N, hipStream_t st) {
-  const double L = (double)min(B * T, n - kb * T), nn = (double)n;
-  hipEvent_t ev;
-  if (int rc = prof_open(st, &ev)) return rc;
+static int lead_rounds(int32_t* d, int32_t n, int32_t kb, int32_t B, int32_t N, int prio, hipStream_t st) {
   for (int l = 0; l < B; ++l)
-    if (int rc = lead_round<T, VEC, PK>(d, n, kb + l, kb, B, N, st)) return rc;
-  return prof_close(st, ev, FWX_PROF_LEAD, (nn * nn - (nn - L) * (nn - L)) * L);
+    if (int rc = lead_round<T, VEC, PK>(d, n, kb + l, kb, B, N, prio, st)) return rc;
+  return FWX_OK;
 }
 
-// Plain schedule: lead(s), rest(s), lead(s+1), ... on one stream.
+static double lead_work(int32_t n, int32_t T, int32_t kb, int32_t B) {
+  const double L = (double)min(B * T, n - kb * T), nn = (double)n;
+  return (nn * nn - (nn - L) * (nn - L)) * L;
+}
+
+// The lead rounds of one stage, bracketed for the profiler.
+template <int T, bool VEC, bool PK>
+static int lead_stage(int32_t* d, int32_t n, int32_t kb, int32_t B, int32_t N, int prio, hipStream_t st) {
+  hipEvent_t ev;
+  if (int rc = prof_open(st, &ev)) return rc;
+  if (int rc = lead_rounds<T, VEC, PK>(d, n, kb, B, N, prio, st)) return rc;
+  return prof_close(st, ev, FWX_PROF_LEAD, lead_work(n, T, kb, B));
+}
+
+// Plain schedule: lead(s), rest(s), lead(s+1), ... on one stream.  While
+// profiling, one event between adjacent groups ends one and starts the next.
 template <int T, bool VEC, bool PK>
 static int run_blocked(int32_t* d, int32_t n, int32_t kappa, hipStream_t st) {
   const int N = (n + T - 1) / T;
   const int span = kappa * T;
   const int R = (span % 128 == 0) ? 128 : (span % 64 == 0) ? 64 : 32;
+  hipEvent_t ev;
+  if (int rc = prof_open(st, &ev)) return rc;
   for (int kb = 0; kb < N; kb += kappa) {
     const int B = min(kappa, N - kb);
     const int s0 = kb * T, S = B * T;
     const double L = (double)min(S, n - s0), nn = (double)n;
-    if (int rc = lead_stage<T, VEC, PK>(d, n, kb, B, N, st)) return rc;
-    hipEvent_t ev;
-    if (int rc = prof_open(st, &ev)) return rc;
+    if (int rc = lead_rounds<T, VEC, PK>(d, n, kb, B, N, 0, st)) return rc;
+    if (int rc = prof_chain(st, &ev, FWX_PROF_LEAD, lead_work(n, T, kb, B))) return rc;
     if (int rc = rest_dispatch<VEC, PK>(R, d, n, s0, S, (S + R - 1) / R, st)) return rc;
-    if (int rc = prof_close(st, ev, FWX_PROF_REST, (nn - L) * (nn - L) * L)) return rc;
+    if (int rc = prof_chain(st, &ev, FWX_PROF_REST, (nn - L) * (nn - L) * L)) return rc;
   }
   return FWX_OK;
+}
+
+// FWX_LEAD_PRIO=0 keeps the look-ahead's lead kernels at the default wave
+// priority (read on every call).
+static bool lead_priority_enabled() {
+  const char* e = std::getenv("FWX_LEAD_PRIO");
+  return !(e && e[0] == '0');
 }
 
 // Look-ahead schedule.  Stage s+1's lead rounds only read and write tiles
@@ -398,34 +428,95 @@ static int run_blocked_lookahead(int32_t* d, int32_t n, int32_t kappa, hipStream
   const int span = kappa * T;
   const int R = (span % 128 == 0) ? 128 : (span % 64 == 0) ? 64 : 32;
   const double nn = (double)n;
-  if (int rc = lead_stage<T, VEC, PK>(d, n, 0, min(kappa, N), N, st)) return rc;
+  const int prio = lead_priority_enabled() ? 1 : 0;
+  // While profiling, `ev` chains the caller's stream: the event that ends
+  // one group starts the next (so a band group also holds the wait for the
+  // side stream's lead), and the band's end event is the one the side stream
+  // waits on.  Two markers per stage on the caller's stream, as unprofiled.
+  hipEvent_t ev;
+  if (int rc = prof_open(st, &ev)) return rc;
+  const int B0 = min(kappa, N);
+  if (int rc = lead_rounds<T, VEC, PK>(d, n, 0, B0, N, 0, st)) return rc;
+  if (int rc = prof_chain(st, &ev, FWX_PROF_LEAD, lead_work(n, T, 0, B0))) return rc;
   for (int kb = 0; kb < N; kb += kappa) {
     const int B = min(kappa, N - kb);
     const int s0 = kb * T, S = B * T;
     const double L = (double)min(S, n - s0);
     const int en = (S + R - 1) / R;
     const int kb2 = kb + kappa;
-    hipEvent_t ev;
     if (kb2 < N) {
       const int B2 = min(kappa, N - kb2);
       const int fn = (B2 * T + R - 1) / R;
       const double L2 = (double)min(B2 * T, n - kb2 * T);
-      if (int rc = prof_open(st, &ev)) return rc;
       if (int rc = band_dispatch<VEC, PK>(R, d, n, s0, S, en, fn, st)) return rc;
-      if (int rc = prof_close(st, ev, FWX_PROF_BAND, ((nn - L) * (nn - L) - (nn - L - L2) * (nn - L - L2)) * L))
+      if (int rc = prof_chain(st, &ev, FWX_PROF_BAND, ((nn - L) * (nn - L) - (nn - L - L2) * (nn - L - L2)) * L))
         return rc;
-      FWX_HIP(hipEventRecord(evA, st));
-      FWX_HIP(hipStreamWaitEvent(side, evA, 0));
-      if (int rc = lead_stage<T, VEC, PK>(d, n, kb2, B2, N, side)) return rc;
+      if (!ev) FWX_HIP(hipEventRecord(evA, st));
+      FWX_HIP(hipStreamWaitEvent(side, ev ? ev : evA, 0));
+      if (int rc = lead_stage<T, VEC, PK>(d, n, kb2, B2, N, prio, side)) return rc;
       FWX_HIP(hipEventRecord(evB, side));
-      if (int rc = prof_open(st, &ev)) return rc;
       if (int rc = rest_dispatch<VEC, PK>(R, d, n, s0, S, en + fn, st)) return rc;
-      if (int rc = prof_close(st, ev, FWX_PROF_REST, (nn - L - L2) * (nn - L - L2) * L)) return rc;
+      if (int rc = prof_chain(st, &ev, FWX_PROF_REST, (nn - L - L2) * (nn - L - L2) * L)) return rc;
       FWX_HIP(hipStreamWaitEvent(st, evB, 0));
     } else {
-      if (int rc = prof_open(st, &ev)) return rc;
       if (int rc = rest_dispatch<VEC, PK>(R, d, n, s0, S, en, st)) return rc;
-      if (int rc = prof_close(st, ev, FWX_PROF_REST, (nn - L) * (nn - L) * L)) return rc;
+      if (int rc = prof_chain(st, &ev, FWX_PROF_REST, (nn - L) * (nn - L) * L)) return rc;
+    }
+  }
+  return FWX_OK;
+}
+
+// Side-band schedule: the caller's stream runs only rest passes.  The side
+// stream runs, per stage s, band(s) -- the rest(s) tiles with a row or a
+// column in band s+1 -- and then lead(s+1); the caller's stream runs the
+// other rest(s) tiles.  The two write disjoint cells and neither writes what
+// the other reads: rest_other(s) reads band s rows and columns outside band
+// s+1 and writes cells outside bands s and s+1; band(s) and lead(s+1) write
+// only cells with a row or a column in band s+1 (none in band s), and read
+// band s and band s+1.  Ordering: rest_other(s) waits for lead(s) (event L);
+// band(s+1) waits for rest_other(s) (event R), which wrote the cells of the
+// band s+2 cross that band(s+1) updates next.
+template <int T, bool VEC, bool PK>
+static int run_blocked_sideband(int32_t* d, int32_t n, int32_t kappa, hipStream_t st, hipStream_t side,
+                                hipEvent_t evR, hipEvent_t evL) {
+  const int N = (n + T - 1) / T;
+  const int span = kappa * T;
+  const int R = (span % 128 == 0) ? 128 : (span % 64 == 0) ? 64 : 32;
+  const double nn = (double)n;
+  const int prio = lead_priority_enabled() ? 1 : 0;
+  FWX_HIP(hipEventRecord(evR, st));  // the side stream starts after the caller's prior work
+  FWX_HIP(hipStreamWaitEvent(side, evR, 0));
+  if (int rc = lead_stage<T, VEC, PK>(d, n, 0, min(kappa, N), N, prio, side)) return rc;
+  FWX_HIP(hipEventRecord(evL, side));
+  hipEvent_t ev = nullptr;  // profiling chain on the caller's stream
+  for (int kb = 0; kb < N; kb += kappa) {
+    const int B = min(kappa, N - kb);
+    const int s0 = kb * T, S = B * T;
+    const double L = (double)min(S, n - s0);
+    const int en = (S + R - 1) / R;
+    const int kb2 = kb + kappa;
+    FWX_HIP(hipStreamWaitEvent(st, evL, 0));
+    if (kb == 0)
+      if (int rc = prof_open(st, &ev)) return rc;
+    if (kb2 < N) {
+      const int B2 = min(kappa, N - kb2);
+      const int fn = (B2 * T + R - 1) / R;
+      const double L2 = (double)min(B2 * T, n - kb2 * T);
+      hipEvent_t es;
+      if (int rc = prof_open(side, &es)) return rc;
+      if (int rc = band_dispatch<VEC, PK>(R, d, n, s0, S, en, fn, side)) return rc;
+      if (int rc = prof_chain(side, &es, FWX_PROF_BAND, ((nn - L) * (nn - L) - (nn - L - L2) * (nn - L - L2)) * L))
+        return rc;
+      if (int rc = lead_rounds<T, VEC, PK>(d, n, kb2, B2, N, prio, side)) return rc;
+      if (int rc = prof_close(side, es, FWX_PROF_LEAD, lead_work(n, T, kb2, B2))) return rc;
+      FWX_HIP(hipEventRecord(evL, side));
+      if (int rc = rest_dispatch<VEC, PK>(R, d, n, s0, S, en + fn, st)) return rc;
+      if (int rc = prof_chain(st, &ev, FWX_PROF_REST, (nn - L - L2) * (nn - L - L2) * L)) return rc;
+      if (!ev) FWX_HIP(hipEventRecord(evR, st));
+      FWX_HIP(hipStreamWaitEvent(side, ev ? ev : evR, 0));
+    } else {
+      if (int rc = rest_dispatch<VEC, PK>(R, d, n, s0, S, en, st)) return rc;
+      if (int rc = prof_chain(st, &ev, FWX_PROF_REST, (nn - L) * (nn - L) * L)) return rc;
     }
   }
   return FWX_OK;
@@ -445,26 +536,30 @@ int side_stream(hipStream_t* out) {
   return FWX_OK;
 }
 
-// FWX_LOOKAHEAD=0 selects the single-stream schedule (read on every call).
-static bool lookahead_enabled() {
+// FWX_LOOKAHEAD selects the schedule (read on every call): 0 = single
+// stream, 1 = look-ahead with the band on the caller's stream, 2 =
+// side-band (the default).
+static int lookahead_mode() {
   const char* e = std::getenv("FWX_LOOKAHEAD");
-  return !(e && e[0] == '0');
+  return (e && e[0] >= '0' && e[0] <= '2') ? e[0] - '0' : 2;
 }
 
 template <int T, bool VEC, bool PK>
 static int run_any(int32_t* d, int32_t n, int32_t kappa, hipStream_t st) {
   const int N = (n + T - 1) / T;
-  if (!lookahead_enabled() || N <= kappa) return run_blocked<T, VEC, PK>(d, n, kappa, st);
+  const int mode = lookahead_mode();
+  if (mode == 0 || N <= kappa) return run_blocked<T, VEC, PK>(d, n, kappa, st);
   hipStream_t side;
   if (int rc = side_stream(&side)) return rc;
   hipEvent_t evA, evB;
-  FWX_HIP(hipEventCreateWithFlags(&evA, hipEventDisableTiming));
-  hipError_t e = hipEventCreateWithFlags(&evB, hipEventDisableTiming);
+  FWX_HIP(hipEventCreateWithFlags(&evA, event_flags(false)));
+  hipError_t e = hipEventCreateWithFlags(&evB, event_flags(false));
   if (e != hipSuccess) {
     (void)hipEventDestroy(evA);
     return hip_fail(e, "hipEventCreateWithFlags", __FILE__, __LINE__);
   }
-  const int rc = run_blocked_lookahead<T, VEC, PK>(d, n, kappa, st, side, evA, evB);
+  const int rc = mode == 2 ? run_blocked_sideband<T, VEC, PK>(d, n, kappa, st, side, evA, evB)
+                           : run_blocked_lookahead<T, VEC, PK>(d, n, kappa, st, side, evA, evB);
   (void)hipEventDestroy(evA);
   (void)hipEventDestroy(evB);
   return rc;
@@ -510,8 +605,9 @@ namespace {
 struct GraphKey {
   int32_t* d;
   int32_t n, tile, kappa;
-  bool vec, pk, la;
-  int knobs;  // FWX_DPANEL / FWX_REST_VARIANT / FWX_BAND / FWX_DIAG_NTH choices
+  bool vec, pk;
+  int la;
+  int knobs;  // FWX_DPANEL / FWX_REST_VARIANT / FWX_BAND / FWX_DIAG_NTH / FWX_LEAD_PRIO choices
   hipStream_t st;
   int dev;
   bool operator==(const GraphKey& o) const {
@@ -547,8 +643,9 @@ int enqueue_blocked(int32_t* d, int32_t n, int32_t tile, int32_t kappa, hipStrea
   if (n > kGraphMaxN || !graphs_enabled() || profiling()) return enqueue_direct(d, n, tile, kappa, vec, pk, st);
   int dev = 0;
   FWX_HIP(hipGetDevice(&dev));
-  const int knobs = (fused_dpanel() ? 1 : 0) | (rest_variant() << 1) | (band_narrow() ? 8 : 0) | (diag_threads() << 4);
-  const GraphKey key{d, n, tile, kappa, vec, pk, lookahead_enabled(), knobs, st, dev};
+  const int knobs = (fused_dpanel() ? 1 : 0) | (rest_variant() << 1) | (band_narrow() ? 8 : 0) | (diag_threads() << 4) |
+                    (lead_priority_enabled() ? 1 << 20 : 0);
+  const GraphKey key{d, n, tile, kappa, vec, pk, lookahead_mode(), knobs, st, dev};
   std::lock_guard<std::mutex> lk(g_graph_mu);
   auto replay = [&](GraphEntry& e) -> int {
     e.used = ++g_graph_clock;

@@ -494,8 +494,8 @@ int forward_impl(const float* W, float* D, int32_t* P, int32_t n, int32_t batch,
   hipStream_t side;
   if (int rc = side_stream(&side)) return rc;
   hipEvent_t evA, evB;
-  FWX_HIP(hipEventCreateWithFlags(&evA, hipEventDisableTiming));
-  hipError_t e = hipEventCreateWithFlags(&evB, hipEventDisableTiming);
+  FWX_HIP(hipEventCreateWithFlags(&evA, event_flags(false)));
+  hipError_t e = hipEventCreateWithFlags(&evB, event_flags(false));
   if (e != hipSuccess) {
     (void)hipEventDestroy(evA);
     return hip_fail(e, "hipEventCreateWithFlags", __FILE__, __LINE__);

@@ -41,8 +41,16 @@ int enqueue_refshape(int32_t* d, int32_t n, hipStream_t st);
 // profiling is on (else leaves *a null); prof_close records the end event.
 // Profiled work must be launched directly, never captured into a graph.
 bool profiling();
+// Flags for every event the library creates (timing or ordering only).
+unsigned event_flags(bool timing);
 int prof_open(hipStream_t st, hipEvent_t* a);
 int prof_close(hipStream_t st, hipEvent_t a, int cls, double work);
+// prof_chain closes the group started at *a (if profiling) with a new event
+// and makes that event the start of the next group on the same stream: one
+// marker between two adjacent groups instead of two.
+int prof_chain(hipStream_t st, hipEvent_t* a, int cls, double work);
+// Records a fresh timing event on st (released at fwx_profile_end).
+int prof_mark(hipStream_t st, hipEvent_t* e);
 
 // Range check of count cells at d (fwx_range.hip): *ok = every cell in
 // [0, 2^30), i.e. the pk64 relaxation form is exact for the whole run.

@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include <atomic>
+#include <cstdlib>
 #include <mutex>
 #include <vector>
 
@@ -28,6 +29,7 @@ struct ProfRec {
 std::atomic<bool> g_on{false};
 std::mutex g_mu;
 std::vector<ProfRec> g_recs;
+std::vector<hipEvent_t> g_events;  // every event created while profiling
 double g_ms[FWX_PROF_NCLASS];
 double g_work[FWX_PROF_NCLASS];
 int64_t g_groups[FWX_PROF_NCLASS];
@@ -44,30 +46,59 @@ void clear_totals() {
 
 bool profiling() { return g_on.load(std::memory_order_relaxed); }
 
+// Every event here orders or times work on one device, so none needs the
+// default system-scope release (a full L2 writeback and invalidate, about
+// 10 us per record on a solve that keeps a 256 MiB matrix hot).  Events are
+// created with hipEventDisableSystemFence: the device-scope fence of an
+// ordinary kernel boundary.  FWX_EVENT_SYSFENCE=1 restores the default.
+unsigned event_flags(bool timing) {
+  const char* e = std::getenv("FWX_EVENT_SYSFENCE");
+  const bool sys = e && e[0] == '1';
+  return (timing ? 0u : (unsigned)hipEventDisableTiming) | (sys ? 0u : (unsigned)hipEventDisableSystemFence);
+}
+
 int prof_open(hipStream_t st, hipEvent_t* a) {
   *a = nullptr;
   if (!profiling()) return FWX_OK;
-  FWX_HIP(hipEventCreate(a));
-  FWX_HIP(hipEventRecord(*a, st));
+  return prof_mark(st, a);
+}
+
+int prof_mark(hipStream_t st, hipEvent_t* e) {
+  FWX_HIP(hipEventCreateWithFlags(e, event_flags(true)));
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_events.push_back(*e);
+  }
+  FWX_HIP(hipEventRecord(*e, st));
   return FWX_OK;
 }
 
 int prof_close(hipStream_t st, hipEvent_t a, int cls, double work) {
   if (!a) return FWX_OK;
   hipEvent_t b;
-  FWX_HIP(hipEventCreate(&b));
-  FWX_HIP(hipEventRecord(b, st));
+  if (int rc = prof_mark(st, &b)) return rc;
   std::lock_guard<std::mutex> lk(g_mu);
   g_recs.push_back(ProfRec{cls, a, b, work});
   return FWX_OK;
 }
 
-static void prof_release(std::vector<ProfRec>& recs) {
-  for (ProfRec& r : recs) {
-    (void)hipEventDestroy(r.a);
-    (void)hipEventDestroy(r.b);
+int prof_chain(hipStream_t st, hipEvent_t* a, int cls, double work) {
+  if (!*a) return FWX_OK;
+  hipEvent_t b;
+  if (int rc = prof_mark(st, &b)) return rc;
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_recs.push_back(ProfRec{cls, *a, b, work});
   }
-  recs.clear();
+  *a = b;
+  return FWX_OK;
+}
+
+// Caller holds g_mu.
+static void prof_release() {
+  for (hipEvent_t e : g_events) (void)hipEventDestroy(e);
+  g_events.clear();
+  g_recs.clear();
 }
 
 }  // namespace fwx
@@ -79,7 +110,7 @@ extern "C" {
 int fwx_profile_begin(void) {
   clear_error();
   std::lock_guard<std::mutex> lk(g_mu);
-  prof_release(g_recs);
+  prof_release();
   clear_totals();
   g_on.store(true);
   return FWX_OK;
@@ -103,7 +134,7 @@ int fwx_profile_end(double* ms, double* work, int64_t* groups) {
       g_groups[r.cls] += 1;
     }
   }
-  prof_release(g_recs);
+  prof_release();
   for (int c = 0; c < FWX_PROF_NCLASS; ++c) {
     if (ms) ms[c] = g_ms[c];
     if (work) work[c] = g_work[c];

@@ -157,15 +157,21 @@ def test_large_values_take_exact_path():
         assert (gpu(g, algo, tile, kappa) == expect).all(), (algo, tile, kappa)
 
 
-@pytest.mark.parametrize("lookahead", ["0", "1"])
-def test_schedules_agree(monkeypatch, lookahead):
-    """The single-stream and the look-ahead (two-stream) schedules."""
+@pytest.mark.parametrize("lookahead,prio", [("0", "1"), ("1", "1"), ("1", "0"), ("2", "1"), ("2", "0")])
+def test_schedules_agree(monkeypatch, lookahead, prio):
+    """The single-stream, look-ahead and side-band (two-stream) schedules,
+    with and without the lead kernels' raised wave priority, graph-replayed
+    and launched directly."""
     monkeypatch.setenv("FWX_LOOKAHEAD", lookahead)
+    monkeypatch.setenv("FWX_LEAD_PRIO", prio)
     rng = np.random.default_rng(8)
-    for n, t, k in [(1000, 64, 2), (777, 64, 3), (1537, 128, 2), (900, 32, 4), (2048, 64, 4)]:
+    for n, t, k in [(1000, 64, 2), (777, 64, 3), (1537, 128, 2), (900, 32, 4), (2048, 64, 4), (1100, 64, 1)]:
         g = fx.generate_graph(n, float(rng.uniform(0.1, 0.9)), int(rng.integers(1, 2**31)), 1, 1000)
-        say("schedule", lookahead, n, t, k)
-        assert (gpu(g, "multi_layer_tiling_gpu", t, k) == po.fw_rowpar(g)).all(), (n, t, k)
+        expect = po.fw_rowpar(g)
+        for graph in ("1", "0"):
+            monkeypatch.setenv("FWX_GRAPH", graph)
+            say("schedule", lookahead, prio, graph, n, t, k)
+            assert (gpu(g, "multi_layer_tiling_gpu", t, k) == expect).all(), (n, t, k, graph)
 
 
 @pytest.mark.parametrize("variant", ["0", "1", "2"])
@@ -266,13 +272,15 @@ def test_config3_n32768_spot_rows():
     assert ok.all(), rows[~ok][:10]
 
 
-def test_graph_replay_cache(monkeypatch):
+@pytest.mark.parametrize("lookahead", ["0", "1", "2"])
+def test_graph_replay_cache(monkeypatch, lookahead):
     """Repeated solves on one buffer replay a captured hipGraph; every replay
     must equal the oracle, whichever relaxation form (pk64 / min3) the input
     selects, interleaved with direct (FWX_GRAPH=0) and profiled solves --
     profiled solves launch directly and their totals add up to n^3 each."""
     import ctypes
     from fw_cuda_amd import _lib
+    monkeypatch.setenv("FWX_LOOKAHEAD", lookahead)
     n = 1000
     g = fx.generate_graph(n, 0.5, 11, 1, 1000)
     expect = po.fw_rowpar(g)

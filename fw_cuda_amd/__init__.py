@@ -78,6 +78,34 @@ def has_negative_cycle_device(d_ptr, n, stream=None):
     return bool(r.value)
 
 
+def certify_rows_device(d_W, n, d_src, nsrc, d_rows, d_ok, stream=None):
+    """fwx_certify_rows on device buffers (raw pointers): d_ok[q] = 1 iff claimed
+    row q is the exact shortest-distance row from d_src[q]; returns the number
+    of failing rows."""
+    failed = ctypes.c_int32()
+    s = ctypes.c_void_p(stream) if stream else None
+    check(lib().fwx_certify_rows(d_W, int(n), d_src, int(nsrc), d_rows, d_ok, ctypes.byref(failed), s))
+    return int(failed.value)
+
+
+def certify_rows(W, src, rows):
+    """Host convenience wrapper of fwx_certify_rows: W (n x n), src (nsrc,),
+    rows (nsrc x n) claimed distance rows -> per-row bool array."""
+    import torch
+    W = np.ascontiguousarray(W, dtype=np.int32)
+    src = np.ascontiguousarray(src, dtype=np.int32)
+    rows = np.ascontiguousarray(rows, dtype=np.int32)
+    n = W.shape[0]
+    assert W.shape == (n, n) and rows.shape == (src.size, n)
+    dev = torch.device("cuda")
+    tw, ts, tr = (torch.from_numpy(a).to(dev) for a in (W, src, rows))
+    ok = torch.zeros(src.size, dtype=torch.int32, device=dev)
+    stream = torch.cuda.current_stream(dev).cuda_stream
+    certify_rows_device(ctypes.c_void_p(tw.data_ptr()), n, ctypes.c_void_p(ts.data_ptr()), src.size,
+                        ctypes.c_void_p(tr.data_ptr()), ctypes.c_void_p(ok.data_ptr()), stream)
+    return ok.cpu().numpy().astype(bool)
+
+
 class DeviceMatrix:
     """n x n int32 device buffer owned by the caller (reference GPUMemory,
     src/gpu_memory.cu:6-27): allocate, copy in, run in place, copy out, free."""

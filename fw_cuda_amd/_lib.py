@@ -65,6 +65,7 @@ SIGNATURES = {
     "fwx_copy_to_host_async": (ctypes.c_int, [_vp, _vp, _i32, _vp]),
     "fwx_stream_synchronize": (ctypes.c_int, [_vp]),
     "fwx_predecessors": (ctypes.c_int, [_vp, _vp, _vp, _i32, _vp]),
+    "fwx_certify_rows": (ctypes.c_int, [_vp, _i32, _vp, _i32, _vp, _vp, _i32p, _vp]),
     "fwx_batch_plan_create": (ctypes.c_int, [ctypes.POINTER(_vp), _vp, _vp, _i32, _vp, _vp]),
     "fwx_batch_plan_run": (ctypes.c_int, [_vp, _i32]),
     "fwx_batch_plan_stats": (ctypes.c_int, [_vp, _i32p, _i32p, ctypes.POINTER(_i64)]),

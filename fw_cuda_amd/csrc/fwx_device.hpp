@@ -46,6 +46,13 @@ __device__ __forceinline__ int32_t vmin3(int32_t a, int32_t b, int32_t c) {
   return r;
 }
 
+// v_min3_u32 as one instruction (same reason as vmin3).
+__device__ __forceinline__ uint32_t vmin3u(uint32_t a, uint32_t b, uint32_t c) {
+  uint32_t r;
+  asm("v_min3_u32 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+  return r;
+}
+
 // pair relaxation in pk64 form: a = (a1:a0), b = (b1:b0)
 __device__ __forceinline__ int32_t relax2_pk64(int32_t c, uint64_t a, uint64_t b) {
   const uint64_t s = a + b;

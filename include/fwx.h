@@ -153,6 +153,24 @@ int fwx_stream_synchronize(void* stream);
  */
 int fwx_predecessors(const int32_t* d_W, const int32_t* d_D, int32_t* d_P, int32_t n, void* stream);
 
+/* ---------------- certificate of solved rows (validation) --------------- */
+/*
+ * On-device check of claimed shortest-distance rows, without solving
+ * anything.  Replaces the host-side checks after a GPU solve: the full
+ * comparison with fwBaselineCPU (src/tests.cpp:37-57, out of reach at
+ * n=32768) and the host Dijkstra spot check of large solves.  d_W: the input
+ * weights (zero diagonal, off-diagonal in [1, FWX_INF]; the INF-completed
+ * graph, as src/fw_baseline_cpu.cpp:9 treats it); d_src: nsrc source
+ * vertices; d_rows: nsrc x n claimed rows, row q = the claimed distances
+ * from d_src[q].  Writes d_ok[q] = 1 iff row q is exactly the shortest
+ * distance row (d[s] == 0, every d[v] in [0, INF], and d[v] == min over
+ * u != v of d[u] + W[u][v] for v != s), else 0, and *failed = the number of
+ * failing rows.  nsrc * n^2 relaxations on `stream`; synchronises it.
+ * FWX_ERR_INVALID_ARG if W is outside the domain (then d_ok is undefined).
+ */
+int fwx_certify_rows(const int32_t* d_W, int32_t n, const int32_t* d_src, int32_t nsrc, const int32_t* d_rows,
+                     int32_t* d_ok, int32_t* failed, void* stream);
+
 /* ---------------- batched FW (no reference counterpart) ----------------- */
 /*
  * Many independent graphs solved together.  Graph g is an n_g x n_g row-major

@@ -247,24 +247,34 @@ def test_config2_n8192_all_kappas():
 @pytest.mark.slow
 def test_config3_n32768_spot_rows():
     """n=32768 (4 GiB int32), density 0.1, weights U[1,1000], seed 7:
-    multi_layer_tiling_gpu (tuned) certified exact on 4096 random rows by the
-    shortest-path optimality certificate, 64 of them also equal to host
-    Dijkstra over the INF-completed graph; tiling_gpu bit-identical to it on
-    every cell."""
+    multi_layer_tiling_gpu (tuned) certified exact on every row by the device
+    certificate (fwx_certify_rows), on 4096 random rows by the oracle's
+    certificate, 64 of them also equal to host Dijkstra over the INF-completed
+    graph; tiling_gpu bit-identical to it on every cell."""
     n = 32768
     g = fx.generate_graph(n, 0.1, 7, 1, 1000)
     rows = np.sort(np.random.default_rng(7).choice(n, 4096, replace=False)).astype(np.int32)
     say("config3: dijkstra on 64 rows")
     dij = po.dijkstra_rows(g, rows[::64])
-    with fx.DeviceMatrix(n) as dm:
+    import ctypes
+    import torch
+    with fx.DeviceMatrix(n) as dm, fx.DeviceMatrix(n) as dw:
         dm.upload(g)
+        dw.upload(g)
         say("config3 multi_layer_tiling_gpu")
         fx.run_device("multi_layer_tiling_gpu", dm.ptr, n, 0, 0)
         ml = dm.download()
+        say("config3: device certificate of all", n, "rows")
+        src = torch.arange(n, dtype=torch.int32, device="cuda")
+        ok = torch.zeros(n, dtype=torch.int32, device="cuda")
+        failed = fx.certify_rows_device(dw.ptr, n, ctypes.c_void_p(src.data_ptr()), n, dm.ptr,
+                                        ctypes.c_void_p(ok.data_ptr()))
+        all_rows_ok = failed == 0 and bool((ok == 1).all())
         dm.upload(g)
         say("config3 tiling_gpu")
         fx.run_device("tiling_gpu", dm.ptr, n, 0, 0)
         same = bool((dm.download() == ml).all())
+    assert all_rows_ok, failed
     assert same, "tiling_gpu and multi_layer_tiling_gpu differ"
     assert (ml[rows[::64]] == dij).all()
     say("config3: certificate on", rows.size, "rows")

@@ -13,6 +13,8 @@ from oracle import pyoracle as po
 
 pytestmark = pytest.mark.gpu
 
+INF = fx.INF
+
 
 def say(*a):
     print(*a, flush=True)
@@ -84,6 +86,27 @@ def test_predecessors_match_oracle(n):
         D, P = fx.floyd_warshall_with_pred(W)
         assert (D == po.fw_rowpar(W)).all()
         assert (P == po.predecessors(W, D)).all()
+
+
+def test_predecessor_keys_saturate():
+    """The key (sat16(t) << 16) | u must clamp t, not truncate it: a
+    candidate u=1 that misses by exactly 2^16 (and 2^17 for another cell)
+    would otherwise look tight and beat the true predecessor u=2.  Also
+    large weights (U[1, 10^6]), where t often exceeds 2^16, against the
+    oracle."""
+    W = np.full((5, 5), INF, dtype=np.int32)
+    np.fill_diagonal(W, 0)
+    W[0, 1], W[0, 2], W[2, 3], W[1, 3] = 5, 3, 7, 65541     # t(u=1 -> 3) = 65536
+    W[2, 4], W[1, 4] = 9, 5 + 131072 - 5 + 7                 # t(u=1 -> 4) = 131072 + ...
+    D, P = fx.floyd_warshall_with_pred(W)
+    assert (D == po.fw_rowpar(W)).all()
+    assert (P == po.predecessors(W, D)).all()
+    assert P[0, 3] == 2 and P[0, 4] == 2
+    rng = np.random.default_rng(17)
+    for n in (200, 513):
+        W = fx.generate_graph(n, 0.3, int(rng.integers(1, 2**31)), 1, 1_000_000)
+        D, P = fx.floyd_warshall_with_pred(W)
+        assert (P == po.predecessors(W, D)).all(), n
 
 
 def test_predecessors_walks_at_4096():

@@ -163,6 +163,10 @@ def test_batched_and_diff_argument_errors():
     assert L.fwx_predecessors(None, dummy, dummy, 4, None) == _lib.ERR_INVALID_ARG
     assert L.fwx_predecessors(dummy, dummy, dummy, 0, None) == _lib.ERR_INVALID_ARG
     assert L.fwx_copy_to_device_async(None, None, 4, None) == _lib.ERR_INVALID_ARG
+    assert L.fwx_certify_rows(None, 4, dummy, 1, dummy, dummy, ctypes.byref(r), None) == _lib.ERR_INVALID_ARG
+    assert L.fwx_certify_rows(dummy, 4, dummy, 0, dummy, dummy, ctypes.byref(r), None) == _lib.ERR_INVALID_ARG
+    assert L.fwx_certify_rows(dummy, 32770, dummy, 1, dummy, dummy, ctypes.byref(r), None) == _lib.ERR_INVALID_ARG
+    assert L.fwx_certify_rows(dummy, 4, dummy, 1, dummy, dummy, None, None) == _lib.ERR_INVALID_ARG
     assert L.fwx_batch_plan_run(None, 3) == _lib.ERR_INVALID_ARG
     assert L.fwx_batch_plan_set_wait_limit(None, 1.0) == _lib.ERR_INVALID_ARG
     assert L.fwx_batch_plan_queues(None, None, None, None) == _lib.ERR_INVALID_ARG

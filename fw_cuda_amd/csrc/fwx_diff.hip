@@ -235,19 +235,11 @@ __global__ __launch_bounds__(256) void k_fpanel(float* __restrict__ D, int32_t* 
   fstore<PRED>(C, acc, pacc);
 }
 
-// Rest tiles of stage c.  mode 0: every (i, j) with i, j != c, (N-1)^2
-// blocks.  Look-ahead split (c + 1 < N): mode 1, the band that stage c+1's
-// closure and panels read -- row c+1 (j != c, N-1 blocks) and column c+1
-// (i != c, c+1, N-2 blocks); mode 2, the others (i, j not in {c, c+1}),
-// (N-2)^2 blocks.
-template <bool PRED>
-__global__ __launch_bounds__(256) void k_frest(float* __restrict__ D, int32_t* __restrict__ P, int32_t n,
-                                               int32_t c, int32_t N, int32_t mode) {
-  __shared__ FSmem s;
-  const int64_t g = blockIdx.y;
-  float* Dg = D + g * (int64_t)n * n;
-  int32_t* Pg = PRED ? P + g * (int64_t)n * n : nullptr;
-  const int b = blockIdx.x;
+// Rest tile (i, j) of block b in stage c.  mode 0: every (i, j) with
+// i, j != c, (N-1)^2 blocks.  mode 1: the band stage c+1's closure and
+// panels read -- row c+1 (j != c, N-1 blocks) and column c+1 (i != c, c+1,
+// N-2 blocks).  mode 2: the others (i, j not in {c, c+1}), (N-2)^2 blocks.
+__device__ __forceinline__ void rest_tile(int b, int c, int N, int mode, int* I, int* J) {
   int i, j;
   if (mode == 0) {
     i = b / (N - 1);
@@ -269,6 +261,19 @@ __global__ __launch_bounds__(256) void k_frest(float* __restrict__ D, int32_t* _
     i = i < c ? i : i + 2;
     j = j < c ? j : j + 2;
   }
+  *I = i;
+  *J = j;
+}
+
+template <bool PRED>
+__global__ __launch_bounds__(256) void k_frest(float* __restrict__ D, int32_t* __restrict__ P, int32_t n,
+                                               int32_t c, int32_t N, int32_t mode) {
+  __shared__ FSmem s;
+  const int64_t g = blockIdx.y;
+  float* Dg = D + g * (int64_t)n * n;
+  int32_t* Pg = PRED ? P + g * (int64_t)n * n : nullptr;
+  int i, j;
+  rest_tile(blockIdx.x, c, N, mode, &i, &j);
   const FView A = fview(Dg, Pg, n, i * FT, c * FT);
   const FView B = fview(Dg, Pg, n, c * FT, j * FT);
   const FView C = fview(Dg, Pg, n, i * FT, j * FT);
@@ -277,6 +282,184 @@ __global__ __launch_bounds__(256) void k_frest(float* __restrict__ D, int32_t* _
   fload<PRED>(C, acc, pacc);
   fprod<PRED, false>(s, A, B, FT, 0, 0, false, acc, pacc);
   fstore<PRED>(C, acc, pacc);
+}
+
+// ---------------------------------------------------------------------------
+// Fast rest-tile product (k_frest4).  Same arithmetic and candidate order as
+// fprod -- per cell, k ascending, a + b in fp32, strict < -- so D and P are
+// bit-identical; laid out for 16-byte LDS reads.  Thread (tx, ty) owns rows
+// ty + 16 r and the 4 adjacent columns 4 tx + c; per 4 k it reads 4 A rows
+// (one float4 each: 4 k values), and 4 B and 4 predecessor rows (one
+// float4 / int4 each: its 4 columns).  The 32-deep chunks are
+// double-buffered: the next chunk's global loads are in flight while the
+// current one is computed.
+struct alignas(16) FSmem2 {
+  alignas(16) float a[2][FT * FLDA];
+  alignas(16) float b[2][FKC * FT];
+  alignas(16) int32_t bp[2][FKC * FT];
+};
+
+template <bool PRED, bool VEC>
+struct FProd4 {
+  // staging: A 64 rows x 8 quads, B / BP 32 rows x 16 quads -> 2 units each
+  float4 ra[2], rb[2];
+  int4 rp[2];
+
+  __device__ __forceinline__ void fetch(const FView& A, const FView& B, int32_t kc) {
+    const int tid = threadIdx.x;
+    const float4 inf4 = make_float4(kFInf, kFInf, kFInf, kFInf);
+#pragma unroll
+    for (int p = 0; p < 2; ++p) {
+      const int u = p * 256 + tid;
+      const int r = u >> 3, k = kc + (u & 7) * 4;
+      if (VEC) {
+        ra[p] = (r < A.rlim && k < A.clim) ? *reinterpret_cast<const float4*>(A.p + r * A.ld + k) : inf4;
+      } else {
+        float v[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) v[e] = (r < A.rlim && k + e < A.clim) ? A.p[r * A.ld + k + e] : kFInf;
+        ra[p] = make_float4(v[0], v[1], v[2], v[3]);
+      }
+      const int kr = kc + (u >> 4), col = (u & 15) * 4;
+      if (VEC) {
+        const bool in = kr < B.rlim && col < B.clim;
+        rb[p] = in ? *reinterpret_cast<const float4*>(B.p + kr * B.ld + col) : inf4;
+        if (PRED) rp[p] = in ? *reinterpret_cast<const int4*>(B.pp + kr * B.ld + col) : make_int4(-1, -1, -1, -1);
+      } else {
+        float v[4];
+        int32_t q[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const bool in = kr < B.rlim && col + e < B.clim;
+          v[e] = in ? B.p[kr * B.ld + col + e] : kFInf;
+          if (PRED) q[e] = in ? B.pp[kr * B.ld + col + e] : -1;
+        }
+        rb[p] = make_float4(v[0], v[1], v[2], v[3]);
+        if (PRED) rp[p] = make_int4(q[0], q[1], q[2], q[3]);
+      }
+    }
+  }
+
+  __device__ __forceinline__ void stash(FSmem2& s, int buf) const {
+    const int tid = threadIdx.x;
+#pragma unroll
+    for (int p = 0; p < 2; ++p) {
+      const int u = p * 256 + tid;
+      *reinterpret_cast<float4*>(&s.a[buf][(u >> 3) * FLDA + (u & 7) * 4]) = ra[p];
+      *reinterpret_cast<float4*>(&s.b[buf][(u >> 4) * FT + (u & 15) * 4]) = rb[p];
+      if (PRED) *reinterpret_cast<int4*>(&s.bp[buf][(u >> 4) * FT + (u & 15) * 4]) = rp[p];
+    }
+  }
+
+  __device__ __forceinline__ static void compute(const FSmem2& s, int buf, float (&acc)[4][4],
+                                                 int32_t (&pacc)[4][4]) {
+    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+#pragma unroll 2
+    for (int q = 0; q < FKC / 4; ++q) {
+      float4 a4[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) a4[r] = *reinterpret_cast<const float4*>(&s.a[buf][(ty + 16 * r) * FLDA + 4 * q]);
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const float4 b4 = *reinterpret_cast<const float4*>(&s.b[buf][(4 * q + e) * FT + 4 * tx]);
+        int4 p4;
+        if (PRED) p4 = *reinterpret_cast<const int4*>(&s.bp[buf][(4 * q + e) * FT + 4 * tx]);
+        const float bv[4] = {b4.x, b4.y, b4.z, b4.w};
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const float ae = (&a4[r].x)[e];
+#pragma unroll
+          for (int c = 0; c < 4; ++c) {
+            const float cand = ae + bv[c];
+            if (cand < acc[r][c]) {
+              acc[r][c] = cand;
+              if (PRED) pacc[r][c] = (&p4.x)[c];
+            }
+          }
+        }
+      }
+    }
+  }
+};
+
+template <bool PRED, bool VEC>
+__device__ __forceinline__ void fload4(const FView& C, float (&acc)[4][4], int32_t (&pacc)[4][4]) {
+  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int rr = ty + 16 * r, cc = 4 * tx;
+    if (VEC) {
+      const bool in = rr < C.rlim && cc < C.clim;
+      const float4 v = in ? *reinterpret_cast<const float4*>(C.p + rr * C.ld + cc)
+                          : make_float4(kFInf, kFInf, kFInf, kFInf);
+      acc[r][0] = v.x, acc[r][1] = v.y, acc[r][2] = v.z, acc[r][3] = v.w;
+      if (PRED) {
+        const int4 q = in ? *reinterpret_cast<const int4*>(C.pp + rr * C.ld + cc) : make_int4(-1, -1, -1, -1);
+        pacc[r][0] = q.x, pacc[r][1] = q.y, pacc[r][2] = q.z, pacc[r][3] = q.w;
+      }
+    } else {
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const bool in = C.in(rr, cc + c);
+        acc[r][c] = in ? C.p[rr * C.ld + cc + c] : kFInf;
+        if (PRED) pacc[r][c] = in ? C.pp[rr * C.ld + cc + c] : -1;
+      }
+    }
+  }
+}
+
+template <bool PRED, bool VEC>
+__device__ __forceinline__ void fstore4(const FView& C, const float (&acc)[4][4], const int32_t (&pacc)[4][4]) {
+  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int rr = ty + 16 * r, cc = 4 * tx;
+    if (VEC) {
+      if (rr < C.rlim && cc < C.clim) {
+        *reinterpret_cast<float4*>(C.p + rr * C.ld + cc) = make_float4(acc[r][0], acc[r][1], acc[r][2], acc[r][3]);
+        if (PRED)
+          *reinterpret_cast<int4*>(C.pp + rr * C.ld + cc) = make_int4(pacc[r][0], pacc[r][1], pacc[r][2], pacc[r][3]);
+      }
+    } else {
+#pragma unroll
+      for (int c = 0; c < 4; ++c)
+        if (C.in(rr, cc + c)) {
+          C.p[rr * C.ld + cc + c] = acc[r][c];
+          if (PRED) C.pp[rr * C.ld + cc + c] = pacc[r][c];
+        }
+    }
+  }
+}
+
+// Rest tiles of stage c with FProd4 (same tile decode as k_frest).
+template <bool PRED, bool VEC>
+__global__ __launch_bounds__(256) void k_frest4(float* __restrict__ D, int32_t* __restrict__ P, int32_t n,
+                                                int32_t c, int32_t N, int32_t mode) {
+  __shared__ FSmem2 s;
+  const int64_t g = blockIdx.y;
+  float* Dg = D + g * (int64_t)n * n;
+  int32_t* Pg = PRED ? P + g * (int64_t)n * n : nullptr;
+  int i, j;
+  rest_tile(blockIdx.x, c, N, mode, &i, &j);
+  const FView A = fview(Dg, Pg, n, i * FT, c * FT);
+  const FView B = fview(Dg, Pg, n, c * FT, j * FT);
+  const FView C = fview(Dg, Pg, n, i * FT, j * FT);
+  float acc[4][4];
+  int32_t pacc[4][4];
+  fload4<PRED, VEC>(C, acc, pacc);
+  FProd4<PRED, VEC> pr;
+  pr.fetch(A, B, 0);
+  pr.stash(s, 0);
+  __syncthreads();
+  constexpr int nch = FT / FKC;
+#pragma unroll
+  for (int ch = 0; ch < nch; ++ch) {
+    if (ch + 1 < nch) pr.fetch(A, B, (ch + 1) * FKC);
+    FProd4<PRED, VEC>::compute(s, ch & 1, acc, pacc);
+    if (ch + 1 < nch) pr.stash(s, (ch + 1) & 1);
+    __syncthreads();
+  }
+  fstore4<PRED, VEC>(C, acc, pacc);
 }
 
 // Predecessors of sources [s0, s0+rows) recomputed from D and W:
@@ -451,10 +634,25 @@ __global__ __launch_bounds__(kBackThreads) void k_fback(const float* __restrict_
   }
 }
 
-// FWX_LOOKAHEAD=0 selects the single-stream schedule (read on every call).
-static bool diff_lookahead() {
-  const char* e = std::getenv("FWX_LOOKAHEAD");
+// Rest-pass kernel: FWX_DIFF_REST=0 selects the first version (k_frest,
+// scalar LDS reads, single-buffered), bit-identical to the default k_frest4
+// (read on every call).
+static bool diff_rest_fast() {
+  const char* e = std::getenv("FWX_DIFF_REST");
   return !(e && e[0] == '0');
+}
+
+template <bool PRED>
+void frest_launch(float* D, int32_t* P, int32_t n, int32_t c, int32_t N, int32_t mode, int32_t blocks,
+                  int32_t batch, hipStream_t st) {
+  const bool vec = (n % 4 == 0) && ((reinterpret_cast<uintptr_t>(D) & 15u) == 0) &&
+                   (!PRED || (reinterpret_cast<uintptr_t>(P) & 15u) == 0);
+  if (!diff_rest_fast())
+    k_frest<PRED><<<dim3(blocks, batch), 256, 0, st>>>(D, P, n, c, N, mode);
+  else if (vec)
+    k_frest4<PRED, true><<<dim3(blocks, batch), 256, 0, st>>>(D, P, n, c, N, mode);
+  else
+    k_frest4<PRED, false><<<dim3(blocks, batch), 256, 0, st>>>(D, P, n, c, N, mode);
 }
 
 template <bool PRED>
@@ -468,12 +666,52 @@ int lead_round(float* D, int32_t* P, int32_t n, int32_t c, int32_t N, int32_t ba
   return FWX_OK;
 }
 
-// Three-phase blocked FW over 64-tiles.  With the look-ahead (two streams),
-// stage c's rest pass first updates the band stage c+1's closure and panels
-// read, then the side stream runs stage c+1's closure and panels while the
-// main stream updates the other tiles of stage c; the main stream waits for
-// them before stage c+1's band.  Every tile sees the stages in order, so the
-// result is the same bits as the single-stream schedule.
+// Three-phase blocked FW over 64-tiles.  Schedules (FWX_LOOKAHEAD, read on
+// every call; all give the same bits, since every tile sees the stages in
+// order):
+//  0  one stream: lead(c), rest(c), lead(c+1), ...
+//  1  look-ahead: stage c's rest pass first updates the band that stage
+//     c+1's closure and panels read (row and column c+1), then the side
+//     stream runs lead(c+1) while the main stream updates the other tiles;
+//     the main stream waits for it before stage c+1's band.
+//  2  side-band (default): the side stream runs band(c) and then lead(c+1);
+//     the main stream runs only the other rest tiles of each stage.  band(c)
+//     and lead(c+1) write only row/column c+1 tiles; rest_other(c) reads
+//     row/column c tiles and writes tiles outside rows/columns c and c+1.
+//     rest_other(c) waits for lead(c) (event L); band(c+1) waits for
+//     rest_other(c) (event R), which wrote the row/column c+2 tiles it
+//     updates next.
+static int diff_schedule() {
+  const char* e = std::getenv("FWX_LOOKAHEAD");
+  return (e && e[0] >= '0' && e[0] <= '2') ? e[0] - '0' : 2;
+}
+
+template <bool PRED>
+int forward_sideband(float* D, int32_t* P, int32_t n, int32_t N, int32_t batch, hipStream_t st, hipStream_t side,
+                     hipEvent_t evR, hipEvent_t evL) {
+  FWX_HIP(hipEventRecord(evR, st));  // the side stream starts after the caller's prior work
+  FWX_HIP(hipStreamWaitEvent(side, evR, 0));
+  if (int rc = lead_round<PRED>(D, P, n, 0, N, batch, side)) return rc;
+  FWX_HIP(hipEventRecord(evL, side));
+  for (int c = 0; c < N; ++c) {
+    FWX_HIP(hipStreamWaitEvent(st, evL, 0));
+    if (c + 1 < N) {
+      frest_launch<PRED>(D, P, n, c, N, 1, 2 * N - 3, batch, side);
+      FWX_LAUNCHED("k_frest");
+      if (int rc = lead_round<PRED>(D, P, n, c + 1, N, batch, side)) return rc;
+      FWX_HIP(hipEventRecord(evL, side));
+      frest_launch<PRED>(D, P, n, c, N, 2, (N - 2) * (N - 2), batch, st);
+      FWX_LAUNCHED("k_frest");
+      FWX_HIP(hipEventRecord(evR, st));
+      FWX_HIP(hipStreamWaitEvent(side, evR, 0));
+    } else {
+      frest_launch<PRED>(D, P, n, c, N, 0, (N - 1) * (N - 1), batch, st);
+      FWX_LAUNCHED("k_frest");
+    }
+  }
+  return FWX_OK;
+}
+
 template <bool PRED>
 int forward_impl(const float* W, float* D, int32_t* P, int32_t n, int32_t batch, hipStream_t st) {
   const int64_t total = (int64_t)n * n * batch;
@@ -481,11 +719,12 @@ int forward_impl(const float* W, float* D, int32_t* P, int32_t n, int32_t batch,
   k_finit<PRED><<<fillblocks, 256, 0, st>>>(W, D, P, n, total);
   FWX_LAUNCHED("k_finit");
   const int N = (n + FT - 1) / FT;
-  if (N < 3 || !diff_lookahead()) {
+  const int mode = diff_schedule();
+  if (N < 3 || mode == 0) {
     for (int c = 0; c < N; ++c) {
       if (int rc = lead_round<PRED>(D, P, n, c, N, batch, st)) return rc;
       if (N > 1) {
-        k_frest<PRED><<<dim3((N - 1) * (N - 1), batch), 256, 0, st>>>(D, P, n, c, N, 0);
+        frest_launch<PRED>(D, P, n, c, N, 0, (N - 1) * (N - 1), batch, st);
         FWX_LAUNCHED("k_frest");
       }
     }
@@ -500,10 +739,17 @@ int forward_impl(const float* W, float* D, int32_t* P, int32_t n, int32_t batch,
     (void)hipEventDestroy(evA);
     return hip_fail(e, "hipEventCreateWithFlags", __FILE__, __LINE__);
   }
-  int rc = lead_round<PRED>(D, P, n, 0, N, batch, st);
+  int rc = FWX_OK;
+  if (mode == 2) {
+    rc = forward_sideband<PRED>(D, P, n, N, batch, st, side, evA, evB);
+    (void)hipEventDestroy(evA);
+    (void)hipEventDestroy(evB);
+    return rc;
+  }
+  rc = lead_round<PRED>(D, P, n, 0, N, batch, st);
   for (int c = 0; c < N && rc == FWX_OK; ++c) {
     if (c + 1 < N) {
-      k_frest<PRED><<<dim3(2 * N - 3, batch), 256, 0, st>>>(D, P, n, c, N, 1);
+      frest_launch<PRED>(D, P, n, c, N, 1, 2 * N - 3, batch, st);
       e = hipGetLastError();
       if (e == hipSuccess) e = hipEventRecord(evA, st);
       if (e == hipSuccess) e = hipStreamWaitEvent(side, evA, 0);
@@ -514,13 +760,13 @@ int forward_impl(const float* W, float* D, int32_t* P, int32_t n, int32_t batch,
       if ((rc = lead_round<PRED>(D, P, n, c + 1, N, batch, side))) break;
       e = hipEventRecord(evB, side);
       if (e == hipSuccess) {
-        k_frest<PRED><<<dim3((N - 2) * (N - 2), batch), 256, 0, st>>>(D, P, n, c, N, 2);
+        frest_launch<PRED>(D, P, n, c, N, 2, (N - 2) * (N - 2), batch, st);
         e = hipGetLastError();
       }
       if (e == hipSuccess) e = hipStreamWaitEvent(st, evB, 0);
       if (e != hipSuccess) rc = hip_fail(e, "diff look-ahead rest", __FILE__, __LINE__);
     } else {
-      k_frest<PRED><<<dim3((N - 1) * (N - 1), batch), 256, 0, st>>>(D, P, n, c, N, 0);
+      frest_launch<PRED>(D, P, n, c, N, 0, (N - 1) * (N - 1), batch, st);
       e = hipGetLastError();
       if (e != hipSuccess) rc = hip_fail(e, "k_frest", __FILE__, __LINE__);
     }

@@ -8,6 +8,7 @@ path switch (a kink); those are detected by their one-sided slopes
 disagreeing and are reported, not compared.  A second pass at eps = 1e-6
 compares every sampled edge.  Tolerance: relative L2 error < 1e-4.
 """
+import ctypes
 import sys
 from concurrent.futures import ThreadPoolExecutor
 
@@ -70,22 +71,48 @@ def test_forward_and_backward_small():
 
 @pytest.mark.parametrize("batch,n", [(3, 129), (2, 300), (1, 640), (4, 200)])
 def test_forward_schedules_agree(monkeypatch, batch, n):
-    """The look-ahead (two-stream) forward and the single-stream forward give
+    """The single-stream, look-ahead and side-band (two-stream) forwards give
     the same bits for D and P."""
     import torch
     from fw_cuda_amd.autograd import forward_with_pred
     W, _ = graphs(batch, n, 0.3, n + batch)
     Wt = torch.from_numpy(W).cuda()
     out = {}
-    for la in ("0", "1"):
+    for la in ("0", "1", "2"):
         monkeypatch.setenv("FWX_LOOKAHEAD", la)
         D, P = forward_with_pred(Wt)
         out[la] = (D.cpu().numpy(), P.cpu().numpy())
-    assert np.array_equal(out["0"][0], out["1"][0], equal_nan=True)
-    assert (out["0"][1] == out["1"][1]).all()
+    for la in ("1", "2"):
+        assert np.array_equal(out["0"][0], out[la][0], equal_nan=True), la
+        assert (out["0"][1] == out[la][1]).all(), la
     Do, _ = po.fw_f32_pred(W[0])
     fin = np.isfinite(Do)
     assert np.allclose(out["1"][0][0][fin], Do[fin], rtol=2e-6, atol=0)
+
+
+@pytest.mark.parametrize("batch,n", [(3, 129), (2, 300), (1, 640), (4, 200), (2, 258), (1, 1023)])
+def test_forward_rest_kernels_agree(monkeypatch, batch, n):
+    """The vectorised rest product (default) and the first one
+    (FWX_DIFF_REST=0) give the same bits for D and P, with P stored and
+    without, on 16-byte (n % 4 == 0) and scalar (ragged n) paths."""
+    import torch
+    from fw_cuda_amd.autograd import forward_with_pred
+    from fw_cuda_amd import _lib
+    W, _ = graphs(batch, n, 0.3, 7 * n + batch)
+    Wt = torch.from_numpy(W).cuda()
+    out = {}
+    for v in ("0", "1"):
+        monkeypatch.setenv("FWX_DIFF_REST", v)
+        D, P = forward_with_pred(Wt)
+        Dn = torch.empty_like(Wt)
+        s = torch.cuda.current_stream().cuda_stream
+        _lib.check(_lib.lib().fwx_diff_forward(_lib.ptr(Wt), _lib.ptr(Dn), None, n, batch, ctypes.c_void_p(s)))
+        torch.cuda.synchronize()
+        out[v] = (D.cpu().numpy(), P.cpu().numpy(), Dn.cpu().numpy())
+    assert np.array_equal(out["0"][0], out["1"][0], equal_nan=True)
+    assert (out["0"][1] == out["1"][1]).all()
+    assert np.array_equal(out["0"][2], out["1"][2], equal_nan=True)
+    assert np.array_equal(out["1"][0], out["1"][2], equal_nan=True)
 
 
 def test_backward_deep_trees():

@@ -206,9 +206,96 @@ __global__ __launch_bounds__(256) void k_fdiag(float* __restrict__ D, int32_t* _
   fstore<PRED>(V, acc, pacc);
 }
 
+// Diagonal closure, 1024 threads (one row and 4 columns each) and two k
+// steps per barrier.  The owners of rows k, k+1 and columns k, k+1 publish
+// them (values, and the rows' predecessors); every thread then forms column
+// k+1 after step k for its row and row k+1 after step k for its columns --
+// the same candidates, sums and strict comparisons as the sequential loop --
+// and applies step k, then step k+1.  Same bits as k_fdiag
+// (FWX_DIFF_DIAG=0 selects it).  Line buffer p&1 is rewritten at pair p+2,
+// after every thread passed barrier p+1.
+template <bool PRED>
+__global__ __launch_bounds__(1024) void k_fdiag2(float* __restrict__ D, int32_t* __restrict__ P, int32_t n,
+                                                 int32_t c, int32_t prio) {
+  if (prio) __builtin_amdgcn_s_setprio(3);
+  __shared__ float rowv[2][2][FT], colv[2][2][FT];
+  __shared__ int32_t rowp[2][2][FT];
+  const int64_t g = blockIdx.y;
+  float* Dg = D + g * (int64_t)n * n;
+  int32_t* Pg = PRED ? P + g * (int64_t)n * n : nullptr;
+  const FView V = fview(Dg, Pg, n, c * FT, c * FT);
+  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+  float acc[4];
+  int32_t pacc[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const bool in = V.in(ty, tx + 16 * q);
+    acc[q] = in ? V.p[ty * V.ld + tx + 16 * q] : kFInf;
+    if (PRED) pacc[q] = in ? V.pp[ty * V.ld + tx + 16 * q] : -1;
+  }
+  const int kmax = min(FT, n - c * FT);
+#pragma unroll
+  for (int qk = 0; qk < 4; ++qk) {
+    for (int k16 = 0; k16 < 16; k16 += 2) {
+      const int k = qk * 16 + k16;
+      if (k >= kmax) break;
+      const int b = (k >> 1) & 1;
+      if (ty == k || ty == k + 1) {
+        const int h = ty - k;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          rowv[b][h][tx + 16 * q] = acc[q];
+          if (PRED) rowp[b][h][tx + 16 * q] = pacc[q];
+        }
+      }
+      if (tx == k16 || tx == k16 + 1) colv[b][tx - k16][ty] = acc[qk];
+      __syncthreads();
+      const float ci0 = colv[b][0][ty];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const float cand = ci0 + rowv[b][0][tx + 16 * q];
+        if (cand < acc[q]) {
+          acc[q] = cand;
+          if (PRED) pacc[q] = rowp[b][0][tx + 16 * q];
+        }
+      }
+      if (k + 1 < kmax) {
+        const float dkk1 = rowv[b][0][k + 1];  // D[k][k+1]
+        const float dk1k = rowv[b][1][k];      // D[k+1][k]
+        float ci1 = colv[b][1][ty];             // D[ty][k+1] after step k
+        const float t = ci0 + dkk1;
+        if (t < ci1) ci1 = t;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int j = tx + 16 * q;
+          float r1 = rowv[b][1][j];  // D[k+1][j] after step k
+          int32_t p1 = PRED ? rowp[b][1][j] : 0;
+          const float u = dk1k + rowv[b][0][j];
+          if (u < r1) {
+            r1 = u;
+            if (PRED) p1 = rowp[b][0][j];
+          }
+          const float cand = ci1 + r1;
+          if (cand < acc[q]) {
+            acc[q] = cand;
+            if (PRED) pacc[q] = p1;
+          }
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+    if (V.in(ty, tx + 16 * q)) {
+      V.p[ty * V.ld + tx + 16 * q] = acc[q];
+      if (PRED) V.pp[ty * V.ld + tx + 16 * q] = pacc[q];
+    }
+}
+
 template <bool PRED>
 __global__ __launch_bounds__(256) void k_fpanel(float* __restrict__ D, int32_t* __restrict__ P, int32_t n,
-                                                int32_t c, int32_t N) {
+                                                int32_t c, int32_t N, int32_t prio) {
+  if (prio) __builtin_amdgcn_s_setprio(3);
   __shared__ FSmem s;
   const int64_t g = blockIdx.y;
   float* Dg = D + g * (int64_t)n * n;
@@ -434,7 +521,8 @@ __device__ __forceinline__ void fstore4(const FView& C, const float (&acc)[4][4]
 // Rest tiles of stage c with FProd4 (same tile decode as k_frest).
 template <bool PRED, bool VEC>
 __global__ __launch_bounds__(256) void k_frest4(float* __restrict__ D, int32_t* __restrict__ P, int32_t n,
-                                                int32_t c, int32_t N, int32_t mode) {
+                                                int32_t c, int32_t N, int32_t mode, int32_t prio) {
+  if (prio) __builtin_amdgcn_s_setprio(3);
   __shared__ FSmem2 s;
   const int64_t g = blockIdx.y;
   float* Dg = D + g * (int64_t)n * n;
@@ -642,25 +730,45 @@ static bool diff_rest_fast() {
   return !(e && e[0] == '0');
 }
 
+// FWX_DIFF_DIAG=0 selects the first diagonal closure (k_fdiag: 256
+// threads, one k step per barrier), bit-identical to k_fdiag2.
+static bool diff_diag_fast() {
+  const char* e = std::getenv("FWX_DIFF_DIAG");
+  return !(e && e[0] == '0');
+}
+
+// FWX_LEAD_PRIO=0 keeps the side stream's kernels (band, closure, panels)
+// at the default wave priority.
+static int diff_side_prio() {
+  const char* e = std::getenv("FWX_LEAD_PRIO");
+  return (e && e[0] == '0') ? 0 : 1;
+}
+
 template <bool PRED>
 void frest_launch(float* D, int32_t* P, int32_t n, int32_t c, int32_t N, int32_t mode, int32_t blocks,
-                  int32_t batch, hipStream_t st) {
+                  int32_t batch, int32_t prio, hipStream_t st) {
   const bool vec = (n % 4 == 0) && ((reinterpret_cast<uintptr_t>(D) & 15u) == 0) &&
                    (!PRED || (reinterpret_cast<uintptr_t>(P) & 15u) == 0);
   if (!diff_rest_fast())
     k_frest<PRED><<<dim3(blocks, batch), 256, 0, st>>>(D, P, n, c, N, mode);
   else if (vec)
-    k_frest4<PRED, true><<<dim3(blocks, batch), 256, 0, st>>>(D, P, n, c, N, mode);
+    k_frest4<PRED, true><<<dim3(blocks, batch), 256, 0, st>>>(D, P, n, c, N, mode, prio);
   else
-    k_frest4<PRED, false><<<dim3(blocks, batch), 256, 0, st>>>(D, P, n, c, N, mode);
+    k_frest4<PRED, false><<<dim3(blocks, batch), 256, 0, st>>>(D, P, n, c, N, mode, prio);
 }
 
 template <bool PRED>
-int lead_round(float* D, int32_t* P, int32_t n, int32_t c, int32_t N, int32_t batch, hipStream_t st) {
-  k_fdiag<PRED><<<dim3(1, batch), 256, 0, st>>>(D, P, n, c);
-  FWX_LAUNCHED("k_fdiag");
+int lead_round(float* D, int32_t* P, int32_t n, int32_t c, int32_t N, int32_t batch, int32_t prio,
+               hipStream_t st) {
+  if (diff_diag_fast()) {
+    k_fdiag2<PRED><<<dim3(1, batch), 1024, 0, st>>>(D, P, n, c, prio);
+    FWX_LAUNCHED("k_fdiag2");
+  } else {
+    k_fdiag<PRED><<<dim3(1, batch), 256, 0, st>>>(D, P, n, c);
+    FWX_LAUNCHED("k_fdiag");
+  }
   if (N > 1) {
-    k_fpanel<PRED><<<dim3(2 * (N - 1), batch), 256, 0, st>>>(D, P, n, c, N);
+    k_fpanel<PRED><<<dim3(2 * (N - 1), batch), 256, 0, st>>>(D, P, n, c, N, prio);
     FWX_LAUNCHED("k_fpanel");
   }
   return FWX_OK;
@@ -691,21 +799,22 @@ int forward_sideband(float* D, int32_t* P, int32_t n, int32_t N, int32_t batch, 
                      hipEvent_t evR, hipEvent_t evL) {
   FWX_HIP(hipEventRecord(evR, st));  // the side stream starts after the caller's prior work
   FWX_HIP(hipStreamWaitEvent(side, evR, 0));
-  if (int rc = lead_round<PRED>(D, P, n, 0, N, batch, side)) return rc;
+  const int32_t prio = diff_side_prio();
+  if (int rc = lead_round<PRED>(D, P, n, 0, N, batch, prio, side)) return rc;
   FWX_HIP(hipEventRecord(evL, side));
   for (int c = 0; c < N; ++c) {
     FWX_HIP(hipStreamWaitEvent(st, evL, 0));
     if (c + 1 < N) {
-      frest_launch<PRED>(D, P, n, c, N, 1, 2 * N - 3, batch, side);
+      frest_launch<PRED>(D, P, n, c, N, 1, 2 * N - 3, batch, prio, side);
       FWX_LAUNCHED("k_frest");
-      if (int rc = lead_round<PRED>(D, P, n, c + 1, N, batch, side)) return rc;
+      if (int rc = lead_round<PRED>(D, P, n, c + 1, N, batch, prio, side)) return rc;
       FWX_HIP(hipEventRecord(evL, side));
-      frest_launch<PRED>(D, P, n, c, N, 2, (N - 2) * (N - 2), batch, st);
+      frest_launch<PRED>(D, P, n, c, N, 2, (N - 2) * (N - 2), batch, 0, st);
       FWX_LAUNCHED("k_frest");
       FWX_HIP(hipEventRecord(evR, st));
       FWX_HIP(hipStreamWaitEvent(side, evR, 0));
     } else {
-      frest_launch<PRED>(D, P, n, c, N, 0, (N - 1) * (N - 1), batch, st);
+      frest_launch<PRED>(D, P, n, c, N, 0, (N - 1) * (N - 1), batch, 0, st);
       FWX_LAUNCHED("k_frest");
     }
   }
@@ -722,9 +831,9 @@ int forward_impl(const float* W, float* D, int32_t* P, int32_t n, int32_t batch,
   const int mode = diff_schedule();
   if (N < 3 || mode == 0) {
     for (int c = 0; c < N; ++c) {
-      if (int rc = lead_round<PRED>(D, P, n, c, N, batch, st)) return rc;
+      if (int rc = lead_round<PRED>(D, P, n, c, N, batch, 0, st)) return rc;
       if (N > 1) {
-        frest_launch<PRED>(D, P, n, c, N, 0, (N - 1) * (N - 1), batch, st);
+        frest_launch<PRED>(D, P, n, c, N, 0, (N - 1) * (N - 1), batch, 0, st);
         FWX_LAUNCHED("k_frest");
       }
     }
@@ -746,10 +855,10 @@ int forward_impl(const float* W, float* D, int32_t* P, int32_t n, int32_t batch,
     (void)hipEventDestroy(evB);
     return rc;
   }
-  rc = lead_round<PRED>(D, P, n, 0, N, batch, st);
+  rc = lead_round<PRED>(D, P, n, 0, N, batch, 0, st);
   for (int c = 0; c < N && rc == FWX_OK; ++c) {
     if (c + 1 < N) {
-      frest_launch<PRED>(D, P, n, c, N, 1, 2 * N - 3, batch, st);
+      frest_launch<PRED>(D, P, n, c, N, 1, 2 * N - 3, batch, 0, st);
       e = hipGetLastError();
       if (e == hipSuccess) e = hipEventRecord(evA, st);
       if (e == hipSuccess) e = hipStreamWaitEvent(side, evA, 0);
@@ -757,16 +866,16 @@ int forward_impl(const float* W, float* D, int32_t* P, int32_t n, int32_t batch,
         rc = hip_fail(e, "diff look-ahead band", __FILE__, __LINE__);
         break;
       }
-      if ((rc = lead_round<PRED>(D, P, n, c + 1, N, batch, side))) break;
+      if ((rc = lead_round<PRED>(D, P, n, c + 1, N, batch, diff_side_prio(), side))) break;
       e = hipEventRecord(evB, side);
       if (e == hipSuccess) {
-        frest_launch<PRED>(D, P, n, c, N, 2, (N - 2) * (N - 2), batch, st);
+        frest_launch<PRED>(D, P, n, c, N, 2, (N - 2) * (N - 2), batch, 0, st);
         e = hipGetLastError();
       }
       if (e == hipSuccess) e = hipStreamWaitEvent(st, evB, 0);
       if (e != hipSuccess) rc = hip_fail(e, "diff look-ahead rest", __FILE__, __LINE__);
     } else {
-      frest_launch<PRED>(D, P, n, c, N, 0, (N - 1) * (N - 1), batch, st);
+      frest_launch<PRED>(D, P, n, c, N, 0, (N - 1) * (N - 1), batch, 0, st);
       e = hipGetLastError();
       if (e != hipSuccess) rc = hip_fail(e, "k_frest", __FILE__, __LINE__);
     }

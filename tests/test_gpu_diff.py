@@ -92,9 +92,10 @@ def test_forward_schedules_agree(monkeypatch, batch, n):
 
 @pytest.mark.parametrize("batch,n", [(3, 129), (2, 300), (1, 640), (4, 200), (2, 258), (1, 1023)])
 def test_forward_rest_kernels_agree(monkeypatch, batch, n):
-    """The vectorised rest product (default) and the first one
-    (FWX_DIFF_REST=0) give the same bits for D and P, with P stored and
-    without, on 16-byte (n % 4 == 0) and scalar (ragged n) paths."""
+    """The vectorised rest product and two-step diagonal closure (default)
+    and the first versions (FWX_DIFF_REST=0, FWX_DIFF_DIAG=0) give the same
+    bits for D and P, with P stored and without, on 16-byte (n % 4 == 0) and
+    scalar (ragged n) paths."""
     import torch
     from fw_cuda_amd.autograd import forward_with_pred
     from fw_cuda_amd import _lib
@@ -103,6 +104,7 @@ def test_forward_rest_kernels_agree(monkeypatch, batch, n):
     out = {}
     for v in ("0", "1"):
         monkeypatch.setenv("FWX_DIFF_REST", v)
+        monkeypatch.setenv("FWX_DIFF_DIAG", v)
         D, P = forward_with_pred(Wt)
         Dn = torch.empty_like(Wt)
         s = torch.cuda.current_stream().cuda_stream

@@ -55,7 +55,13 @@ def check_all(got, expect, flat_after, flat_before, sizes, offs):
     assert (flat_after[mask] == flat_before[mask]).all()
 
 
-def test_batch_small_mixed():
+@pytest.mark.parametrize("deep", ["default", "off", "64,64,4", "100,128,3"])
+def test_batch_small_mixed(monkeypatch, deep):
+    """Mixed sizes (negative weights in some graphs) under the default batch
+    configuration rule, the plain path's tuned entries ("off"), and two
+    other per-size overrides: every graph equals the oracle."""
+    if deep != "default":
+        monkeypatch.setenv("FWX_BATCH_DEEP", deep)
     rng = np.random.default_rng(3)
     ns = [1, 2, 3, 5, 17, 31, 32, 33, 63, 64, 65, 100, 127, 128, 129, 200, 255, 256, 300, 399, 513, 640]
     ns = ns + list(rng.integers(1, 400, 18))

@@ -44,6 +44,7 @@ def parse():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-diff", action="store_true")
     ap.add_argument("--no-pipeline", action="store_true")
+    ap.add_argument("--no-config3", action="store_true")
     return ap.parse_args()
 
 
@@ -144,6 +145,48 @@ def bench_pipeline(n):
         dt = time.perf_counter() - t0
         out["predecessors_s"] = dt
         out["predecessors_n3_per_s"] = float(n) ** 3 / dt
+    return out
+
+
+def bench_config3():
+    """n=32768 (4 GiB int32), density 0.1, weights U[1,1000], seed 7 (the
+    reference-compatible host generator): one multi_layer_tiling_gpu and one
+    tiling_gpu solve (tuned configurations), n^3/s each, both results equal,
+    and every row of the multi-layer result certified on the device against
+    the input."""
+    import torch
+    import fw_cuda_amd as fx
+    from fw_cuda_amd import _lib
+    n = 32768
+    dev = torch.device("cuda:0")
+    t0 = time.perf_counter()
+    h = fx.generate_graph(n, 0.1, 7, 1, 1000)
+    gen_s = time.perf_counter() - t0
+    w = torch.from_numpy(h).to(dev)
+    del h
+    stream = torch.cuda.current_stream(dev).cuda_stream
+    out = {"n": n, "host_generator_s": gen_s}
+    res = {}
+    for name in ("multi_layer_tiling_gpu", "tiling_gpu"):
+        d = w.clone()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        fx.run_device(name, _lib.ptr(d), n, 0, 0, stream)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        tile, kappa = fx.tuned_config(n)
+        out[name] = {"seconds": dt, "n3_per_s": float(n) ** 3 / dt, "tile": tile,
+                     "kappa": kappa if name == "multi_layer_tiling_gpu" else 1}
+        res[name] = d
+    out["equal"] = bool(torch.equal(res["multi_layer_tiling_gpu"], res["tiling_gpu"]))
+    del res["tiling_gpu"]
+    srcv = torch.arange(n, dtype=torch.int32, device=dev)
+    okv = torch.zeros(n, dtype=torch.int32, device=dev)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    failed = fx.certify_rows_device(_lib.ptr(w), n, _lib.ptr(srcv), n, _lib.ptr(res["multi_layer_tiling_gpu"]),
+                                    _lib.ptr(okv), stream)
+    out["certificate"] = {"rows_certified": n, "rows_failed": failed, "seconds": time.perf_counter() - t0}
     return out
 
 
@@ -374,6 +417,8 @@ def main():
         out["differentiable"] = bench_diff(3)
     if not a.no_pipeline:
         out["input_pipeline"] = bench_pipeline(n)
+    if not a.no_config3:
+        out["config3"] = bench_config3()
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(a.cpu_n, a.cpu_n_warm)
     print(json.dumps(out), flush=True)

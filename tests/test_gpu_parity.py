@@ -282,6 +282,35 @@ def test_config3_n32768_spot_rows():
     assert ok.all(), rows[~ok][:10]
 
 
+def test_max_n_32769():
+    """n = MAX_N = 32769 (include/constants.h:5; 4.3 GB int32, ragged: no
+    16-byte path, edge tiles everywhere): multi_layer_tiling_gpu certified
+    exact on every row by the device certificate, 32 rows equal to host
+    Dijkstra over the INF-completed graph."""
+    import ctypes
+    import torch
+    n = fx.MAX_N
+    assert n == 32769
+    g = fx.generate_graph(n, 0.05, 11, 1, 1000)
+    rows = np.linspace(0, n - 1, 32).astype(np.int32)
+    say("max n: dijkstra on", rows.size, "rows")
+    dij = po.dijkstra_rows(g, rows)
+    with fx.DeviceMatrix(n) as dm, fx.DeviceMatrix(n) as dw:
+        dm.upload(g)
+        dw.upload(g)
+        del g
+        say("max n: multi_layer_tiling_gpu")
+        fx.run_device("multi_layer_tiling_gpu", dm.ptr, n, 0, 0)
+        src = torch.arange(n, dtype=torch.int32, device="cuda")
+        ok = torch.zeros(n, dtype=torch.int32, device="cuda")
+        say("max n: device certificate of every row")
+        failed = fx.certify_rows_device(dw.ptr, n, ctypes.c_void_p(src.data_ptr()), n, dm.ptr,
+                                        ctypes.c_void_p(ok.data_ptr()))
+        got = dm.download()[rows]
+    assert failed == 0
+    assert (got == dij).all()
+
+
 @pytest.mark.parametrize("lookahead", ["0", "1", "2"])
 def test_graph_replay_cache(monkeypatch, lookahead):
     """Repeated solves on one buffer replay a captured hipGraph; every replay

@@ -55,11 +55,15 @@ def check_all(got, expect, flat_after, flat_before, sizes, offs):
     assert (flat_after[mask] == flat_before[mask]).all()
 
 
-@pytest.mark.parametrize("deep", ["default", "off", "64,64,4", "100,128,3"])
+@pytest.mark.parametrize("deep", ["default", "off", "64,64,4", "100,128,3", "off/diag256"])
 def test_batch_small_mixed(monkeypatch, deep):
     """Mixed sizes (negative weights in some graphs) under the default batch
-    configuration rule, the plain path's tuned entries ("off"), and two
-    other per-size overrides: every graph equals the oracle."""
+    configuration rule, the plain path's tuned entries ("off", also with
+    256-thread 64-wide closures), and two other per-size overrides: every
+    graph equals the oracle."""
+    if deep.endswith("/diag256"):
+        monkeypatch.setenv("FWX_BATCH_DIAG", "256")
+        deep = deep.split("/")[0]
     if deep != "default":
         monkeypatch.setenv("FWX_BATCH_DEEP", deep)
     rng = np.random.default_rng(3)

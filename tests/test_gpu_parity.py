@@ -174,6 +174,21 @@ def test_schedules_agree(monkeypatch, lookahead, prio):
             assert (gpu(g, "multi_layer_tiling_gpu", t, k) == expect).all(), (n, t, k, graph)
 
 
+@pytest.mark.parametrize("var,val", [("FWX_BAND", "1"), ("FWX_DPANEL", "1"), ("FWX_EVENT_SYSFENCE", "1")])
+def test_switches_agree(monkeypatch, var, val):
+    """The remaining schedule switches (128-wide band pass, fused closure +
+    panels, system-scope event fences), graph-replayed and direct."""
+    monkeypatch.setenv(var, val)
+    rng = np.random.default_rng(44)
+    for n, t, k in [(1537, 128, 2), (777, 64, 3), (900, 32, 4), (2048, 64, 4), (1100, 128, 1)]:
+        g = fx.generate_graph(n, float(rng.uniform(0.1, 0.9)), int(rng.integers(1, 2**31)), 1, 1000)
+        expect = po.fw_rowpar(g)
+        for graph in ("1", "0"):
+            monkeypatch.setenv("FWX_GRAPH", graph)
+            say("switch", var, val, graph, n, t, k)
+            assert (gpu(g, "multi_layer_tiling_gpu", t, k) == expect).all(), (n, t, k, graph)
+
+
 @pytest.mark.parametrize("variant", ["0", "1", "2"])
 def test_rest_kernel_variants(monkeypatch, variant):
     """Every shape of the 128-wide rest kernel (FWX_REST_VARIANT), ragged and

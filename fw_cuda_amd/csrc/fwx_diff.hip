@@ -35,6 +35,13 @@ namespace {
 constexpr int FT = 64;   // tile
 constexpr int FKC = 32;  // k chunk
 constexpr int FLDA = FKC + 4;
+// Chunk depth of the fast rest product.  16 halves its LDS (25.6 KB per
+// block, up to 5 blocks per CU instead of 3) for twice the barriers; at
+// n=4096 the forward takes 8.55 ms with 16 and 9.40 ms with 32.
+#ifndef FWX_DIFF_KC
+#define FWX_DIFF_KC 16
+#endif
+constexpr int kRestKC = FWX_DIFF_KC;
 constexpr float kFInf = __builtin_huge_valf();
 
 // Window of a row-major fp32 matrix and its predecessor matrix.
@@ -377,28 +384,33 @@ __global__ __launch_bounds__(256) void k_frest(float* __restrict__ D, int32_t* _
 // bit-identical; laid out for 16-byte LDS reads.  Thread (tx, ty) owns rows
 // ty + 16 r and the 4 adjacent columns 4 tx + c; per 4 k it reads 4 A rows
 // (one float4 each: 4 k values), and 4 B and 4 predecessor rows (one
-// float4 / int4 each: its 4 columns).  The 32-deep chunks are
+// float4 / int4 each: its 4 columns).  The chunks (kRestKC deep) are
 // double-buffered: the next chunk's global loads are in flight while the
 // current one is computed.
+template <int KC>
 struct alignas(16) FSmem2 {
-  alignas(16) float a[2][FT * FLDA];
-  alignas(16) float b[2][FKC * FT];
-  alignas(16) int32_t bp[2][FKC * FT];
+  alignas(16) float a[2][FT * (KC + 4)];
+  alignas(16) float b[2][KC * FT];
+  alignas(16) int32_t bp[2][KC * FT];
 };
 
-template <bool PRED, bool VEC>
+template <bool PRED, bool VEC, int KC>
 struct FProd4 {
-  // staging: A 64 rows x 8 quads, B / BP 32 rows x 16 quads -> 2 units each
-  float4 ra[2], rb[2];
-  int4 rp[2];
+  static constexpr int LDA = KC + 4;
+  static constexpr int AQ = KC / 4;         // 4-k quads per A row
+  static constexpr int NP = KC / 16;        // staging units per thread (A and B alike)
+  static_assert(KC % 16 == 0, "chunk of 16 or 32");
+  // staging: A 64 rows x AQ quads, B / BP KC rows x 16 quads
+  float4 ra[NP], rb[NP];
+  int4 rp[NP];
 
   __device__ __forceinline__ void fetch(const FView& A, const FView& B, int32_t kc) {
     const int tid = threadIdx.x;
     const float4 inf4 = make_float4(kFInf, kFInf, kFInf, kFInf);
 #pragma unroll
-    for (int p = 0; p < 2; ++p) {
+    for (int p = 0; p < NP; ++p) {
       const int u = p * 256 + tid;
-      const int r = u >> 3, k = kc + (u & 7) * 4;
+      const int r = u / AQ, k = kc + (u % AQ) * 4;
       if (VEC) {
         ra[p] = (r < A.rlim && k < A.clim) ? *reinterpret_cast<const float4*>(A.p + r * A.ld + k) : inf4;
       } else {
@@ -427,25 +439,25 @@ struct FProd4 {
     }
   }
 
-  __device__ __forceinline__ void stash(FSmem2& s, int buf) const {
+  __device__ __forceinline__ void stash(FSmem2<KC>& s, int buf) const {
     const int tid = threadIdx.x;
 #pragma unroll
-    for (int p = 0; p < 2; ++p) {
+    for (int p = 0; p < NP; ++p) {
       const int u = p * 256 + tid;
-      *reinterpret_cast<float4*>(&s.a[buf][(u >> 3) * FLDA + (u & 7) * 4]) = ra[p];
+      *reinterpret_cast<float4*>(&s.a[buf][(u / AQ) * LDA + (u % AQ) * 4]) = ra[p];
       *reinterpret_cast<float4*>(&s.b[buf][(u >> 4) * FT + (u & 15) * 4]) = rb[p];
       if (PRED) *reinterpret_cast<int4*>(&s.bp[buf][(u >> 4) * FT + (u & 15) * 4]) = rp[p];
     }
   }
 
-  __device__ __forceinline__ static void compute(const FSmem2& s, int buf, float (&acc)[4][4],
+  __device__ __forceinline__ static void compute(const FSmem2<KC>& s, int buf, float (&acc)[4][4],
                                                  int32_t (&pacc)[4][4]) {
     const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
 #pragma unroll 2
-    for (int q = 0; q < FKC / 4; ++q) {
+    for (int q = 0; q < KC / 4; ++q) {
       float4 a4[4];
 #pragma unroll
-      for (int r = 0; r < 4; ++r) a4[r] = *reinterpret_cast<const float4*>(&s.a[buf][(ty + 16 * r) * FLDA + 4 * q]);
+      for (int r = 0; r < 4; ++r) a4[r] = *reinterpret_cast<const float4*>(&s.a[buf][(ty + 16 * r) * LDA + 4 * q]);
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
         const float4 b4 = *reinterpret_cast<const float4*>(&s.b[buf][(4 * q + e) * FT + 4 * tx]);
@@ -523,7 +535,7 @@ template <bool PRED, bool VEC>
 __global__ __launch_bounds__(256) void k_frest4(float* __restrict__ D, int32_t* __restrict__ P, int32_t n,
                                                 int32_t c, int32_t N, int32_t mode, int32_t prio) {
   if (prio) __builtin_amdgcn_s_setprio(3);
-  __shared__ FSmem2 s;
+  __shared__ FSmem2<kRestKC> s;
   const int64_t g = blockIdx.y;
   float* Dg = D + g * (int64_t)n * n;
   int32_t* Pg = PRED ? P + g * (int64_t)n * n : nullptr;
@@ -535,15 +547,15 @@ __global__ __launch_bounds__(256) void k_frest4(float* __restrict__ D, int32_t* 
   float acc[4][4];
   int32_t pacc[4][4];
   fload4<PRED, VEC>(C, acc, pacc);
-  FProd4<PRED, VEC> pr;
+  FProd4<PRED, VEC, kRestKC> pr;
   pr.fetch(A, B, 0);
   pr.stash(s, 0);
   __syncthreads();
-  constexpr int nch = FT / FKC;
+  constexpr int nch = FT / kRestKC;
 #pragma unroll
   for (int ch = 0; ch < nch; ++ch) {
-    if (ch + 1 < nch) pr.fetch(A, B, (ch + 1) * FKC);
-    FProd4<PRED, VEC>::compute(s, ch & 1, acc, pacc);
+    if (ch + 1 < nch) pr.fetch(A, B, (ch + 1) * kRestKC);
+    FProd4<PRED, VEC, kRestKC>::compute(s, ch & 1, acc, pacc);
     if (ch + 1 < nch) pr.stash(s, (ch + 1) & 1);
     __syncthreads();
   }

@@ -469,11 +469,13 @@ struct FProd4 {
           const float ae = (&a4[r].x)[e];
 #pragma unroll
           for (int c = 0; c < 4; ++c) {
+            // The value is min(acc, cand) (v_min_f32: no NaN, and a
+            // candidate equal to acc leaves the same value), so only the
+            // predecessor select reads the strict-< mask, and the min can
+            // fill the wait states the mask needs.
             const float cand = ae + bv[c];
-            if (cand < acc[r][c]) {
-              acc[r][c] = cand;
-              if (PRED) pacc[r][c] = (&p4.x)[c];
-            }
+            if (PRED && cand < acc[r][c]) pacc[r][c] = (&p4.x)[c];
+            acc[r][c] = fminf(acc[r][c], cand);
           }
         }
       }

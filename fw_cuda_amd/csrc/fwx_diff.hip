@@ -564,6 +564,50 @@ __global__ __launch_bounds__(256) void k_frest4(float* __restrict__ D, int32_t* 
   fstore4<PRED, VEC>(C, acc, pacc);
 }
 
+// Panels of stage c with FProd4 (same tiles and operands as k_fpanel: the
+// row/column tile's own pre-update values are one operand, staged chunk by
+// chunk before the block stores anything).
+template <bool PRED, bool VEC>
+__global__ __launch_bounds__(256) void k_fpanel4(float* __restrict__ D, int32_t* __restrict__ P, int32_t n,
+                                                 int32_t c, int32_t N, int32_t prio) {
+  if (prio) __builtin_amdgcn_s_setprio(3);
+  __shared__ FSmem2<kRestKC> s;
+  const int64_t g = blockIdx.y;
+  float* Dg = D + g * (int64_t)n * n;
+  int32_t* Pg = PRED ? P + g * (int64_t)n * n : nullptr;
+  const int b = blockIdx.x;
+  const FView diag = fview(Dg, Pg, n, c * FT, c * FT);
+  FView A, B, C;
+  if (b < N - 1) {
+    const int j = b + (b >= c);
+    C = fview(Dg, Pg, n, c * FT, j * FT);
+    A = diag;
+    B = C;
+  } else {
+    int i = b - (N - 1);
+    i += (i >= c);
+    C = fview(Dg, Pg, n, i * FT, c * FT);
+    A = C;
+    B = diag;
+  }
+  float acc[4][4];
+  int32_t pacc[4][4];
+  fload4<PRED, VEC>(C, acc, pacc);
+  FProd4<PRED, VEC, kRestKC> pr;
+  pr.fetch(A, B, 0);
+  pr.stash(s, 0);
+  __syncthreads();
+  constexpr int nch = FT / kRestKC;
+#pragma unroll
+  for (int ch = 0; ch < nch; ++ch) {
+    if (ch + 1 < nch) pr.fetch(A, B, (ch + 1) * kRestKC);
+    FProd4<PRED, VEC, kRestKC>::compute(s, ch & 1, acc, pacc);
+    if (ch + 1 < nch) pr.stash(s, (ch + 1) & 1);
+    __syncthreads();
+  }
+  fstore4<PRED, VEC>(C, acc, pacc);
+}
+
 // Predecessors of sources [s0, s0+rows) recomputed from D and W:
 // Pw[r][t] = argmin_{u != t} D[s0+r][u] + W[u][t], -1 for t == s or
 // unreachable t.  Grid: (ceil(rows/64) * ceil(n/64), batch).
@@ -736,9 +780,9 @@ __global__ __launch_bounds__(kBackThreads) void k_fback(const float* __restrict_
   }
 }
 
-// Rest-pass kernel: FWX_DIFF_REST=0 selects the first version (k_frest,
-// scalar LDS reads, single-buffered), bit-identical to the default k_frest4
-// (read on every call).
+// Product kernels: FWX_DIFF_REST=0 selects the first versions (k_frest,
+// k_fpanel: scalar LDS reads, single-buffered), bit-identical to the default
+// k_frest4 / k_fpanel4 (read on every call).
 static bool diff_rest_fast() {
   const char* e = std::getenv("FWX_DIFF_REST");
   return !(e && e[0] == '0');
@@ -782,7 +826,14 @@ int lead_round(float* D, int32_t* P, int32_t n, int32_t c, int32_t N, int32_t ba
     FWX_LAUNCHED("k_fdiag");
   }
   if (N > 1) {
-    k_fpanel<PRED><<<dim3(2 * (N - 1), batch), 256, 0, st>>>(D, P, n, c, N, prio);
+    const bool vec = (n % 4 == 0) && ((reinterpret_cast<uintptr_t>(D) & 15u) == 0) &&
+                     (!PRED || (reinterpret_cast<uintptr_t>(P) & 15u) == 0);
+    if (!diff_rest_fast())
+      k_fpanel<PRED><<<dim3(2 * (N - 1), batch), 256, 0, st>>>(D, P, n, c, N, prio);
+    else if (vec)
+      k_fpanel4<PRED, true><<<dim3(2 * (N - 1), batch), 256, 0, st>>>(D, P, n, c, N, prio);
+    else
+      k_fpanel4<PRED, false><<<dim3(2 * (N - 1), batch), 256, 0, st>>>(D, P, n, c, N, prio);
     FWX_LAUNCHED("k_fpanel");
   }
   return FWX_OK;

@@ -653,7 +653,7 @@ __global__ __launch_bounds__(256) void k_ffill(float* __restrict__ x, int64_t to
 // Backward for sources [s0, s0 + rows) of every graph.  Pr points at the
 // predecessor rows of those sources (row stride n, graph stride pstride).
 //
-// Level-synchronous tree accumulation, one 1024-thread workgroup per source
+// Level-synchronous tree accumulation, one workgroup (NT threads) per source
 // s.  The predecessor row is a tree rooted at s; a vertex's gradient
 // g[t] = G[s][t] + sum of its children's g is final once every deeper level
 // is done, so:
@@ -666,9 +666,9 @@ __global__ __launch_bounds__(256) void k_ffill(float* __restrict__ x, int64_t to
 // sort of all n keys followed by one lane walking n vertices.
 // Dynamic LDS, 14 bytes per vertex + 8: par, dep, aux (int16: vertex ids
 // and depths < 2^15), g (float), off (int32, n + 2).
-constexpr int kBackThreads = 1024;
 
-__global__ __launch_bounds__(kBackThreads) void k_fback(const float* __restrict__ D, const int32_t* __restrict__ Pr,
+template <int NT, int PER>
+__global__ __launch_bounds__(NT) void k_fback(const float* __restrict__ D, const int32_t* __restrict__ Pr,
                                                         int64_t pstride, const float* __restrict__ G,
                                                         float* __restrict__ dW, int32_t n, int32_t s0) {
   extern __shared__ __align__(16) unsigned char smem[];
@@ -692,7 +692,7 @@ __global__ __launch_bounds__(kBackThreads) void k_fback(const float* __restrict_
     s_changed[0] = 0;
     s_changed[1] = 0;
   }
-  for (int t = tid; t < n; t += kBackThreads) {
+  for (int t = tid; t < n; t += NT) {
     const int32_t p = Prow[t];
     const bool reach = t != src && p >= 0 && Drow[t] < kFInf;
     par[t] = (int16_t)(reach ? p : -1);
@@ -700,20 +700,20 @@ __global__ __launch_bounds__(kBackThreads) void k_fback(const float* __restrict_
     aux[t] = (int16_t)(reach ? p : -1);
     g[t] = Grow[t];
   }
-  for (int d = tid; d < n + 2; d += kBackThreads) off[d] = 0;
+  for (int d = tid; d < n + 2; d += NT) off[d] = 0;
   __syncthreads();
   // 1. pointer jumping: after it, aux[t] == src and dep[t] = hops for every
   // reachable t.  Rounds are synchronous (all reads, barrier, all writes,
   // barrier); the "anything moved" flag of round r lives in slot r & 1 and
   // is cleared for round r + 1 only after every thread has passed round r's
   // first barrier, i.e. finished reading the slot of round r - 1.
-  constexpr int kPer = 8;  // n <= 8192 = 8 * 1024
+  constexpr int kPer = PER;  // the host launches with n <= NT * PER
   for (int round = 0; round < 16; ++round) {
     int16_t nd[kPer], na[kPer];
     int changed = 0;
 #pragma unroll
     for (int e = 0; e < kPer; ++e) {
-      const int t = tid + e * kBackThreads;
+      const int t = tid + e * NT;
       nd[e] = 0;
       na[e] = -1;
       if (t < n) {
@@ -729,7 +729,7 @@ __global__ __launch_bounds__(kBackThreads) void k_fback(const float* __restrict_
     if (changed) s_changed[round & 1] = 1;
 #pragma unroll
     for (int e = 0; e < kPer; ++e) {
-      const int t = tid + e * kBackThreads;
+      const int t = tid + e * NT;
       if (t < n && na[e] >= 0) {
         dep[t] = (int16_t)(dep[t] + nd[e]);
         aux[t] = na[e];
@@ -742,7 +742,7 @@ __global__ __launch_bounds__(kBackThreads) void k_fback(const float* __restrict_
   }
   // 2. counting sort by depth: off[d] counts level d, then exclusive prefix
   // sums (off[0] = 0 stays), then scatter into aux (order by level)
-  for (int t = tid; t < n; t += kBackThreads) {
+  for (int t = tid; t < n; t += NT) {
     const int d = dep[t];
     if (d > 0 && d <= n) {
       atomicAdd(&off[d], 1);
@@ -760,7 +760,7 @@ __global__ __launch_bounds__(kBackThreads) void k_fback(const float* __restrict_
     }
   }
   __syncthreads();
-  for (int t = tid; t < n; t += kBackThreads) {
+  for (int t = tid; t < n; t += NT) {
     const int d = dep[t];
     if (d > 0 && d <= n) aux[atomicAdd(&off[d], 1)] = (int16_t)t;
   }
@@ -769,7 +769,7 @@ __global__ __launch_bounds__(kBackThreads) void k_fback(const float* __restrict_
   // 3. deepest level first
   for (int d = H; d >= 1; --d) {
     const int q0 = off[d - 1], q1 = off[d];
-    for (int q = q0 + tid; q < q1; q += kBackThreads) {
+    for (int q = q0 + tid; q < q1; q += NT) {
       const int t = aux[q];
       const int p = par[t];
       const float gt = g[t];
@@ -989,12 +989,26 @@ int fwx_diff_backward(const float* d_W, const float* d_D, const int32_t* d_P, co
   k_ffill<<<(int)std::min<int64_t>((total + 255) / 256, 65536), 256, 0, st>>>(d_dW, total);
   FWX_LAUNCHED("k_ffill");
   const size_t lds = (size_t)n * 14 + 8;
-  if (lds > 65536)
-    FWX_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fback),
-                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  // one workgroup per source: 256 threads up to n = 1024, 512 up to 4096,
+  // 1024 above (each thread holds at most PER vertices in the jumping rounds)
+  auto back = [&](const int32_t* pr, int64_t pstride, int32_t rows, int32_t s0) -> hipError_t {
+    const dim3 grid(rows, batch);
+    if (n <= 1024) {
+      k_fback<256, 4><<<grid, 256, lds, st>>>(d_D, pr, pstride, d_G, d_dW, n, s0);
+    } else if (n <= 4096) {
+      k_fback<512, 8><<<grid, 512, lds, st>>>(d_D, pr, pstride, d_G, d_dW, n, s0);
+    } else {
+      if (lds > 65536) {
+        const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fback<1024, 8>),
+                                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+      }
+      k_fback<1024, 8><<<grid, 1024, lds, st>>>(d_D, pr, pstride, d_G, d_dW, n, s0);
+    }
+    return hipGetLastError();
+  };
   if (d_P) {
-    k_fback<<<dim3(n, batch), kBackThreads, lds, st>>>(d_D, d_P, (int64_t)n * n, d_G, d_dW, n, 0);
-    FWX_LAUNCHED("k_fback");
+    FWX_HIP(back(d_P, (int64_t)n * n, n, 0));
   } else {
     // source blocks of 256 rows; workspace 256 x n x batch int32
     const int32_t rows = std::min<int32_t>(256, n);
@@ -1006,10 +1020,7 @@ int fwx_diff_backward(const float* d_W, const float* d_D, const int32_t* d_P, co
       const int nt = (n + FT - 1) / FT;
       k_fpred<<<dim3(((r + FT - 1) / FT) * nt, batch), 256, 0, st>>>(d_W, d_D, ws, n, s0, r);
       hipError_t e = hipGetLastError();
-      if (e == hipSuccess) {
-        k_fback<<<dim3(r, batch), kBackThreads, lds, st>>>(d_D, ws, (int64_t)r * n, d_G, d_dW, n, s0);
-        e = hipGetLastError();
-      }
+      if (e == hipSuccess) e = back(ws, (int64_t)r * n, r, s0);
       if (e != hipSuccess) rc = hip_fail(e, "k_fpred/k_fback", __FILE__, __LINE__);
     }
     (void)hipFreeAsync(ws, st);

@@ -608,6 +608,122 @@ __global__ __launch_bounds__(256) void k_fpanel4(float* __restrict__ D, int32_t*
   fstore4<PRED, VEC>(C, acc, pacc);
 }
 
+// ---------------------------------------------------------------------------
+// Multi-layer stages (kappa k-tiles per stage, FWX_DIFF_KAPPA).  Per stage
+// of k-tiles [e0, e0+en): for each c in it, the closure of (c, c), the
+// panels of row and column c over every tile, and the cross update of the
+// stage's other row/column tiles with k-tile c; then one rest pass of depth
+// en*64 over every tile outside the stage's rows and columns.  Each product
+// walks its k ascending with the same strict < rule, so ties keep the
+// earlier candidate as before; sums may round differently from the
+// one-k-tile schedule (a different decomposition of the same paths).
+
+// Tile (i, j) of block b of a multi-layer rest pass.  Stage tiles E =
+// [e0, e0+en), next stage F = [f0, f0+fn) with f0 = e0 + en.
+//   mode 0: i, j not in E, (N-en)^2 blocks;
+//   mode 1: the band F's lead reads -- i in F, j not in E (fn (N-en)
+//           blocks), then i not in E or F, j in F ((N-en-fn) fn blocks);
+//   mode 2: i, j not in E or F, (N-en-fn)^2 blocks.
+__device__ __forceinline__ void rest_tile_ml(int b, int e0, int en, int fn, int N, int mode, int* I, int* J) {
+  int i, j;
+  if (mode == 0) {
+    const int m = N - en;
+    i = b / m;
+    j = b % m;
+    i += (i >= e0) ? en : 0;
+    j += (j >= e0) ? en : 0;
+  } else if (mode == 1) {
+    const int m = N - en;
+    const int f0 = e0 + en;
+    if (b < fn * m) {
+      i = f0 + b / m;
+      j = b % m;
+      j += (j >= e0) ? en : 0;
+    } else {
+      b -= fn * m;
+      i = b / fn;
+      j = f0 + b % fn;
+      i += (i >= e0) ? en + fn : 0;
+    }
+  } else {
+    const int m = N - en - fn;
+    i = b / m;
+    j = b % m;
+    i += (i >= e0) ? en + fn : 0;
+    j += (j >= e0) ? en + fn : 0;
+  }
+  *I = i;
+  *J = j;
+}
+
+template <bool PRED, bool VEC>
+__device__ __forceinline__ void ftile_product(FSmem2<kRestKC>& s, const FView& A, const FView& B, const FView& C,
+                                              int32_t depth) {
+  float acc[4][4];
+  int32_t pacc[4][4];
+  fload4<PRED, VEC>(C, acc, pacc);
+  FProd4<PRED, VEC, kRestKC> pr;
+  pr.fetch(A, B, 0);
+  pr.stash(s, 0);
+  __syncthreads();
+  const int nch = depth / kRestKC;
+  for (int ch = 0; ch < nch; ++ch) {
+    if (ch + 1 < nch) pr.fetch(A, B, (ch + 1) * kRestKC);
+    FProd4<PRED, VEC, kRestKC>::compute(s, ch & 1, acc, pacc);
+    if (ch + 1 < nch) pr.stash(s, (ch + 1) & 1);
+    __syncthreads();
+  }
+  fstore4<PRED, VEC>(C, acc, pacc);
+}
+
+template <bool PRED, bool VEC>
+__global__ __launch_bounds__(256) void k_frest_ml(float* __restrict__ D, int32_t* __restrict__ P, int32_t n,
+                                                  int32_t e0, int32_t en, int32_t fn, int32_t N, int32_t mode,
+                                                  int32_t prio) {
+  if (prio) __builtin_amdgcn_s_setprio(3);
+  __shared__ FSmem2<kRestKC> s;
+  const int64_t g = blockIdx.y;
+  float* Dg = D + g * (int64_t)n * n;
+  int32_t* Pg = PRED ? P + g * (int64_t)n * n : nullptr;
+  int i, j;
+  rest_tile_ml(blockIdx.x, e0, en, fn, N, mode, &i, &j);
+  ftile_product<PRED, VEC>(s, fview(Dg, Pg, n, i * FT, e0 * FT), fview(Dg, Pg, n, e0 * FT, j * FT),
+                           fview(Dg, Pg, n, i * FT, j * FT), en * FT);
+}
+
+// Cross update of stage [kb, kb+Bst) with k-tile c: stage rows (except c) x
+// every column (except c), (Bst-1)(N-1) tiles, then other rows x stage
+// columns (except c), (N-Bst)(Bst-1) tiles.
+template <bool PRED, bool VEC>
+__global__ __launch_bounds__(256) void k_fcross(float* __restrict__ D, int32_t* __restrict__ P, int32_t n,
+                                                int32_t c, int32_t kb, int32_t Bst, int32_t N, int32_t prio) {
+  if (prio) __builtin_amdgcn_s_setprio(3);
+  __shared__ FSmem2<kRestKC> s;
+  const int64_t g = blockIdx.y;
+  float* Dg = D + g * (int64_t)n * n;
+  int32_t* Pg = PRED ? P + g * (int64_t)n * n : nullptr;
+  int b = blockIdx.x;
+  const int l = c - kb;
+  const int r1 = (Bst - 1) * (N - 1);
+  int i, j;
+  if (b < r1) {
+    int m = b / (N - 1);
+    const int jj = b % (N - 1);
+    m += (m >= l);
+    i = kb + m;
+    j = jj + (jj >= c);
+  } else {
+    b -= r1;
+    const int ii = b / (Bst - 1);
+    int m = b % (Bst - 1);
+    m += (m >= l);
+    j = kb + m;
+    i = ii + (ii >= kb ? Bst : 0);
+  }
+  ftile_product<PRED, VEC>(s, fview(Dg, Pg, n, i * FT, c * FT), fview(Dg, Pg, n, c * FT, j * FT),
+                           fview(Dg, Pg, n, i * FT, j * FT), FT);
+}
+
 // Predecessors of sources [s0, s0+rows) recomputed from D and W:
 // Pw[r][t] = argmin_{u != t} D[s0+r][u] + W[u][t], -1 for t == s or
 // unreachable t.  Grid: (ceil(rows/64) * ceil(n/64), batch).
@@ -886,6 +1002,115 @@ int forward_sideband(float* D, int32_t* P, int32_t n, int32_t N, int32_t batch, 
   return FWX_OK;
 }
 
+// Stage width in k-tiles for N 64-wide tiles: 4 from N = 128 (n = 8192),
+// 2 from N = 48, else 1 -- forward ms measured (best of 3 x 4 calls):
+// n=8192 64.1 / 57.7 / 52.9 (kappa 1 / 2 / 4); n=4096 8.05 / 7.44 / 8.15;
+// n=3072 6.80 / 6.54; n=2048 4.16 / 4.23; n=512 (batch 64) 1.24 / 1.39.
+// FWX_DIFF_KAPPA=1..8 overrides (read on every call).
+static int diff_kappa(int N) {
+  if (const char* e = std::getenv("FWX_DIFF_KAPPA")) {
+    const int k = std::atoi(e);
+    if (k >= 1 && k <= 8) return k;
+  }
+  return N >= 128 ? 4 : N >= 48 ? 2 : 1;
+}
+
+template <bool PRED>
+static bool fvec(const float* D, const int32_t* P, int32_t n) {
+  return (n % 4 == 0) && ((reinterpret_cast<uintptr_t>(D) & 15u) == 0) &&
+         (!PRED || (reinterpret_cast<uintptr_t>(P) & 15u) == 0);
+}
+
+// The lead rounds of stage [kb, kb+Bst): closure, panels and cross of each
+// of its k-tiles.
+template <bool PRED>
+int lead_stage_ml(float* D, int32_t* P, int32_t n, int32_t kb, int32_t Bst, int32_t N, int32_t batch,
+                  int32_t prio, hipStream_t st) {
+  const bool vec = fvec<PRED>(D, P, n);
+  for (int l = 0; l < Bst; ++l) {
+    const int c = kb + l;
+    if (int rc = lead_round<PRED>(D, P, n, c, N, batch, prio, st)) return rc;
+    if (Bst > 1) {
+      const int cnt = (Bst - 1) * (2 * N - Bst - 1);
+      if (vec)
+        k_fcross<PRED, true><<<dim3(cnt, batch), 256, 0, st>>>(D, P, n, c, kb, Bst, N, prio);
+      else
+        k_fcross<PRED, false><<<dim3(cnt, batch), 256, 0, st>>>(D, P, n, c, kb, Bst, N, prio);
+      FWX_LAUNCHED("k_fcross");
+    }
+  }
+  return FWX_OK;
+}
+
+template <bool PRED>
+void frest_ml_launch(float* D, int32_t* P, int32_t n, int32_t e0, int32_t en, int32_t fn, int32_t N, int32_t mode,
+                     int32_t batch, int32_t prio, hipStream_t st) {
+  const int m = N - en;
+  const int blocks = mode == 0 ? m * m : mode == 1 ? fn * m + (m - fn) * fn : (m - fn) * (m - fn);
+  if (blocks <= 0) return;
+  if (fvec<PRED>(D, P, n))
+    k_frest_ml<PRED, true><<<dim3(blocks, batch), 256, 0, st>>>(D, P, n, e0, en, fn, N, mode, prio);
+  else
+    k_frest_ml<PRED, false><<<dim3(blocks, batch), 256, 0, st>>>(D, P, n, e0, en, fn, N, mode, prio);
+}
+
+// Multi-layer forward: one stream (mode 0) or side-band (otherwise; the
+// same read/write-set argument as the one-k-tile schedule, with stages in
+// place of k-tiles).
+template <bool PRED>
+int forward_ml(float* D, int32_t* P, int32_t n, int32_t N, int32_t kappa, int32_t batch, int mode,
+               hipStream_t st) {
+  if (mode == 0) {
+    for (int kb = 0; kb < N; kb += kappa) {
+      const int Bst = std::min(kappa, N - kb);
+      if (int rc = lead_stage_ml<PRED>(D, P, n, kb, Bst, N, batch, 0, st)) return rc;
+      frest_ml_launch<PRED>(D, P, n, kb, Bst, 0, N, 0, batch, 0, st);
+      FWX_LAUNCHED("k_frest_ml");
+    }
+    return FWX_OK;
+  }
+  hipStream_t side;
+  if (int rc = side_stream(&side)) return rc;
+  hipEvent_t evR, evL;
+  FWX_HIP(hipEventCreateWithFlags(&evR, event_flags(false)));
+  hipError_t e = hipEventCreateWithFlags(&evL, event_flags(false));
+  if (e != hipSuccess) {
+    (void)hipEventDestroy(evR);
+    return hip_fail(e, "hipEventCreateWithFlags", __FILE__, __LINE__);
+  }
+  const int32_t prio = diff_side_prio();
+  auto body = [&]() -> int {
+    FWX_HIP(hipEventRecord(evR, st));
+    FWX_HIP(hipStreamWaitEvent(side, evR, 0));
+    if (int rc = lead_stage_ml<PRED>(D, P, n, 0, std::min(kappa, N), N, batch, prio, side)) return rc;
+    FWX_HIP(hipEventRecord(evL, side));
+    for (int kb = 0; kb < N; kb += kappa) {
+      const int Bst = std::min(kappa, N - kb);
+      const int kb2 = kb + kappa;
+      FWX_HIP(hipStreamWaitEvent(st, evL, 0));
+      if (kb2 < N) {
+        const int B2 = std::min(kappa, N - kb2);
+        frest_ml_launch<PRED>(D, P, n, kb, Bst, B2, N, 1, batch, prio, side);
+        FWX_LAUNCHED("k_frest_ml");
+        if (int rc = lead_stage_ml<PRED>(D, P, n, kb2, B2, N, batch, prio, side)) return rc;
+        FWX_HIP(hipEventRecord(evL, side));
+        frest_ml_launch<PRED>(D, P, n, kb, Bst, B2, N, 2, batch, 0, st);
+        FWX_LAUNCHED("k_frest_ml");
+        FWX_HIP(hipEventRecord(evR, st));
+        FWX_HIP(hipStreamWaitEvent(side, evR, 0));
+      } else {
+        frest_ml_launch<PRED>(D, P, n, kb, Bst, 0, N, 0, batch, 0, st);
+        FWX_LAUNCHED("k_frest_ml");
+      }
+    }
+    return FWX_OK;
+  };
+  const int rc = body();
+  (void)hipEventDestroy(evR);
+  (void)hipEventDestroy(evL);
+  return rc;
+}
+
 template <bool PRED>
 int forward_impl(const float* W, float* D, int32_t* P, int32_t n, int32_t batch, hipStream_t st) {
   const int64_t total = (int64_t)n * n * batch;
@@ -894,6 +1119,8 @@ int forward_impl(const float* W, float* D, int32_t* P, int32_t n, int32_t batch,
   FWX_LAUNCHED("k_finit");
   const int N = (n + FT - 1) / FT;
   const int mode = diff_schedule();
+  const int kappa = diff_kappa(N);
+  if (kappa > 1 && N > kappa) return forward_ml<PRED>(D, P, n, N, kappa, batch, mode, st);
   if (N < 3 || mode == 0) {
     for (int c = 0; c < N; ++c) {
       if (int rc = lead_round<PRED>(D, P, n, c, N, batch, 0, st)) return rc;

@@ -117,6 +117,31 @@ def test_forward_rest_kernels_agree(monkeypatch, batch, n):
     assert np.array_equal(out["1"][0], out["1"][2], equal_nan=True)
 
 
+@pytest.mark.parametrize("kappa,batch,n", [(2, 2, 1000), (3, 1, 1300), (4, 3, 700), (2, 1, 2100)])
+def test_forward_multilayer(monkeypatch, kappa, batch, n):
+    """Multi-layer stages (kappa k-tiles per stage): the single-stream and
+    side-band schedules give the same bits, and D matches the oracle's fp32
+    FW (predecessors on >= 99.9 % of cells: a different decomposition may
+    round a near-tie differently)."""
+    import torch
+    from fw_cuda_amd.autograd import forward_with_pred
+    monkeypatch.setenv("FWX_DIFF_KAPPA", str(kappa))
+    W, _ = graphs(batch, n, 0.3, 11 * n + kappa)
+    Wt = torch.from_numpy(W).cuda()
+    out = {}
+    for la in ("0", "2"):
+        monkeypatch.setenv("FWX_LOOKAHEAD", la)
+        D, P = forward_with_pred(Wt)
+        out[la] = (D.cpu().numpy(), P.cpu().numpy())
+    assert np.array_equal(out["0"][0], out["2"][0], equal_nan=True)
+    assert (out["0"][1] == out["2"][1]).all()
+    Do, Po = po.fw_f32_pred(W[0])
+    fin = np.isfinite(Do)
+    assert (np.isfinite(out["2"][0][0]) == fin).all()
+    assert np.allclose(out["2"][0][0][fin], Do[fin], rtol=2e-6, atol=0)
+    assert (out["2"][1][0] == Po).mean() > 0.999
+
+
 def test_backward_deep_trees():
     """Shortest-path trees hundreds of levels deep (a ring with a few light
     chords, and a bare chain): the level-by-level accumulation must give the

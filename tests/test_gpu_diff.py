@@ -117,7 +117,7 @@ def test_forward_rest_kernels_agree(monkeypatch, batch, n):
     assert np.array_equal(out["1"][0], out["1"][2], equal_nan=True)
 
 
-@pytest.mark.parametrize("kappa,batch,n", [(2, 2, 1000), (3, 1, 1300), (4, 3, 700), (2, 1, 2100)])
+@pytest.mark.parametrize("kappa,batch,n", [(2, 2, 1000), (3, 1, 1300), (4, 3, 700), (2, 1, 2100), (3, 2, 1003), (2, 1, 1537)])
 def test_forward_multilayer(monkeypatch, kappa, batch, n):
     """Multi-layer stages (kappa k-tiles per stage): the single-stream and
     side-band schedules give the same bits, and D matches the oracle's fp32

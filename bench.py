@@ -370,6 +370,35 @@ def main():
              "verdict": "every row exact" if failed == 0 else "MISMATCH"}
     del srcv, okv
 
+    # the same rest kernel with the CUs to itself: two profiled steps of the
+    # single-stream schedule (outside the timed region)
+    alone = None
+    if a.algo == "multi_layer_tiling_gpu":
+        prev = os.environ.get("FWX_LOOKAHEAD")
+        os.environ["FWX_LOOKAHEAD"] = "0"
+        try:
+            step()
+            torch.cuda.synchronize()
+            _lib.check(L.fwx_profile_begin())
+            for _ in range(2):
+                step()
+            torch.cuda.synchronize()
+            ms0 = (ctypes.c_double * _lib.PROF_NCLASS)()
+            wk0 = (ctypes.c_double * _lib.PROF_NCLASS)()
+            grp0 = (ctypes.c_int64 * _lib.PROF_NCLASS)()
+            _lib.check(L.fwx_profile_end(ms0, wk0, grp0))
+        finally:
+            if prev is None:
+                os.environ.pop("FWX_LOOKAHEAD", None)
+            else:
+                os.environ["FWX_LOOKAHEAD"] = prev
+        r_ms, r_wk, r_n = ms0[_lib.PROF_REST], wk0[_lib.PROF_REST], grp0[_lib.PROF_REST]
+        if r_ms > 0:
+            ach = r_wk / (r_ms * 1e-3)
+            alone = {"schedule": "single stream (FWX_LOOKAHEAD=0), 2 steps outside the timed region",
+                     "launches": int(r_n), "avg_launch_ms": r_ms / max(1, r_n), "achieved": ach,
+                     "frac": ach / PEAK_2OP}
+
     compare = {}
     if not a.no_compare:
         for name, algo, t, k in (("tiling_gpu_refshape", "tiling_gpu_refshape", 0, 0),
@@ -407,7 +436,8 @@ def main():
                      "work_per_step": rest_work / a.steps,
                      "share_of_step": rest_ms / a.steps / ms_per_step,
                      "lead_ms_per_step": lead_ms / a.steps,
-                     "band_ms_per_step": band_ms / a.steps},
+                     "band_ms_per_step": band_ms / a.steps,
+                     "kernel_alone": alone},
         "parity": check,
         "comparators": compare,
     }

@@ -524,7 +524,16 @@ static int run_blocked_sideband(int32_t* d, int32_t n, int32_t kappa, hipStream_
 
 // One non-blocking side stream per device (default priority), created on
 // first use and kept for the life of the process.
+// While this thread captures a solve into a graph, its side stream is a
+// private one (t_capture_side), so that no other thread's work on the shared
+// side stream can enter, or be refused by, the capture.
+static thread_local hipStream_t t_capture_side = nullptr;
+
 int side_stream(hipStream_t* out) {
+  if (t_capture_side) {
+    *out = t_capture_side;
+    return FWX_OK;
+  }
   static std::mutex mu;
   static hipStream_t streams[64] = {};
   int dev = 0;
@@ -654,15 +663,24 @@ int enqueue_blocked(int32_t* d, int32_t n, int32_t tile, int32_t kappa, hipStrea
   };
   for (GraphEntry& e : g_graphs)
     if (e.key == key) return replay(e);
-  // capture on a private stream: `st` may be the legacy default stream
-  hipStream_t cap;
+  // capture on a private stream (`st` may be the legacy default stream),
+  // with a private side stream for the two-stream schedules
+  hipStream_t cap, cap_side;
   FWX_HIP(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
+  const hipError_t es = hipStreamCreateWithFlags(&cap_side, hipStreamNonBlocking);
+  if (es != hipSuccess) {
+    (void)hipStreamDestroy(cap);
+    return hip_fail(es, "hipStreamCreateWithFlags", __FILE__, __LINE__);
+  }
   hipError_t e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
   int rc = FWX_OK;
+  t_capture_side = cap_side;
   if (e == hipSuccess) rc = enqueue_direct(d, n, tile, kappa, vec, pk, cap);
+  t_capture_side = nullptr;
   hipGraph_t graph = nullptr;
   const hipError_t e2 = hipStreamEndCapture(cap, &graph);
   (void)hipStreamDestroy(cap);
+  (void)hipStreamDestroy(cap_side);
   if (e != hipSuccess) return hip_fail(e, "hipStreamBeginCapture", __FILE__, __LINE__);
   if (rc || e2 != hipSuccess) {
     if (graph) (void)hipGraphDestroy(graph);

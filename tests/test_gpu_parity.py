@@ -297,6 +297,44 @@ def test_config3_n32768_spot_rows():
     assert ok.all(), rows[~ok][:10]
 
 
+def test_concurrent_solves_on_two_streams():
+    """Two host threads solving at once on their own streams: one captures and
+    replays graphs (n = 1000, a new buffer each time), the other launches
+    directly (n = 4500) through the shared side stream.  Every result equals
+    the oracle."""
+    import threading
+    import torch
+    from fw_cuda_amd import _lib
+    ga = fx.generate_graph(1000, 0.4, 91, 1, 1000)
+    gb = fx.generate_graph(4500, 0.2, 92, 1, 1000)
+    ea, eb = po.fw_rowpar(ga), po.fw_rowpar(gb)
+    errors, results = [], {}
+
+    def worker(name, g, reps):
+        try:
+            s = torch.cuda.Stream()
+            out = []
+            for _ in range(reps):
+                t = torch.from_numpy(g).cuda()
+                torch.cuda.synchronize()
+                fx.run_device("multi_layer_tiling_gpu", _lib.ptr(t), g.shape[0], 0, 0, s.cuda_stream)
+                out.append(t.cpu().numpy())
+            results[name] = out
+        except Exception as e:  # reported below
+            errors.append((name, repr(e)))
+
+    th = [threading.Thread(target=worker, args=("graphs", ga, 6)),
+          threading.Thread(target=worker, args=("direct", gb, 3))]
+    say("concurrent solves")
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    assert not errors, errors
+    assert all((r == ea).all() for r in results["graphs"])
+    assert all((r == eb).all() for r in results["direct"])
+
+
 def test_max_n_32769():
     """n = MAX_N = 32769 (include/constants.h:5; 4.3 GB int32, ragged: no
     16-byte path, edge tiles everywhere): multi_layer_tiling_gpu certified

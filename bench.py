@@ -45,6 +45,7 @@ def parse():
     ap.add_argument("--no-diff", action="store_true")
     ap.add_argument("--no-pipeline", action="store_true")
     ap.add_argument("--no-config3", action="store_true")
+    ap.add_argument("--no-config1", action="store_true")
     return ap.parse_args()
 
 
@@ -145,6 +146,54 @@ def bench_pipeline(n):
         dt = time.perf_counter() - t0
         out["predecessors_s"] = dt
         out["predecessors_n3_per_s"] = float(n) ** 3 / dt
+    return out
+
+
+def bench_config1(reps=20):
+    """n=1024, density 0.9, weights U[1,1000], seed 42 (the reference-compatible
+    host generator), for each GPU algorithm: per-call time with HIP events on
+    the launch stream, mean and min of `reps` calls each on a fresh device copy
+    of the input (a call returns after its stream is idle, as the reference's
+    entry points do); the reference's protocol (src/timer.cu:43-55: one warmup
+    call, then one event-timed call on the already-solved matrix); and every
+    row of the result certified on the device against the input."""
+    import torch
+    import fw_cuda_amd as fx
+    from fw_cuda_amd import _lib
+    n = 1024
+    dev = torch.device("cuda:0")
+    src = torch.from_numpy(fx.generate_graph(n, 0.9, 42, 1, 1000)).to(dev)
+    work = torch.empty_like(src)
+    st = torch.cuda.current_stream(dev)
+    sp = st.cuda_stream
+    srcv = torch.arange(n, dtype=torch.int32, device=dev)
+    okv = torch.zeros(n, dtype=torch.int32, device=dev)
+    out = {"n": n}
+    for algo in ("multi_layer_tiling_gpu", "tiling_gpu", "baseline_gpu"):
+        work.copy_(src)
+        fx.run_device(algo, _lib.ptr(work), n, 0, 0, sp)  # first call: plan and graph capture
+        ms = []
+        for _ in range(reps):
+            work.copy_(src)
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(st)
+            fx.run_device(algo, _lib.ptr(work), n, 0, 0, sp)
+            e1.record(st)
+            e1.synchronize()
+            ms.append(e0.elapsed_time(e1))
+        failed = fx.certify_rows_device(_lib.ptr(src), n, _lib.ptr(srcv), n, _lib.ptr(work), _lib.ptr(okv), sp)
+        work.copy_(src)
+        fx.run_device(algo, _lib.ptr(work), n, 0, 0, sp)  # the protocol's warmup call solves the matrix
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(st)
+        fx.run_device(algo, _lib.ptr(work), n, 0, 0, sp)
+        e1.record(st)
+        e1.synchronize()
+        mean = sum(ms) / len(ms)
+        out[algo] = {"ms_mean": mean, "ms_min": min(ms), "n3_per_s": float(n) ** 3 / (mean * 1e-3),
+                     "reference_protocol_ms": e0.elapsed_time(e1), "rows_certified": n, "rows_failed": failed}
+    tile, kappa = fx.tuned_config(n)
+    out["multi_layer_tiling_gpu"].update({"tile": tile, "kappa": kappa})
     return out
 
 
@@ -253,7 +302,8 @@ def cpu_baseline(n, n_warm):
     own sources) or, if that is absent, the oracle's restatement, on one core:
     one cold call over an n x n sample of the workload (the reported value),
     plus the reference's own protocol (src/timer.cu:16-33: one warmup call,
-    then one timed call on the already-solved matrix) at n_warm."""
+    then one timed call on the already-solved matrix) at n_warm, and both at
+    n=1024 (the n=1024 workload's CPU figure)."""
     import numpy as np
     import fw_cuda_amd as fx
     from oracle import pyoracle as po
@@ -268,6 +318,8 @@ def cpu_baseline(n, n_warm):
     dt, _ = once(fx.generate_graph(n, 0.9, 42, 1, 1000))
     _, solved = once(fx.generate_graph(n_warm, 0.9, 42, 1, 1000))
     dt_warm, _ = once(solved)
+    dt1, solved1 = once(fx.generate_graph(1024, 0.9, 42, 1, 1000))
+    dt1_warm, _ = once(solved1)
     cpu = "unknown"
     try:
         for line in open("/proc/cpuinfo"):
@@ -281,7 +333,9 @@ def cpu_baseline(n, n_warm):
                       f"{dt:.2f} s on 1 thread of {cpu} (nproc={os.cpu_count()})",
             "reference_protocol": {"n": n_warm, "seconds": dt_warm,
                                    "value": float(n_warm) ** 3 / dt_warm,
-                                   "note": "timed call on the matrix solved by the warmup call"}}
+                                   "note": "timed call on the matrix solved by the warmup call"},
+            "config1": {"n": 1024, "cold_seconds": dt1, "reference_protocol_seconds": dt1_warm,
+                        "value": 1024.0 ** 3 / dt1}}
 
 
 def traffic_from_profiles(n, tile, kappa):
@@ -447,6 +501,8 @@ def main():
         out["differentiable"] = bench_diff(3)
     if not a.no_pipeline:
         out["input_pipeline"] = bench_pipeline(n)
+    if not a.no_config1:
+        out["config1"] = bench_config1()
     if not a.no_config3:
         out["config3"] = bench_config3()
     if not a.no_cpu:

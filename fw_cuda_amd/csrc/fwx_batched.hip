@@ -247,6 +247,7 @@ struct Launch {
   int kind;
   int64_t first;  // index into the kind's item array
   int32_t count;
+  bool side;      // on the side stream, beside the step's other launches
 };
 
 // (tile, kappa) of one graph inside a batch.  Graphs with n >= 128 take
@@ -343,6 +344,57 @@ void schedule(int64_t base, int32_t n, int32_t T, int32_t kappa, std::vector<std
   }
 }
 
+using ProdSteps = std::vector<std::vector<std::pair<int, ProdItem>>>;
+using DiagSteps = std::vector<std::vector<std::pair<int, DiagItem>>>;
+
+int prod_block(int kind) { return kind == kProd32 ? 32 : kind == kProd64 ? 64 : 128; }
+
+// Side branches of one graph's schedule.  A closure step that follows a
+// product step needs, from that step, only the item whose C block holds the
+// diagonal tile (the next lead round's tile in a cross step, or the next
+// stage's first diagonal tile in a rest step).  That item and the closure
+// move into a side branch of the product step, and the closure step goes:
+// the closure then runs beside the step's other products instead of after
+// them.  The side item's C block is outside the row and column panels the
+// other items read, and its operands are panels no item of the step writes,
+// so the two branches touch disjoint cells; the next step waits for both.
+// Writes the side items (sps) and side closures (sds) per step.
+void hoist_closures(ProdSteps& ps, DiagSteps& ds, ProdSteps& sps, DiagSteps& sds, int T) {
+  sps.assign(ps.size(), {});
+  sds.assign(ps.size(), {});
+  size_t j = 1;
+  while (j < ps.size()) {
+    bool moved = false;
+    if (ds[j].size() == 1 && ps[j].empty() && sds[j - 1].empty() && !ps[j - 1].empty()) {
+      const int c0 = ds[j][0].second.c0;
+      auto& prev = ps[j - 1];
+      for (size_t i = 0; i < prev.size(); ++i) {
+        const ProdItem& it = prev[i].second;
+        const int RR = prod_block(prev[i].first);
+        if (it.cr <= c0 && c0 + T <= it.cr + RR && it.cc <= c0 && c0 + T <= it.cc + RR) {
+          sps[j - 1].push_back(prev[i]);
+          prev.erase(prev.begin() + (std::ptrdiff_t)i);
+          sds[j - 1].push_back(ds[j][0]);
+          ps.erase(ps.begin() + (std::ptrdiff_t)j);
+          ds.erase(ds.begin() + (std::ptrdiff_t)j);
+          sps.erase(sps.begin() + (std::ptrdiff_t)j);
+          sds.erase(sds.begin() + (std::ptrdiff_t)j);
+          moved = true;
+          break;
+        }
+      }
+    }
+    if (!moved) ++j;
+  }
+}
+
+// FWX_BATCH_HOIST=0 keeps every closure in a step of its own (read at plan
+// creation).
+bool hoist_enabled() {
+  const char* e = std::getenv("FWX_BATCH_HOIST");
+  return !(e && std::strcmp(e, "0") == 0);
+}
+
 }  // namespace
 
 }  // namespace fwx
@@ -350,7 +402,9 @@ void schedule(int64_t base, int32_t n, int32_t T, int32_t kappa, std::vector<std
 struct fwx_batch_plan {
   int32_t* data = nullptr;
   hipStream_t stream = nullptr;
-  std::vector<fwx::Launch> launches;  // in order; a step's launches are contiguous
+  hipStream_t side = nullptr;          // side branches (hoisted closures)
+  hipEvent_t fork = nullptr, join = nullptr;
+  std::vector<fwx::Launch> launches;  // in order; a step's launches are contiguous, side ones first
   std::vector<int32_t> step_of;       // step index of each launch
   fwx::ProdItem* d_prod[fwx::kKinds] = {};
   fwx::DiagItem* d_diag[fwx::kKinds] = {};
@@ -377,24 +431,50 @@ struct fwx_batch_plan {
 
 namespace fwx {
 
+// Step by step on p->stream; a step's side launches run on p->side, forked
+// from p->stream at the step's start and joined back before the next step.
 template <bool PK>
 static int enqueue_plan(fwx_batch_plan* p) {
-  for (const Launch& L : p->launches) {
+  int cur = -1;
+  bool open = false;
+  for (size_t li = 0; li < p->launches.size(); ++li) {
+    const Launch& L = p->launches[li];
+    if (p->step_of[li] != cur) {
+      if (open) {
+        FWX_HIP(hipEventRecord(p->join, p->side));
+        FWX_HIP(hipStreamWaitEvent(p->stream, p->join, 0));
+        open = false;
+      }
+      cur = p->step_of[li];
+    }
+    hipStream_t st = p->stream;
+    if (L.side) {
+      if (!open) {
+        FWX_HIP(hipEventRecord(p->fork, p->stream));
+        FWX_HIP(hipStreamWaitEvent(p->side, p->fork, 0));
+        open = true;
+      }
+      st = p->side;
+    }
     switch (L.kind) {
-      case kDiag32: k_batch_diag<32, 256><<<L.count, 256, 0, p->stream>>>(p->data, p->d_diag[kDiag32] + L.first); break;
+      case kDiag32: k_batch_diag<32, 256><<<L.count, 256, 0, st>>>(p->data, p->d_diag[kDiag32] + L.first); break;
       case kDiag64:
         if (batch_diag_wide())
-          k_batch_diag<64, 512><<<L.count, 512, 0, p->stream>>>(p->data, p->d_diag[kDiag64] + L.first);
+          k_batch_diag<64, 512><<<L.count, 512, 0, st>>>(p->data, p->d_diag[kDiag64] + L.first);
         else
-          k_batch_diag<64, 256><<<L.count, 256, 0, p->stream>>>(p->data, p->d_diag[kDiag64] + L.first);
+          k_batch_diag<64, 256><<<L.count, 256, 0, st>>>(p->data, p->d_diag[kDiag64] + L.first);
         break;
-      case kDiag128: k_batch_diag<128, 512><<<L.count, 512, 0, p->stream>>>(p->data, p->d_diag[kDiag128] + L.first); break;
-      case kProd32: k_batch_prod<32, PK><<<L.count, 256, 0, p->stream>>>(p->data, p->d_prod[kProd32] + L.first); break;
-      case kProd64: k_batch_prod<64, PK><<<L.count, 256, 0, p->stream>>>(p->data, p->d_prod[kProd64] + L.first); break;
-      case kProd128: k_batch_prod512<PK><<<L.count, 512, 0, p->stream>>>(p->data, p->d_prod[kProd128] + L.first); break;
+      case kDiag128: k_batch_diag<128, 512><<<L.count, 512, 0, st>>>(p->data, p->d_diag[kDiag128] + L.first); break;
+      case kProd32: k_batch_prod<32, PK><<<L.count, 256, 0, st>>>(p->data, p->d_prod[kProd32] + L.first); break;
+      case kProd64: k_batch_prod<64, PK><<<L.count, 256, 0, st>>>(p->data, p->d_prod[kProd64] + L.first); break;
+      case kProd128: k_batch_prod512<PK><<<L.count, 512, 0, st>>>(p->data, p->d_prod[kProd128] + L.first); break;
       default: return fail(FWX_ERR_INVALID_ARG, "corrupt batch plan");
     }
     FWX_LAUNCHED("k_batch");
+  }
+  if (open) {
+    FWX_HIP(hipEventRecord(p->join, p->side));
+    FWX_HIP(hipStreamWaitEvent(p->stream, p->join, 0));
   }
   return FWX_OK;
 }
@@ -529,6 +609,9 @@ static int build_queues(fwx_batch_plan* p, const std::vector<int32_t>& nsz, cons
 
 static void destroy_plan(fwx_batch_plan* p) {
   if (!p) return;
+  if (p->fork) (void)hipEventDestroy(p->fork);
+  if (p->join) (void)hipEventDestroy(p->join);
+  if (p->side) (void)hipStreamDestroy(p->side);
   for (int m = 0; m < 2; ++m) {
     if (p->exec[m]) (void)hipGraphExecDestroy(p->exec[m]);
     if (p->graph[m]) (void)hipGraphDestroy(p->graph[m]);
@@ -573,14 +656,21 @@ int fwx_batch_plan_create(fwx_batch_plan** out, const int32_t* sizes, const int6
   for (int32_t g = 0; g < count; ++g) order[g] = g;
   std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) { return sizes[a] > sizes[b]; });
 
-  std::vector<std::vector<std::vector<std::pair<int, ProdItem>>>> P(count);
-  std::vector<std::vector<std::vector<std::pair<int, DiagItem>>>> Dg(count);
+  std::vector<ProdSteps> P(count), SP(count);
+  std::vector<DiagSteps> Dg(count), SD(count);
+  const bool hoist = hoist_enabled();
   int32_t steps = 0;
   for (int32_t q = 0; q < count; ++q) {
     const int32_t g = order[q];
     int32_t T, kappa;
     batch_config(sizes[g], &T, &kappa);
     schedule(offsets[g], sizes[g], T, kappa, P[q], Dg[q]);
+    if (hoist) {
+      hoist_closures(P[q], Dg[q], SP[q], SD[q], T);
+    } else {
+      SP[q].assign(P[q].size(), {});
+      SD[q].assign(P[q].size(), {});
+    }
     steps = std::max<int32_t>(steps, (int32_t)P[q].size());
   }
 
@@ -590,27 +680,45 @@ int fwx_batch_plan_create(fwx_batch_plan** out, const int32_t* sizes, const int6
     p->span = std::max<int64_t>(p->span, offsets[g] + (int64_t)sizes[g] * sizes[g]);
   p->stream = as_stream(stream);
   p->steps = steps;
+  {
+    hipError_t e = hipStreamCreateWithFlags(&p->side, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&p->fork, event_flags(false));
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&p->join, event_flags(false));
+    if (e != hipSuccess) {
+      destroy_plan(p);
+      return hip_fail(e, "batch plan streams", __FILE__, __LINE__);
+    }
+  }
   std::vector<ProdItem> prod[kKinds];
   std::vector<DiagItem> diag[kKinds];
+  auto add_prod = [&](int k, const std::vector<ProdItem>& v, int32_t j, bool side) {
+    if (v.empty()) return;
+    p->launches.push_back(Launch{k, (int64_t)prod[k].size(), (int32_t)v.size(), side});
+    p->step_of.push_back(j);
+    prod[k].insert(prod[k].end(), v.begin(), v.end());
+  };
+  auto add_diag = [&](int k, const std::vector<DiagItem>& v, int32_t j, bool side) {
+    if (v.empty()) return;
+    p->launches.push_back(Launch{k, (int64_t)diag[k].size(), (int32_t)v.size(), side});
+    p->step_of.push_back(j);
+    diag[k].insert(diag[k].end(), v.begin(), v.end());
+  };
   for (int32_t j = 0; j < steps; ++j) {
-    std::vector<ProdItem> sp[kKinds];
-    std::vector<DiagItem> sd[kKinds];
+    std::vector<ProdItem> sp[kKinds], ssp[kKinds];
+    std::vector<DiagItem> sd[kKinds], ssd[kKinds];
     for (int32_t q = 0; q < count; ++q) {
       if (j >= (int32_t)P[q].size()) continue;
       for (auto& kv : P[q][j]) sp[kv.first].push_back(kv.second);
       for (auto& kv : Dg[q][j]) sd[kv.first].push_back(kv.second);
+      for (auto& kv : SP[q][j]) ssp[kv.first].push_back(kv.second);
+      for (auto& kv : SD[q][j]) ssd[kv.first].push_back(kv.second);
     }
+    // side branch first (its products, then its closures), then the rest
+    for (int k = 0; k < kKinds; ++k) add_prod(k, ssp[k], j, true);
+    for (int k = 0; k < kKinds; ++k) add_diag(k, ssd[k], j, true);
     for (int k = 0; k < kKinds; ++k) {
-      if (!sd[k].empty()) {
-        p->launches.push_back(Launch{k, (int64_t)diag[k].size(), (int32_t)sd[k].size()});
-        p->step_of.push_back(j);
-        diag[k].insert(diag[k].end(), sd[k].begin(), sd[k].end());
-      }
-      if (!sp[k].empty()) {
-        p->launches.push_back(Launch{k, (int64_t)prod[k].size(), (int32_t)sp[k].size()});
-        p->step_of.push_back(j);
-        prod[k].insert(prod[k].end(), sp[k].begin(), sp[k].end());
-      }
+      add_diag(k, sd[k], j, false);
+      add_prod(k, sp[k], j, false);
     }
   }
   for (int k = 0; k < kKinds; ++k) {

@@ -55,15 +55,19 @@ def check_all(got, expect, flat_after, flat_before, sizes, offs):
     assert (flat_after[mask] == flat_before[mask]).all()
 
 
-@pytest.mark.parametrize("deep", ["default", "off", "64,64,4", "100,128,3", "off/diag256"])
+@pytest.mark.parametrize("deep", ["default", "default/nohoist", "off", "off/nohoist", "64,64,4", "100,128,3",
+                                  "off/diag256"])
 def test_batch_small_mixed(monkeypatch, deep):
     """Mixed sizes (negative weights in some graphs) under the default batch
     configuration rule, the plain path's tuned entries ("off", also with
-    256-thread 64-wide closures), and two other per-size overrides: every
-    graph equals the oracle."""
-    if deep.endswith("/diag256"):
+    256-thread 64-wide closures), and two other per-size overrides, with the
+    closures on side branches (default) and in steps of their own
+    ("nohoist"): every graph equals the oracle."""
+    deep, *flags = deep.split("/")
+    if "diag256" in flags:
         monkeypatch.setenv("FWX_BATCH_DIAG", "256")
-        deep = deep.split("/")[0]
+    if "nohoist" in flags:
+        monkeypatch.setenv("FWX_BATCH_HOIST", "0")
     if deep != "default":
         monkeypatch.setenv("FWX_BATCH_DEEP", deep)
     rng = np.random.default_rng(3)

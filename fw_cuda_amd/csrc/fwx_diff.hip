@@ -777,7 +777,9 @@ __global__ __launch_bounds__(256) void k_ffill(float* __restrict__ x, int64_t to
 //      anc = anc[anc]);
 //   2. vertices bucketed by depth (counting sort in LDS);
 //   3. from the deepest level up: each vertex adds g[t] to its parent's g
-//      (LDS float atomic) and to dW[P[s][t]][t] (global atomic).
+//      (LDS float atomic); then the source's final g row goes to gout for
+//      k_fcolumns, or, with gout null, each vertex adds g[t] to dW[P[s][t]][t]
+//      with a global atomic as its level is done.
 // Work O(n log n) per source with (log n + height) barriers, instead of a
 // sort of all n keys followed by one lane walking n vertices.
 // Dynamic LDS, 14 bytes per vertex + 8: par, dep, aux (int16: vertex ids
@@ -786,7 +788,8 @@ __global__ __launch_bounds__(256) void k_ffill(float* __restrict__ x, int64_t to
 template <int NT, int PER>
 __global__ __launch_bounds__(NT) void k_fback(const float* __restrict__ D, const int32_t* __restrict__ Pr,
                                                         int64_t pstride, const float* __restrict__ G,
-                                                        float* __restrict__ dW, int32_t n, int32_t s0) {
+                                                        float* dW, int32_t n, int32_t s0, float* gout,
+                                                        int64_t gstride) {
   extern __shared__ __align__(16) unsigned char smem[];
   float* g = reinterpret_cast<float*>(smem);
   int32_t* off = reinterpret_cast<int32_t*>(g + n);
@@ -890,10 +893,112 @@ __global__ __launch_bounds__(NT) void k_fback(const float* __restrict__ D, const
       const int p = par[t];
       const float gt = g[t];
       atomicAdd(&g[p], gt);
-      atomicAdd(&dWg[(int64_t)p * n + t], gt);
+      if (!gout) atomicAdd(&dWg[(int64_t)p * n + t], gt);
     }
     __syncthreads();
   }
+  // column form: the final g of every vertex the levels above processed, 0
+  // for the others (the source, unreachable vertices), one coalesced row per
+  // source; k_fcolumns adds them into dW by predecessor.  gout may be dW
+  // itself: no other block of this launch reads or writes row lr of it.
+  if (gout) {
+    float* row = gout + gi * gstride + (int64_t)lr * n;
+    for (int t = tid; t < n; t += NT) {
+      const int d = dep[t];
+      row[t] = d > 0 && d <= n ? g[t] : 0.0f;
+    }
+  }
+}
+
+// dW[u][t] = sum of g_s[t] over the sources s of one chunk with P[s][t] = u
+// (plus what earlier chunks left in dW unless `first`).  One workgroup per
+// (CT target columns, graph) holds the whole column tile, n x CT floats, in
+// LDS: it adds every source's g_s[t] of its columns there, at its
+// predecessor, with an LDS float atomic, then writes each dW cell of the tile
+// once.  Many sources reach a target through the same last edge, so the
+// per-source global atomics on those cells were the backward's bottleneck;
+// here the global traffic is one coalesced read of P and g and one write of
+// dW.  Gb may alias dW (one chunk holding every source, first = 1): a
+// workgroup reads only its own column tile of Gb, all of it before the
+// barrier, and writes only that tile of dW after it.
+template <int CT>
+__global__ __launch_bounds__(1024) void k_fcolumns(const int32_t* __restrict__ P, int64_t pstride, const float* Gb,
+                                                   int64_t gstride, int32_t rows, float* dW, int32_t n,
+                                                   int32_t first) {
+  constexpr int NT = 1024, U = 8;  // NT is a multiple of CT: a thread keeps one column
+  extern __shared__ __align__(16) unsigned char smem[];
+  float* acc = reinterpret_cast<float*>(smem);  // [u][c]
+  const int c = threadIdx.x % CT, t = blockIdx.x * CT + c;
+  const int64_t gi = blockIdx.y;
+  const int64_t cells = (int64_t)n * CT;
+  for (int64_t e = threadIdx.x; e < cells; e += NT) acc[e] = 0.0f;
+  __syncthreads();
+  const int32_t* Pg = P + gi * pstride;
+  const float* Gg = Gb + gi * gstride;
+  const int64_t items = (int64_t)rows * CT;
+  // U loads in flight per thread before their LDS atomics
+  for (int64_t e0 = threadIdx.x; e0 < items; e0 += (int64_t)NT * U) {
+    int32_t p[U];
+    float v[U];
+#pragma unroll
+    for (int k = 0; k < U; ++k) {
+      const int64_t e = e0 + (int64_t)k * NT;
+      const bool in = e < items && t < n;
+      const int64_t at = (e / CT) * n + t;
+      p[k] = in ? Pg[at] : -1;
+      v[k] = in ? Gg[at] : 0.0f;
+    }
+#pragma unroll
+    for (int k = 0; k < U; ++k)
+      if (p[k] >= 0 && p[k] < n) atomicAdd(&acc[p[k] * CT + c], v[k]);
+  }
+  __syncthreads();
+  if (t >= n) return;
+  float* dWg = dW + gi * (int64_t)n * n;
+  for (int64_t e = threadIdx.x; e < cells; e += NT) {
+    float* o = dWg + (e / CT) * n + t;
+    *o = first ? acc[e] : *o + acc[e];
+  }
+}
+
+// Columns per workgroup: the largest power of two in [4, 64] with n x CT
+// floats within 128 KiB of LDS.
+static int fcolumns_ct(int32_t n) {
+  int ct = 64;
+  while (ct > 4 && (int64_t)n * ct * 4 > (128 << 10)) ct >>= 1;
+  return ct;
+}
+
+template <int CT>
+static hipError_t fcolumns_launch(const int32_t* P, int64_t pstride, const float* Gb, int64_t gstride, int32_t rows,
+                                  float* dW, int32_t n, int32_t batch, bool first, hipStream_t st) {
+  const size_t lds = (size_t)n * CT * sizeof(float);
+  if (lds > 65536) {
+    const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fcolumns<CT>),
+                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+  }
+  k_fcolumns<CT><<<dim3((n + CT - 1) / CT, batch), 1024, lds, st>>>(P, pstride, Gb, gstride, rows, dW, n,
+                                                                     first ? 1 : 0);
+  return hipGetLastError();
+}
+
+static hipError_t fcolumns(const int32_t* P, int64_t pstride, const float* Gb, int64_t gstride, int32_t rows, float* dW,
+                           int32_t n, int32_t batch, bool first, hipStream_t st) {
+  switch (fcolumns_ct(n)) {
+    case 64: return fcolumns_launch<64>(P, pstride, Gb, gstride, rows, dW, n, batch, first, st);
+    case 32: return fcolumns_launch<32>(P, pstride, Gb, gstride, rows, dW, n, batch, first, st);
+    case 16: return fcolumns_launch<16>(P, pstride, Gb, gstride, rows, dW, n, batch, first, st);
+    case 8: return fcolumns_launch<8>(P, pstride, Gb, gstride, rows, dW, n, batch, first, st);
+    default: return fcolumns_launch<4>(P, pstride, Gb, gstride, rows, dW, n, batch, first, st);
+  }
+}
+
+// FWX_DIFF_BACK=0: each source adds its contributions to dW with one global
+// atomic per vertex (read on every call).
+static bool diff_back_columns() {
+  const char* e = std::getenv("FWX_DIFF_BACK");
+  return !(e && e[0] == '0');
 }
 
 // Product kernels: FWX_DIFF_REST=0 selects the first versions (k_frest,
@@ -1212,47 +1317,64 @@ int fwx_diff_backward(const float* d_W, const float* d_D, const int32_t* d_P, co
   if (!d_dW) return fail(FWX_ERR_INVALID_ARG, "null gradient output");
   if (!d_P && !d_W) return fail(FWX_ERR_INVALID_ARG, "recomputing predecessors needs W");
   hipStream_t st = as_stream(stream);
-  const int64_t total = (int64_t)n * n * batch;
-  k_ffill<<<(int)std::min<int64_t>((total + 255) / 256, 65536), 256, 0, st>>>(d_dW, total);
-  FWX_LAUNCHED("k_ffill");
   const size_t lds = (size_t)n * 14 + 8;
+  // Column form (default): k_fback writes each source's g row, k_fcolumns
+  // adds the rows into dW by predecessor.  With stored predecessors the g
+  // rows go into dW itself (one chunk of every source); recomputed
+  // predecessors come in chunks of 256 sources with a g buffer beside them.
+  // FWX_DIFF_BACK=0 zeroes dW and adds to it with global atomics.
+  const bool columns = diff_back_columns();
+  if (!columns) {
+    const int64_t total = (int64_t)n * n * batch;
+    k_ffill<<<(int)std::min<int64_t>((total + 255) / 256, 65536), 256, 0, st>>>(d_dW, total);
+    FWX_LAUNCHED("k_ffill");
+  }
   // one workgroup per source: 256 threads up to n = 1024, 512 up to 4096,
   // 1024 above (each thread holds at most PER vertices in the jumping rounds)
-  auto back = [&](const int32_t* pr, int64_t pstride, int32_t rows, int32_t s0) -> hipError_t {
+  auto back = [&](const int32_t* pr, int64_t pstride, int32_t rows, int32_t s0, float* gb,
+                  int64_t gstride) -> hipError_t {
     const dim3 grid(rows, batch);
     if (n <= 1024) {
-      k_fback<256, 4><<<grid, 256, lds, st>>>(d_D, pr, pstride, d_G, d_dW, n, s0);
+      k_fback<256, 4><<<grid, 256, lds, st>>>(d_D, pr, pstride, d_G, d_dW, n, s0, gb, gstride);
     } else if (n <= 4096) {
-      k_fback<512, 8><<<grid, 512, lds, st>>>(d_D, pr, pstride, d_G, d_dW, n, s0);
+      k_fback<512, 8><<<grid, 512, lds, st>>>(d_D, pr, pstride, d_G, d_dW, n, s0, gb, gstride);
     } else {
       if (lds > 65536) {
         const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fback<1024, 8>),
                                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
       }
-      k_fback<1024, 8><<<grid, 1024, lds, st>>>(d_D, pr, pstride, d_G, d_dW, n, s0);
+      k_fback<1024, 8><<<grid, 1024, lds, st>>>(d_D, pr, pstride, d_G, d_dW, n, s0, gb, gstride);
     }
-    return hipGetLastError();
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess || !gb) return e;
+    return fcolumns(pr, pstride, gb, gstride, rows, d_dW, n, batch, s0 == 0, st);
   };
+  int rc = FWX_OK;
   if (d_P) {
-    FWX_HIP(back(d_P, (int64_t)n * n, n, 0));
+    const int64_t nn = (int64_t)n * n;
+    const hipError_t e = back(d_P, nn, n, 0, columns ? d_dW : nullptr, nn);
+    if (e != hipSuccess) rc = hip_fail(e, "k_fback", __FILE__, __LINE__);
   } else {
-    // source blocks of 256 rows; workspace 256 x n x batch int32
+    // source blocks of 256 rows; workspace 256 x n x batch int32 (and as
+    // many floats of g rows in the column form)
     const int32_t rows = std::min<int32_t>(256, n);
+    const size_t cnt = (size_t)rows * n * batch;
     int32_t* ws = nullptr;
-    FWX_HIP(hipMallocAsync(reinterpret_cast<void**>(&ws), (size_t)rows * n * batch * sizeof(int32_t), st));
-    int rc = FWX_OK;
+    const hipError_t ea = hipMallocAsync(reinterpret_cast<void**>(&ws), cnt * (columns ? 8 : 4), st);
+    if (ea != hipSuccess) rc = hip_fail(ea, "hipMallocAsync", __FILE__, __LINE__);
+    float* gb = columns && ws ? reinterpret_cast<float*>(ws + cnt) : nullptr;
     for (int32_t s0 = 0; s0 < n && rc == FWX_OK; s0 += rows) {
       const int32_t r = std::min(rows, n - s0);
       const int nt = (n + FT - 1) / FT;
       k_fpred<<<dim3(((r + FT - 1) / FT) * nt, batch), 256, 0, st>>>(d_W, d_D, ws, n, s0, r);
       hipError_t e = hipGetLastError();
-      if (e == hipSuccess) e = back(ws, (int64_t)r * n, r, s0);
+      if (e == hipSuccess) e = back(ws, (int64_t)r * n, r, s0, gb, (int64_t)r * n);
       if (e != hipSuccess) rc = hip_fail(e, "k_fpred/k_fback", __FILE__, __LINE__);
     }
-    (void)hipFreeAsync(ws, st);
-    if (rc) return rc;
+    if (ws) (void)hipFreeAsync(ws, st);
   }
+  if (rc) return rc;
   FWX_HIP(hipStreamSynchronize(st));
   return FWX_OK;
 }

@@ -182,6 +182,26 @@ def test_backward_recomputed_predecessors_match_stored():
     assert np.allclose(g1, g2, rtol=1e-5, atol=1e-4)
 
 
+@pytest.mark.parametrize("batch,n,keep", [(3, 129, True), (2, 700, True), (1100, 200, True), (1100, 200, False),
+                                          (3, 300, False), (2, 700, False)])
+def test_backward_column_form_matches_atomics(monkeypatch, batch, n, keep):
+    """dW from each source's g row summed per target column in LDS (default)
+    against per-vertex global atomics (FWX_DIFF_BACK=0): equal up to fp32
+    summation order.  Stored predecessors put the g rows in dW itself;
+    recomputed ones come in chunks of 256 sources (n = 300 and 700: a first
+    chunk that writes dW, then chunks that add to it)."""
+    W, G = graphs(batch, n, 0.3, batch + n)
+    monkeypatch.setenv("FWX_DIFF_BACK", "0")
+    _, g0 = gpu_fwd_bwd(W, G, keep_pred=keep)
+    monkeypatch.delenv("FWX_DIFF_BACK")
+    _, g1 = gpu_fwd_bwd(W, G, keep_pred=keep)
+    say("column form", batch, n, keep, float(np.abs(g1 - g0).max()))
+    assert (g1[~np.isfinite(W)] == 0).all()
+    assert np.allclose(g1, g0, rtol=1e-5, atol=1e-4)
+    err = float(np.linalg.norm(g1 - g0) / np.linalg.norm(g0))
+    assert err < 1e-6, err
+
+
 def _fd(W64, G64, u, v, eps):
     Wp, Wm = W64.copy(), W64.copy()
     Wp[u, v] += eps

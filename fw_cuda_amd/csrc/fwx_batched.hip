@@ -251,17 +251,19 @@ struct Launch {
 };
 
 // (tile, kappa) of one graph inside a batch.  Graphs with n >= 128 take
-// (128, 2): 128-wide rest items of depth 256 on the 512-thread product and
-// 128-wide closures, with half the dependent steps of the plain path's
+// (128, 3): 128-wide rest items of depth 384 on the 512-thread product and
+// 128-wide closures, with far fewer dependent steps than the plain path's
 // 64-wide entries for these sizes (tuned for one graph alone, where
 // parallelism per launch matters more).  Measured on the 512-graph
-// configuration (n log-uniform in [64, 2048]): 18.3 -> 16.5 ms per batch.
+// configuration (n log-uniform in [64, 2048]): 18.3 ms per batch on the
+// tuned entries, 16.5 with (128, 2); with the closures on side branches
+// 15.84 for (128, 2), 15.77 for (128, 3), 15.88 for (128, 4).
 // Smaller graphs keep the plain path's tuned entry.  FWX_BATCH_DEEP="n0,T,
 // kappa" replaces the rule (graphs with n >= n0 take (T, kappa)); "off" uses
 // the tuned entries for every graph.
 void batch_config(int32_t n, int32_t* T, int32_t* kappa) {
   tuned_config(n, T, kappa);
-  int n0 = 128, t = 128, k = 2;
+  int n0 = 128, t = 128, k = 3;
   if (const char* e = std::getenv("FWX_BATCH_DEEP")) {
     if (std::strcmp(e, "off") == 0) return;
     int a = 0, b = 0, c = 0;

@@ -44,8 +44,8 @@ def parse():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-diff", action="store_true")
     ap.add_argument("--no-pipeline", action="store_true")
-    ap.add_argument("--no-config3", action="store_true")
-    ap.add_argument("--no-config1", action="store_true")
+    ap.add_argument("--no-n32768", action="store_true")
+    ap.add_argument("--no-n1024", action="store_true")
     return ap.parse_args()
 
 
@@ -57,7 +57,7 @@ def bench_batched(steps):
     import torch
     import fw_cuda_amd as fx
     from fw_cuda_amd import _lib
-    ns, dens, seeds = fx.batch_config4()
+    ns, dens, seeds = fx.mixed_batch()
     offs, total = fx.batch_layout(ns)
     flat = np.zeros(total, dtype=np.int32)
     for n, p, sd, o in zip(ns, dens, seeds, offs):
@@ -149,7 +149,7 @@ def bench_pipeline(n):
     return out
 
 
-def bench_config1(reps=20):
+def bench_n1024(reps=20):
     """n=1024, density 0.9, weights U[1,1000], seed 42 (the reference-compatible
     host generator), for each GPU algorithm: per-call time with HIP events on
     the launch stream, mean and min of `reps` calls each on a fresh device copy
@@ -197,7 +197,7 @@ def bench_config1(reps=20):
     return out
 
 
-def bench_config3():
+def bench_n32768():
     """n=32768 (4 GiB int32), density 0.1, weights U[1,1000], seed 7 (the
     reference-compatible host generator): one multi_layer_tiling_gpu and one
     tiling_gpu solve (tuned configurations), n^3/s each, both results equal,
@@ -334,7 +334,7 @@ def cpu_baseline(n, n_warm):
             "reference_protocol": {"n": n_warm, "seconds": dt_warm,
                                    "value": float(n_warm) ** 3 / dt_warm,
                                    "note": "timed call on the matrix solved by the warmup call"},
-            "config1": {"n": 1024, "cold_seconds": dt1, "reference_protocol_seconds": dt1_warm,
+            "n1024": {"n": 1024, "cold_seconds": dt1, "reference_protocol_seconds": dt1_warm,
                         "value": 1024.0 ** 3 / dt1}}
 
 
@@ -501,10 +501,10 @@ def main():
         out["differentiable"] = bench_diff(3)
     if not a.no_pipeline:
         out["input_pipeline"] = bench_pipeline(n)
-    if not a.no_config1:
-        out["config1"] = bench_config1()
-    if not a.no_config3:
-        out["config3"] = bench_config3()
+    if not a.no_n1024:
+        out["n1024"] = bench_n1024()
+    if not a.no_n32768:
+        out["n32768"] = bench_n32768()
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(a.cpu_n, a.cpu_n_warm)
     print(json.dumps(out), flush=True)

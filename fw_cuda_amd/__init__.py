@@ -240,7 +240,7 @@ def batch_layout(sizes, align=4):
     return np.array(offs, dtype=np.int64), pos
 
 
-def batch_config4(seed=11, count=512, lo=64, hi=2048):
+def mixed_batch(seed=11, count=512, lo=64, hi=2048):
     """The batched workload: `count` graphs with n log-uniform in [lo, hi]
     (numpy PCG64 seeded with `seed`), density per graph U[0.05, 0.9], int32
     weights U[1, 1000]; graph g uses generator seed seed*100000 + g + 1."""

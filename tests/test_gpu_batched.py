@@ -165,24 +165,24 @@ def test_batch_persistent_wait_limit():
 
 
 @pytest.mark.slow
-def test_batch_config4():
+def test_batch_512_mixed_sizes():
     """512 graphs, n log-uniform in [64, 2048] (seed 11), density U[0.05, 0.9],
     weights U[1, 1000]: one hipGraph replay, every graph bit-exact."""
     import torch
     from fw_cuda_amd import _lib
-    ns, dens, seeds = fx.batch_config4()
+    ns, dens, seeds = fx.mixed_batch()
     graphs = make_batch(ns, dens, seeds)
     sizes, offs, t, flat0 = upload(graphs)
     plan = fx.BatchPlan(sizes, offs, _lib.ptr(t))
     plan.run(True)
     got, flat = unpack(t, sizes, offs)
-    say("config4 gpu done; oracle", plan.stats())
+    say("512 graphs: gpu done; oracle", plan.stats())
     for g in range(len(graphs)):
         assert (got[g] == po.fw_rowpar(graphs[g])).all(), g
     # a second replay on restored input gives the same matrices
     for mode in ("graph",):
         t.copy_(torch.from_numpy(flat0).to("cuda:0"))
-        say("config4", mode)
+        say("512 graphs", mode)
         plan.run(mode)
         got2, _ = unpack(t, sizes, offs)
         for g in range(len(graphs)):

@@ -211,7 +211,7 @@ def _fd(W64, G64, u, v, eps):
     return float((G64[fin] * Dp[fin]).sum()), float((G64[fin] * Dm[fin]).sum())
 
 
-def test_config5_batch512_finite_differences():
+def test_fd_64_graphs_n512():
     """64 graphs at n=512, density 0.3, weights U[0.5, 10], seed 3: dL/dW on
     256 random present edges (4 per graph) against float64 central FD."""
     W, G = graphs(64, 512, 0.3, 3)
@@ -255,7 +255,7 @@ def test_config5_batch512_finite_differences():
 
 
 @pytest.mark.slow
-def test_config5_n4096():
+def test_fd_n4096():
     """One graph at n=4096: D rows against float64 Dijkstra-free checks (path
     consistency through the GPU's own predecessors) and dW against path
     enumeration over those predecessors."""

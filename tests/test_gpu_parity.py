@@ -82,7 +82,7 @@ def ref_cases():
 def test_reference_seeded_graphs(ref_cases, algo, tile, kappa):
     for n, p, seed, g, expect in ref_cases:
         if algo == "baseline_gpu" and n > 512:
-            continue  # n launches; covered at 1024 by test_config1
+            continue  # n launches; covered at 1024 by test_n1024_every_algorithm
         say("seeded", n, p, seed, algo, tile, kappa)
         assert (gpu(g, algo, tile, kappa) == expect).all(), (n, p, seed)
 
@@ -220,12 +220,12 @@ def test_diag_block_sizes(monkeypatch, nth):
             assert (gpu(gneg, "multi_layer_tiling_gpu", t, k) == po.fw_rowpar(gneg)).all(), (n, t, k)
 
 
-def test_config1_n1024():
+def test_n1024_every_algorithm():
     """n=1024, density 0.9, weights U[1,1000], seed 42: every algorithm."""
     g = fx.generate_graph(1024, 0.9, 42, 1, 1000)
     expect = po.fw_serial(g)
     for algo, tile, kappa in CONFIGS:
-        say("config1", algo, tile, kappa)
+        say("n=1024", algo, tile, kappa)
         assert (gpu(g, algo, tile, kappa) == expect).all(), (algo, tile, kappa)
 
 
@@ -239,28 +239,28 @@ def test_idempotent_on_solved_matrix():
 
 
 @pytest.mark.slow
-def test_config2_n8192_all_kappas():
+def test_n8192_all_kappas():
     """n=8192, density 0.9, U[1,1000], seed 42: multi_layer_tiling_gpu with
     the autotuned tile and kappa in {1,2,4,6,8}, tile 64 with the tuned
     kappa's neighbour, and tiling_gpu, bit-exact against the row-parallel
     oracle (the restatement of fwBaselineCPU)."""
     g = fx.generate_graph(8192, 0.9, 42, 1, 1000)
-    say("config2: oracle start")
+    say("n=8192: oracle start")
     expect = po.fw_rowpar(g)
-    say("config2: oracle done")
+    say("n=8192: oracle done")
     tile, _ = fx.tuned_config(8192)
     runs = [("multi_layer_tiling_gpu", tile, k) for k in (1, 2, 4, 6, 8)]
     runs += [("multi_layer_tiling_gpu", 64, 3), ("tiling_gpu", 0, 0)]
     with fx.DeviceMatrix(8192) as dm:
         for algo, t, k in runs:
             dm.upload(g)
-            say("config2", algo, t, k)
+            say("n=8192", algo, t, k)
             fx.run_device(algo, dm.ptr, 8192, t, k)
             assert (dm.download() == expect).all(), (algo, t, k)
 
 
 @pytest.mark.slow
-def test_config3_n32768_spot_rows():
+def test_n32768_spot_rows():
     """n=32768 (4 GiB int32), density 0.1, weights U[1,1000], seed 7:
     multi_layer_tiling_gpu (tuned) certified exact on every row by the device
     certificate (fwx_certify_rows), on 4096 random rows by the oracle's
@@ -269,30 +269,30 @@ def test_config3_n32768_spot_rows():
     n = 32768
     g = fx.generate_graph(n, 0.1, 7, 1, 1000)
     rows = np.sort(np.random.default_rng(7).choice(n, 4096, replace=False)).astype(np.int32)
-    say("config3: dijkstra on 64 rows")
+    say("n=32768: dijkstra on 64 rows")
     dij = po.dijkstra_rows(g, rows[::64])
     import ctypes
     import torch
     with fx.DeviceMatrix(n) as dm, fx.DeviceMatrix(n) as dw:
         dm.upload(g)
         dw.upload(g)
-        say("config3 multi_layer_tiling_gpu")
+        say("n=32768 multi_layer_tiling_gpu")
         fx.run_device("multi_layer_tiling_gpu", dm.ptr, n, 0, 0)
         ml = dm.download()
-        say("config3: device certificate of all", n, "rows")
+        say("n=32768: device certificate of all", n, "rows")
         src = torch.arange(n, dtype=torch.int32, device="cuda")
         ok = torch.zeros(n, dtype=torch.int32, device="cuda")
         failed = fx.certify_rows_device(dw.ptr, n, ctypes.c_void_p(src.data_ptr()), n, dm.ptr,
                                         ctypes.c_void_p(ok.data_ptr()))
         all_rows_ok = failed == 0 and bool((ok == 1).all())
         dm.upload(g)
-        say("config3 tiling_gpu")
+        say("n=32768 tiling_gpu")
         fx.run_device("tiling_gpu", dm.ptr, n, 0, 0)
         same = bool((dm.download() == ml).all())
     assert all_rows_ok, failed
     assert same, "tiling_gpu and multi_layer_tiling_gpu differ"
     assert (ml[rows[::64]] == dij).all()
-    say("config3: certificate on", rows.size, "rows")
+    say("n=32768: certificate on", rows.size, "rows")
     ok = po.certify_rows(g, rows, ml[rows])
     assert ok.all(), rows[~ok][:10]
 

@@ -182,6 +182,6 @@ def test_batch_layout_alignment():
     offs, total = fx.batch_layout([3, 5, 1, 64])
     assert list(offs) == [0, 12, 40, 44] and total == 44 + 4096
     assert all(o % 4 == 0 for o in offs)
-    ns, dens, seeds = fx.batch_config4()
+    ns, dens, seeds = fx.mixed_batch()
     assert len(ns) == 512 and ns.min() >= 64 and ns.max() <= 2048
     assert 0.05 <= dens.min() and dens.max() <= 0.9

@@ -176,9 +176,11 @@ int fwx_certify_rows(const int32_t* d_W, int32_t n, const int32_t* d_src, int32_
  * Many independent graphs solved together.  Graph g is an n_g x n_g row-major
  * int32 matrix at d_data + offsets[g] (element offsets; `sizes` and
  * `offsets` are host arrays of `count` entries; d_data 16-byte aligned).
- * The plan builds every graph's multi-layer schedule (tile and kappa from
- * the tuned table per size class), merges step j of all graphs into one
- * launch per work kind, largest graphs first, and uploads the work lists.
+ * The plan builds every graph's multi-layer schedule (graphs with n >= 128
+ * on tile 128, kappa 3; smaller ones on the tuned table's entry for their
+ * size), moves each diagonal closure, with the one product item it needs,
+ * onto a side branch of the step before it, merges step j of all graphs into
+ * one launch per work kind, largest graphs first, and uploads the work lists.
  * Run modes:
  *   FWX_BATCH_DIRECT (0)   the step launches enqueued directly;
  *   FWX_BATCH_GRAPH (1)    the step launches captured into a hipGraph on first
@@ -193,8 +195,11 @@ int fwx_certify_rows(const int32_t* d_W, int32_t n, const int32_t* d_src, int32_
  * The two persistent modes are switched off in this build and return
  * FWX_ERR_UNSUPPORTED (their first GPU runs did not complete).
  * Results equal a plain fwx_multi_layer_tiling_gpu call per graph in every
- * mode.  The plan owns device memory (its work lists) until
- * fwx_batch_plan_destroy.
+ * mode.  The plan owns device memory (its work lists), a side stream and two
+ * events until fwx_batch_plan_destroy; side branches fork from the plan's
+ * stream and join it again within each run, and a run returns after the
+ * plan's stream is idle.  One plan runs one batch at a time: concurrent runs
+ * of the same plan from several host threads race on its side stream.
  */
 typedef struct fwx_batch_plan fwx_batch_plan;
 #define FWX_BATCH_DIRECT 0

@@ -234,10 +234,15 @@ int fwx_diff_forward(const float* d_W, float* d_D, int32_t* d_P, int32_t n, int3
 /*
  * Backward of L = sum(G * D): dW[u][v] = sum of G[s][t] over every (s,t)
  * whose chosen path uses edge (u,v); d_dW is overwritten.  Per source s the
- * vertices are taken in decreasing D[s][.] and each adds its accumulated
- * gradient to its tree parent.  d_P = NULL recomputes the predecessors from
- * d_W and d_D, 256 sources at a time (a 256 x n x batch int32 workspace
- * instead of the n x n x batch predecessor matrix).
+ * vertices are taken from the deepest level of its shortest-path tree up and
+ * each adds its accumulated gradient to its tree parent; each source's final
+ * gradient row is then summed into dW by predecessor, per target column.
+ * With d_P given, those rows are staged in d_dW itself and nothing is
+ * allocated.  d_P = NULL recomputes the predecessors from d_W and d_D, 256
+ * sources at a time (a stream-ordered workspace of 256 x n x batch int32
+ * plus as many floats, instead of the n x n x batch predecessor matrix).
+ * FWX_DIFF_BACK=0 in the environment adds to dW with one global atomic per
+ * (source, vertex) instead (the same sums up to fp32 rounding order).
  */
 int fwx_diff_backward(const float* d_W, const float* d_D, const int32_t* d_P, const float* d_G,
                       float* d_dW, int32_t n, int32_t batch, void* stream);

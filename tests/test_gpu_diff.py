@@ -69,7 +69,7 @@ def test_forward_and_backward_small():
     assert (dW[~np.isfinite(W)] == 0).all()
 
 
-@pytest.mark.parametrize("batch,n", [(3, 129), (2, 300), (1, 640), (4, 200)])
+@pytest.mark.parametrize("batch,n", [(3, 129), (2, 300), (1, 640), (4, 200), (5, 70), (64, 512)])
 def test_forward_schedules_agree(monkeypatch, batch, n):
     """The single-stream, look-ahead and side-band (two-stream) forwards give
     the same bits for D and P."""

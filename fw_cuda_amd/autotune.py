@@ -3,8 +3,8 @@
 For every size class it times the candidate configurations of
 fwx_multi_layer_tiling_gpu on a representative graph (density 0.9, weights
 U[1,1000]) on the current GPU and keeps the fastest.  The rest-pass block
-size R (and with it the register micro-tile: R=128 -> 8x8 cells per thread,
-64 -> 4x4, 32 -> 2x2) follows from the stage width kappa*tile, so choosing
+size R (and with it the register micro-tile: R=128 -> 8x4 cells per thread
+on 512 threads, 64 -> 4x4, 32 -> 2x2) follows from the stage width kappa*tile, so choosing
 (tile, kappa) chooses the micro-tile too.
 
 Writes

@@ -789,7 +789,7 @@ template <int NT, int PER>
 __global__ __launch_bounds__(NT) void k_fback(const float* __restrict__ D, const int32_t* __restrict__ Pr,
                                                         int64_t pstride, const float* __restrict__ G,
                                                         float* dW, int32_t n, int32_t s0, float* gout,
-                                                        int64_t gstride) {
+                                                        int64_t gstride, int32_t walk) {
   extern __shared__ __align__(16) unsigned char smem[];
   float* g = reinterpret_cast<float*>(smem);
   int32_t* off = reinterpret_cast<int32_t*>(g + n);
@@ -821,81 +821,110 @@ __global__ __launch_bounds__(NT) void k_fback(const float* __restrict__ D, const
   }
   for (int d = tid; d < n + 2; d += NT) off[d] = 0;
   __syncthreads();
-  // 1. pointer jumping: after it, aux[t] == src and dep[t] = hops for every
-  // reachable t.  Rounds are synchronous (all reads, barrier, all writes,
-  // barrier); the "anything moved" flag of round r lives in slot r & 1 and
-  // is cleared for round r + 1 only after every thread has passed round r's
-  // first barrier, i.e. finished reading the slot of round r - 1.
-  constexpr int kPer = PER;  // the host launches with n <= NT * PER
-  for (int round = 0; round < 16; ++round) {
-    int16_t nd[kPer], na[kPer];
-    int changed = 0;
-#pragma unroll
-    for (int e = 0; e < kPer; ++e) {
-      const int t = tid + e * NT;
-      nd[e] = 0;
-      na[e] = -1;
-      if (t < n) {
-        const int a = aux[t];
-        if (a >= 0 && a != src) {
-          nd[e] = dep[a];
-          na[e] = aux[a];
-          changed = 1;
+  // Shallow trees: when every reachable vertex is within kWalk parent links
+  // of the source (checked by a bounded walk), each vertex adds its own G to
+  // every ancestor below the source with LDS float atomics -- the same sums,
+  // in another order, with two barriers instead of the pointer-jumping
+  // rounds, the sort and one barrier per level.  Otherwise the level pass.
+  constexpr int kWalk = 32;
+  bool deep = !walk;
+  for (int t = tid; t < n && !deep; t += NT) {
+    int v = par[t];
+    for (int h = 0; v >= 0 && v != src && h < kWalk; ++h) v = par[v];
+    deep = v >= 0 && v != src;  // par[t] < 0: the source or unreachable
+    if (par[t] >= 0 && v < 0) deep = true;  // a broken chain: leave it to the level pass
+  }
+  if (!__syncthreads_or(deep)) {
+    for (int t = tid; t < n; t += NT) {
+      int v = par[t];
+      if (v < 0) continue;
+      const float x = Grow[t];
+      while (v != src) {  // at most kWalk links, by the check above
+        atomicAdd(&g[v], x);
+        v = par[v];
+      }
+    }
+    __syncthreads();
+    if (!gout)
+      for (int t = tid; t < n; t += NT)
+        if (par[t] >= 0) atomicAdd(&dWg[(int64_t)par[t] * n + t], g[t]);
+  } else {
+    // 1. pointer jumping: after it, aux[t] == src and dep[t] = hops for every
+    // reachable t.  Rounds are synchronous (all reads, barrier, all writes,
+    // barrier); the "anything moved" flag of round r lives in slot r & 1 and
+    // is cleared for round r + 1 only after every thread has passed round r's
+    // first barrier, i.e. finished reading the slot of round r - 1.
+    constexpr int kPer = PER;  // the host launches with n <= NT * PER
+    for (int round = 0; round < 16; ++round) {
+      int16_t nd[kPer], na[kPer];
+      int changed = 0;
+  #pragma unroll
+      for (int e = 0; e < kPer; ++e) {
+        const int t = tid + e * NT;
+        nd[e] = 0;
+        na[e] = -1;
+        if (t < n) {
+          const int a = aux[t];
+          if (a >= 0 && a != src) {
+            nd[e] = dep[a];
+            na[e] = aux[a];
+            changed = 1;
+          }
         }
       }
+      __syncthreads();
+      if (changed) s_changed[round & 1] = 1;
+  #pragma unroll
+      for (int e = 0; e < kPer; ++e) {
+        const int t = tid + e * NT;
+        if (t < n && na[e] >= 0) {
+          dep[t] = (int16_t)(dep[t] + nd[e]);
+          aux[t] = na[e];
+        }
+      }
+      __syncthreads();
+      const int any = s_changed[round & 1];
+      if (tid == 0) s_changed[(round + 1) & 1] = 0;
+      if (!any) break;
     }
-    __syncthreads();
-    if (changed) s_changed[round & 1] = 1;
-#pragma unroll
-    for (int e = 0; e < kPer; ++e) {
-      const int t = tid + e * NT;
-      if (t < n && na[e] >= 0) {
-        dep[t] = (int16_t)(dep[t] + nd[e]);
-        aux[t] = na[e];
+    // 2. counting sort by depth: off[d] counts level d, then exclusive prefix
+    // sums (off[0] = 0 stays), then scatter into aux (order by level)
+    for (int t = tid; t < n; t += NT) {
+      const int d = dep[t];
+      if (d > 0 && d <= n) {
+        atomicAdd(&off[d], 1);
+        atomicMax(&s_h, d);
       }
     }
     __syncthreads();
-    const int any = s_changed[round & 1];
-    if (tid == 0) s_changed[(round + 1) & 1] = 0;
-    if (!any) break;
-  }
-  // 2. counting sort by depth: off[d] counts level d, then exclusive prefix
-  // sums (off[0] = 0 stays), then scatter into aux (order by level)
-  for (int t = tid; t < n; t += NT) {
-    const int d = dep[t];
-    if (d > 0 && d <= n) {
-      atomicAdd(&off[d], 1);
-      atomicMax(&s_h, d);
-    }
-  }
-  __syncthreads();
-  const int H = s_h;
-  if (tid == 0) {
-    int run = 0;
-    for (int d = 1; d <= H; ++d) {  // start of level d; serial: H is the tree height
-      const int c = off[d];
-      off[d] = run;
-      run += c;
-    }
-  }
-  __syncthreads();
-  for (int t = tid; t < n; t += NT) {
-    const int d = dep[t];
-    if (d > 0 && d <= n) aux[atomicAdd(&off[d], 1)] = (int16_t)t;
-  }
-  __syncthreads();
-  // now level d occupies [off[d - 1], off[d]) of aux (off[0] = 0)
-  // 3. deepest level first
-  for (int d = H; d >= 1; --d) {
-    const int q0 = off[d - 1], q1 = off[d];
-    for (int q = q0 + tid; q < q1; q += NT) {
-      const int t = aux[q];
-      const int p = par[t];
-      const float gt = g[t];
-      atomicAdd(&g[p], gt);
-      if (!gout) atomicAdd(&dWg[(int64_t)p * n + t], gt);
+    const int H = s_h;
+    if (tid == 0) {
+      int run = 0;
+      for (int d = 1; d <= H; ++d) {  // start of level d; serial: H is the tree height
+        const int c = off[d];
+        off[d] = run;
+        run += c;
+      }
     }
     __syncthreads();
+    for (int t = tid; t < n; t += NT) {
+      const int d = dep[t];
+      if (d > 0 && d <= n) aux[atomicAdd(&off[d], 1)] = (int16_t)t;
+    }
+    __syncthreads();
+    // now level d occupies [off[d - 1], off[d]) of aux (off[0] = 0)
+    // 3. deepest level first
+    for (int d = H; d >= 1; --d) {
+      const int q0 = off[d - 1], q1 = off[d];
+      for (int q = q0 + tid; q < q1; q += NT) {
+        const int t = aux[q];
+        const int p = par[t];
+        const float gt = g[t];
+        atomicAdd(&g[p], gt);
+        if (!gout) atomicAdd(&dWg[(int64_t)p * n + t], gt);
+      }
+      __syncthreads();
+    }
   }
   // column form: the final g of every vertex the levels above processed, 0
   // for the others (the source, unreachable vertices), one coalesced row per
@@ -992,6 +1021,13 @@ static hipError_t fcolumns(const int32_t* P, int64_t pstride, const float* Gb, i
     case 8: return fcolumns_launch<8>(P, pstride, Gb, gstride, rows, dW, n, batch, first, st);
     default: return fcolumns_launch<4>(P, pstride, Gb, gstride, rows, dW, n, batch, first, st);
   }
+}
+
+// FWX_DIFF_WALK=0: every source's tree is accumulated level by level, also
+// when it is shallow (read on every call).
+static bool diff_back_walk() {
+  const char* e = std::getenv("FWX_DIFF_WALK");
+  return !(e && e[0] == '0');
 }
 
 // FWX_DIFF_BACK=0: each source adds its contributions to dW with one global
@@ -1324,6 +1360,7 @@ int fwx_diff_backward(const float* d_W, const float* d_D, const int32_t* d_P, co
   // predecessors come in chunks of 256 sources with a g buffer beside them.
   // FWX_DIFF_BACK=0 zeroes dW and adds to it with global atomics.
   const bool columns = diff_back_columns();
+  const int32_t walk = diff_back_walk() ? 1 : 0;
   if (!columns) {
     const int64_t total = (int64_t)n * n * batch;
     k_ffill<<<(int)std::min<int64_t>((total + 255) / 256, 65536), 256, 0, st>>>(d_dW, total);
@@ -1335,16 +1372,16 @@ int fwx_diff_backward(const float* d_W, const float* d_D, const int32_t* d_P, co
                   int64_t gstride) -> hipError_t {
     const dim3 grid(rows, batch);
     if (n <= 1024) {
-      k_fback<256, 4><<<grid, 256, lds, st>>>(d_D, pr, pstride, d_G, d_dW, n, s0, gb, gstride);
+      k_fback<256, 4><<<grid, 256, lds, st>>>(d_D, pr, pstride, d_G, d_dW, n, s0, gb, gstride, walk);
     } else if (n <= 4096) {
-      k_fback<512, 8><<<grid, 512, lds, st>>>(d_D, pr, pstride, d_G, d_dW, n, s0, gb, gstride);
+      k_fback<512, 8><<<grid, 512, lds, st>>>(d_D, pr, pstride, d_G, d_dW, n, s0, gb, gstride, walk);
     } else {
       if (lds > 65536) {
         const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fback<1024, 8>),
                                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
       }
-      k_fback<1024, 8><<<grid, 1024, lds, st>>>(d_D, pr, pstride, d_G, d_dW, n, s0, gb, gstride);
+      k_fback<1024, 8><<<grid, 1024, lds, st>>>(d_D, pr, pstride, d_G, d_dW, n, s0, gb, gstride, walk);
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess || !gb) return e;

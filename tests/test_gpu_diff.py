@@ -186,20 +186,25 @@ def test_backward_recomputed_predecessors_match_stored():
                                           (3, 300, False), (2, 700, False)])
 def test_backward_column_form_matches_atomics(monkeypatch, batch, n, keep):
     """dW from each source's g row summed per target column in LDS (default)
-    against per-vertex global atomics (FWX_DIFF_BACK=0): equal up to fp32
-    summation order.  Stored predecessors put the g rows in dW itself;
+    against per-vertex global atomics (FWX_DIFF_BACK=0), and shallow trees
+    accumulated by ancestor walks (default) against the level-by-level pass
+    (FWX_DIFF_WALK=0): equal up to fp32 summation order.  Stored predecessors put the g rows in dW itself;
     recomputed ones come in chunks of 256 sources (n = 300 and 700: a first
     chunk that writes dW, then chunks that add to it)."""
     W, G = graphs(batch, n, 0.3, batch + n)
     monkeypatch.setenv("FWX_DIFF_BACK", "0")
+    monkeypatch.setenv("FWX_DIFF_WALK", "0")
     _, g0 = gpu_fwd_bwd(W, G, keep_pred=keep)
+    monkeypatch.delenv("FWX_DIFF_WALK")
+    _, gw = gpu_fwd_bwd(W, G, keep_pred=keep)
     monkeypatch.delenv("FWX_DIFF_BACK")
     _, g1 = gpu_fwd_bwd(W, G, keep_pred=keep)
-    say("column form", batch, n, keep, float(np.abs(g1 - g0).max()))
-    assert (g1[~np.isfinite(W)] == 0).all()
-    assert np.allclose(g1, g0, rtol=1e-5, atol=1e-4)
-    err = float(np.linalg.norm(g1 - g0) / np.linalg.norm(g0))
-    assert err < 1e-6, err
+    say("column form", batch, n, keep, float(np.abs(g1 - g0).max()), float(np.abs(gw - g0).max()))
+    for g in (g1, gw):
+        assert (g[~np.isfinite(W)] == 0).all()
+        assert np.allclose(g, g0, rtol=1e-5, atol=1e-4)
+        err = float(np.linalg.norm(g - g0) / np.linalg.norm(g0))
+        assert err < 1e-6, err
 
 
 def _fd(W64, G64, u, v, eps):

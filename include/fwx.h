@@ -233,10 +233,12 @@ int fwx_diff_forward(const float* d_W, float* d_D, int32_t* d_P, int32_t n, int3
                      void* stream);
 /*
  * Backward of L = sum(G * D): dW[u][v] = sum of G[s][t] over every (s,t)
- * whose chosen path uses edge (u,v); d_dW is overwritten.  Per source s the
- * vertices are taken from the deepest level of its shortest-path tree up and
- * each adds its accumulated gradient to its tree parent; each source's final
- * gradient row is then summed into dW by predecessor, per target column.
+ * whose chosen path uses edge (u,v); d_dW is overwritten.  Per source s each
+ * vertex's gradient becomes G[s][t] plus its subtree's: in a tree whose
+ * chains all reach s within 32 links each vertex adds G to its ancestors,
+ * otherwise the vertices are taken from the deepest level up, each adding
+ * its accumulated gradient to its parent.  Each source's final gradient row
+ * is then summed into dW by predecessor, per target column.
  * With d_P given, those rows are staged in d_dW itself and nothing is
  * allocated.  d_P = NULL recomputes the predecessors from d_W and d_D, 256
  * sources at a time (a stream-ordered workspace of 256 x n x batch int32

@@ -821,7 +821,7 @@ __global__ __launch_bounds__(NT) void k_fback(const float* __restrict__ D, const
   }
   for (int d = tid; d < n + 2; d += NT) off[d] = 0;
   __syncthreads();
-  // Shallow trees: when every reachable vertex is within kWalk + 1 parent links
+  // Shallow trees: when every reachable vertex is within kWalk parent links
   // of the source (checked by a bounded walk), each vertex adds its own G to
   // every ancestor below the source with LDS float atomics -- the same sums,
   // in another order, with two barriers instead of the pointer-jumping
@@ -830,7 +830,7 @@ __global__ __launch_bounds__(NT) void k_fback(const float* __restrict__ D, const
   bool deep = !walk;
   for (int t = tid; t < n && !deep; t += NT) {
     int v = par[t];
-    for (int h = 0; v >= 0 && v != src && h < kWalk; ++h) v = par[v];
+    for (int h = 1; v >= 0 && v != src && h < kWalk; ++h) v = par[v];  // h: links followed
     deep = v >= 0 && v != src;  // par[t] < 0: the source or unreachable
     if (par[t] >= 0 && v < 0) deep = true;  // a broken chain: leave it to the level pass
   }
@@ -839,7 +839,7 @@ __global__ __launch_bounds__(NT) void k_fback(const float* __restrict__ D, const
       int v = par[t];
       if (v < 0) continue;
       const float x = Grow[t];
-      while (v != src) {  // at most kWalk + 1 links, by the check above
+      while (v != src) {  // at most kWalk links, by the check above
         atomicAdd(&g[v], x);
         v = par[v];
       }

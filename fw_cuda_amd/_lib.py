@@ -49,6 +49,12 @@ SIGNATURES = {
     "fwx_tiling_gpu_refshape": (ctypes.c_int, [_vp, _i32, _vp]),
     "fwx_multi_layer_tiling_gpu": (ctypes.c_int, [_vp, _i32, _i32, _i32, _vp]),
     "fwx_tuned_config": (ctypes.c_int, [_i32, _i32p, _i32p]),
+    "fwx_relaxation_form": (ctypes.c_int, [_vp, _i32, _i32p, _vp]),
+    "fwx_event_create": (ctypes.c_int, [ctypes.POINTER(_vp)]),
+    "fwx_event_record": (ctypes.c_int, [_vp, _vp]),
+    "fwx_event_synchronize": (ctypes.c_int, [_vp]),
+    "fwx_event_elapsed_ms": (ctypes.c_int, [ctypes.POINTER(ctypes.c_float), _vp, _vp]),
+    "fwx_event_destroy": (ctypes.c_int, [_vp]),
     "fwx_device_alloc": (ctypes.c_int, [ctypes.POINTER(_vp), _i32]),
     "fwx_copy_to_device": (ctypes.c_int, [_vp, _vp, _i32]),
     "fwx_copy_to_host": (ctypes.c_int, [_vp, _vp, _i32]),
@@ -80,6 +86,7 @@ SIGNATURES = {
 }
 
 PROF_REST, PROF_LEAD, PROF_BASELINE, PROF_BAND, PROF_NCLASS = 0, 1, 2, 3, 4
+FORM_MIN3, FORM_PK64 = 0, 1
 
 _lib = None
 

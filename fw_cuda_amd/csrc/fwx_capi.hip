@@ -209,6 +209,53 @@ int fwx_has_negative_cycle(const int32_t* h_D, int32_t n) {
   return 0;
 }
 
+int fwx_relaxation_form(const int32_t* d_D, int32_t n, int32_t* form, void* stream) {
+  clear_error();
+  if (int rc = check_matrix(d_D, n)) return rc;
+  if (!form) return fail(FWX_ERR_INVALID_ARG, "null output pointer");
+  bool pk = false;
+  if (int rc = values_in_pk64_range(d_D, (int64_t)n * n, as_stream(stream), &pk)) return rc;
+  *form = pk ? FWX_FORM_PK64 : FWX_FORM_MIN3;
+  return FWX_OK;
+}
+
+// ---- events (the cudaEvent calls of src/timer.cu:48-63) ---------------------
+int fwx_event_create(void** ev) {
+  clear_error();
+  if (!ev) return fail(FWX_ERR_INVALID_ARG, "null output pointer");
+  hipEvent_t e = nullptr;
+  FWX_HIP(hipEventCreate(&e));
+  *ev = e;
+  return FWX_OK;
+}
+
+int fwx_event_record(void* ev, void* stream) {
+  clear_error();
+  if (!ev) return fail(FWX_ERR_INVALID_ARG, "null event");
+  FWX_HIP(hipEventRecord(static_cast<hipEvent_t>(ev), as_stream(stream)));
+  return FWX_OK;
+}
+
+int fwx_event_synchronize(void* ev) {
+  clear_error();
+  if (!ev) return fail(FWX_ERR_INVALID_ARG, "null event");
+  FWX_HIP(hipEventSynchronize(static_cast<hipEvent_t>(ev)));
+  return FWX_OK;
+}
+
+int fwx_event_elapsed_ms(float* ms, void* start, void* stop) {
+  clear_error();
+  if (!ms || !start || !stop) return fail(FWX_ERR_INVALID_ARG, "null pointer");
+  FWX_HIP(hipEventElapsedTime(ms, static_cast<hipEvent_t>(start), static_cast<hipEvent_t>(stop)));
+  return FWX_OK;
+}
+
+int fwx_event_destroy(void* ev) {
+  clear_error();
+  if (ev) FWX_HIP(hipEventDestroy(static_cast<hipEvent_t>(ev)));
+  return FWX_OK;
+}
+
 int fwx_has_negative_cycle_device(const int32_t* d_D, int32_t n, int32_t* result, void* stream) {
   clear_error();
   if (!d_D || !result || n <= 0 || n > FWX_MAX_N)

@@ -5,12 +5,13 @@
 // earlier values, so if every input cell is in that range every value of the
 // run is too.  One pass over the input decides the form for the whole call.
 //
-// The flag lives in a file-scope device array (no allocation): a call takes
-// one of 64 slots round-robin, so up to 64 concurrent calls on one device do
-// not share a slot.
+// The flag lives in a file-scope device array (no allocation).  A call holds
+// one of its 64 slots from the zero fill until its result is back on the
+// host; a 65th concurrent call waits for a slot to be released, so no two
+// calls in flight ever share one.
 #include <hip/hip_runtime.h>
 
-#include <atomic>
+#include <condition_variable>
 #include <mutex>
 
 #include "fwx_device.hpp"
@@ -47,7 +48,9 @@ __global__ __launch_bounds__(256) void k_range(const int32_t* __restrict__ d, in
 
 std::mutex g_mu;
 int* g_flag_base[64] = {};
-std::atomic<unsigned> g_slot{0};
+// Slots in use, per device (bit s = slot s), and the wait for a free one.
+uint64_t g_busy[64] = {};
+std::condition_variable g_freed;
 
 int flag_base(int** out) {
   int dev = 0;
@@ -63,22 +66,71 @@ int flag_base(int** out) {
   return FWX_OK;
 }
 
+// A slot of the current device's flag array, held until release_slot.
+int acquire_slot(int** f, int* dev_out, int* slot_out) {
+  int* base = nullptr;
+  if (int rc = flag_base(&base)) return rc;
+  int dev = 0;
+  FWX_HIP(hipGetDevice(&dev));
+  std::unique_lock<std::mutex> lk(g_mu);
+  g_freed.wait(lk, [&] { return g_busy[dev] != ~0ull; });
+  const int s = __builtin_ctzll(~g_busy[dev]);
+  g_busy[dev] |= 1ull << s;
+  *f = base + s;
+  *dev_out = dev;
+  *slot_out = s;
+  return FWX_OK;
+}
+
+void release_slot(int dev, int s) {
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_busy[dev] &= ~(1ull << s);
+  }
+  g_freed.notify_one();
+}
+
+// Zero the slot, run `launch` (which ORs into it), read it back; the slot is
+// released on every path.
+template <class F>
+int flag_query(hipStream_t st, F launch, int* h) {
+  int* f = nullptr;
+  int dev = 0, slot = 0;
+  if (int rc = acquire_slot(&f, &dev, &slot)) return rc;
+  int rc = FWX_OK;
+  k_flag_zero<<<1, 1, 0, st>>>(f);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) rc = hip_fail(e, "k_flag_zero", __FILE__, __LINE__);
+  if (!rc) rc = launch(f);
+  if (!rc) {
+    e = hipMemcpyAsync(h, f, sizeof(int), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) rc = hip_fail(e, "flag read-back", __FILE__, __LINE__);
+  } else {
+    // the launch failed after the zero fill was enqueued: let it drain
+    // before the slot can be handed out again
+    (void)hipStreamSynchronize(st);
+  }
+  release_slot(dev, slot);
+  return rc;
+}
+
 }  // namespace
 
 int values_in_pk64_range(const int32_t* d, int64_t count, hipStream_t st, bool* ok) {
-  int* base = nullptr;
-  if (int rc = flag_base(&base)) return rc;
-  int* f = base + (g_slot.fetch_add(1) % kSlots);
-  k_flag_zero<<<1, 1, 0, st>>>(f);
-  FWX_LAUNCHED("k_flag_zero");
   const int64_t work = (count + 3) / 4;
   const int blocks = (int)std::min<int64_t>((work + 255) / 256, 4096);
   const int vec4 = (reinterpret_cast<uintptr_t>(d) & 15u) == 0;
-  k_range<<<blocks > 0 ? blocks : 1, 256, 0, st>>>(d, count, vec4, f);
-  FWX_LAUNCHED("k_range");
   int h = 1;
-  FWX_HIP(hipMemcpyAsync(&h, f, sizeof(int), hipMemcpyDeviceToHost, st));
-  FWX_HIP(hipStreamSynchronize(st));
+  const int rc = flag_query(
+      st,
+      [&](int* f) -> int {
+        k_range<<<blocks > 0 ? blocks : 1, 256, 0, st>>>(d, count, vec4, f);
+        FWX_LAUNCHED("k_range");
+        return FWX_OK;
+      },
+      &h);
+  if (rc) return rc;
   *ok = (h == 0);
   return FWX_OK;
 }
@@ -96,16 +148,16 @@ __global__ __launch_bounds__(256) void k_negative_diagonal(const int32_t* __rest
 }  // namespace
 
 int device_negative_cycle(const int32_t* d, int32_t n, hipStream_t st, bool* neg) {
-  int* base = nullptr;
-  if (int rc = flag_base(&base)) return rc;
-  int* f = base + (g_slot.fetch_add(1) % kSlots);
-  k_flag_zero<<<1, 1, 0, st>>>(f);
-  FWX_LAUNCHED("k_flag_zero");
-  k_negative_diagonal<<<(n + 255) / 256, 256, 0, st>>>(d, n, f);
-  FWX_LAUNCHED("k_negative_diagonal");
   int h = 0;
-  FWX_HIP(hipMemcpyAsync(&h, f, sizeof(int), hipMemcpyDeviceToHost, st));
-  FWX_HIP(hipStreamSynchronize(st));
+  const int rc = flag_query(
+      st,
+      [&](int* f) -> int {
+        k_negative_diagonal<<<(n + 255) / 256, 256, 0, st>>>(d, n, f);
+        FWX_LAUNCHED("k_negative_diagonal");
+        return FWX_OK;
+      },
+      &h);
+  if (rc) return rc;
   *neg = (h != 0);
   return FWX_OK;
 }

@@ -86,6 +86,18 @@ int fwx_multi_layer_tiling_gpu(int32_t* d_D, int32_t n, int32_t tile, int32_t ka
 /* The tuned (tile, kappa) for size n: what tile=0 / kappa=0 resolve to. */
 int fwx_tuned_config(int32_t n, int32_t* tile, int32_t* kappa);
 
+/*
+ * The relaxation form a plain solve of d_D would use, from the same one-pass
+ * range check every solve starts with: FWX_FORM_PK64 when every cell lies in
+ * [0, 2^30) (one VALU instruction per relaxation), else FWX_FORM_MIN3 (1.5).
+ * Both are exact; the form bounds the solve's VALU ceiling (CUs x 64 lanes x
+ * clock relaxations/s for pk64, two thirds of that for min3).  No reference
+ * counterpart.  Synchronises `stream`.
+ */
+#define FWX_FORM_MIN3 0
+#define FWX_FORM_PK64 1
+int fwx_relaxation_form(const int32_t* d_D, int32_t n, int32_t* form, void* stream);
+
 /* ---------------- device memory (reference GPUMemory, src/gpu_memory.cu) - */
 
 int fwx_device_alloc(int32_t** d_D, int32_t n);                        /* gpu_memory.cu:6-11 */
@@ -95,6 +107,15 @@ int fwx_device_free(int32_t* d_D);                                     /* gpu_me
 int fwx_device_synchronize(void);
 /* Empty launch used to warm the context (reference Timer::warmupGPU, src/timer.cu:7-14). */
 int fwx_warmup(void);
+
+/* Timing events (opaque hipEvent_t as void*), the cudaEventCreate / Record /
+ * Synchronize / ElapsedTime / Destroy calls of Timer::measureGPU
+ * (src/timer.cu:48-63).  `stream` NULL = the default stream, as there. */
+int fwx_event_create(void** ev);
+int fwx_event_record(void* ev, void* stream);
+int fwx_event_synchronize(void* ev);
+int fwx_event_elapsed_ms(float* ms, void* start, void* stop);
+int fwx_event_destroy(void* ev);
 
 /* ---------------- host helpers ----------------------------------------- */
 

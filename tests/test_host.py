@@ -176,6 +176,13 @@ def test_batched_and_diff_argument_errors():
     assert L.fwx_diff_forward(dummy, dummy, None, 8, 0, None) == _lib.ERR_INVALID_ARG
     assert L.fwx_diff_backward(None, dummy, None, dummy, dummy, 8, 1, None) == _lib.ERR_INVALID_ARG
     assert L.fwx_diff_backward(dummy, dummy, None, dummy, None, 8, 1, None) == _lib.ERR_INVALID_ARG
+    assert L.fwx_relaxation_form(None, 4, ctypes.byref(r), None) == _lib.ERR_INVALID_ARG
+    assert L.fwx_relaxation_form(dummy, 4, None, None) == _lib.ERR_INVALID_ARG
+    assert L.fwx_event_create(None) == _lib.ERR_INVALID_ARG
+    assert L.fwx_event_record(None, None) == _lib.ERR_INVALID_ARG
+    assert L.fwx_event_synchronize(None) == _lib.ERR_INVALID_ARG
+    assert L.fwx_event_elapsed_ms(None, dummy, dummy) == _lib.ERR_INVALID_ARG
+    assert L.fwx_event_destroy(None) == _lib.OK
 
 
 def test_batch_layout_alignment():

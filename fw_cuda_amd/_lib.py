@@ -60,6 +60,7 @@ SIGNATURES = {
     "fwx_copy_to_host": (ctypes.c_int, [_vp, _vp, _i32]),
     "fwx_device_free": (ctypes.c_int, [_vp]),
     "fwx_device_synchronize": (ctypes.c_int, []),
+    "fwx_device_memory": (ctypes.c_int, [ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64)]),
     "fwx_warmup": (ctypes.c_int, []),
     "fwx_generate_graph": (ctypes.c_int, [_vp, _i32, ctypes.c_double, ctypes.c_uint32, _i32, _i32]),
     "fwx_has_negative_cycle": (ctypes.c_int, [_vp, _i32]),

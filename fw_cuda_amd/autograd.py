@@ -5,13 +5,29 @@
 
 Forward and backward run in libfwx.so (fwx_diff_forward / fwx_diff_backward,
 include/fwx.h) on torch's current stream; this module only allocates the
-tensors.  keep_pred=False stores no predecessor matrix between the passes;
-the backward then recomputes predecessors 256 sources at a time.
-Absent edges are +inf in W and receive zero gradient; weights must be > 0.
+tensors.  The forward keeps the int32 predecessor matrix (4 n^2 B per graph)
+for the backward when it fits: by default (keep_pred=None) when it takes at
+most PRED_FREE_FRACTION of the device memory free at the forward (the
+device's free memory plus what torch's caching allocator holds unused).
+Otherwise, or with keep_pred=False, nothing is stored between the passes and
+the backward recomputes predecessors 256 sources at a time.  keep_pred=True
+always stores it.  Absent edges are +inf in W and receive zero gradient;
+weights must be > 0 (a predecessor row that is not a tree makes the backward
+raise).
 """
+import ctypes
+
 import torch
 
 from . import _lib
+
+# The predecessor matrix is kept when it needs at most this fraction of the
+# free device memory.
+PRED_FREE_FRACTION = 0.25
+
+# Whether the last forward kept its predecessor matrix (for callers and tests
+# that need to know which backward ran).
+last_kept_pred = None
 
 
 def _shape(W):
@@ -22,13 +38,35 @@ def _shape(W):
     raise ValueError("W must be (n, n) or (batch, n, n)")
 
 
+def free_device_memory(device):
+    """Bytes a new allocation on `device` can use: the device's free memory
+    (fwx_device_memory) plus torch's cached, unallocated blocks."""
+    free = ctypes.c_uint64()
+    total = ctypes.c_uint64()
+    with torch.cuda.device(device):
+        _lib.check(_lib.lib().fwx_device_memory(ctypes.byref(free), ctypes.byref(total)))
+    cached = torch.cuda.memory_reserved(device) - torch.cuda.memory_allocated(device)
+    return int(free.value) + max(0, int(cached))
+
+
+def choose_keep_pred(n, batch, device):
+    """True when the n x n x batch int32 predecessor matrix fits in
+    PRED_FREE_FRACTION of the free device memory."""
+    need = 4 * n * n * batch
+    return need <= PRED_FREE_FRACTION * free_device_memory(device)
+
+
 class FloydWarshallFn(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, W, keep_pred=True):
+    def forward(ctx, W, keep_pred=None):
+        global last_kept_pred
         if not (W.is_cuda and W.dtype == torch.float32 and W.shape[-1] == W.shape[-2]):
             raise ValueError("expected a float32 CUDA tensor of square matrices")
         Wc = W.contiguous()
         batch, n = _shape(Wc)
+        if keep_pred is None:
+            keep_pred = choose_keep_pred(n, batch, Wc.device)
+        last_kept_pred = bool(keep_pred)
         D = torch.empty_like(Wc)
         P = torch.empty(Wc.shape, dtype=torch.int32, device=Wc.device) if keep_pred else None
         stream = torch.cuda.current_stream(Wc.device).cuda_stream
@@ -51,7 +89,7 @@ class FloydWarshallFn(torch.autograd.Function):
         return dW, None
 
 
-def floyd_warshall_diff(W, keep_pred=True):
+def floyd_warshall_diff(W, keep_pred=None):
     """Differentiable all-pairs shortest path lengths of W (see module doc)."""
     return FloydWarshallFn.apply(W, keep_pred)
 

@@ -168,6 +168,16 @@ int fwx_device_synchronize(void) {
   return FWX_OK;
 }
 
+int fwx_device_memory(uint64_t* free_bytes, uint64_t* total_bytes) {
+  clear_error();
+  if (!free_bytes || !total_bytes) return fail(FWX_ERR_INVALID_ARG, "null output pointer");
+  size_t f = 0, t = 0;
+  FWX_HIP(hipMemGetInfo(&f, &t));
+  *free_bytes = f;
+  *total_bytes = t;
+  return FWX_OK;
+}
+
 int fwx_warmup(void) {
   clear_error();
   k_warmup<<<1, 1>>>();

@@ -789,7 +789,7 @@ template <int NT, int PER>
 __global__ __launch_bounds__(NT) void k_fback(const float* __restrict__ D, const int32_t* __restrict__ Pr,
                                                         int64_t pstride, const float* __restrict__ G,
                                                         float* dW, int32_t n, int32_t s0, float* gout,
-                                                        int64_t gstride, int32_t walk) {
+                                                        int64_t gstride, int32_t walk, int* __restrict__ err) {
   extern __shared__ __align__(16) unsigned char smem[];
   float* g = reinterpret_cast<float*>(smem);
   int32_t* off = reinterpret_cast<int32_t*>(g + n);
@@ -887,6 +887,14 @@ __global__ __launch_bounds__(NT) void k_fback(const float* __restrict__ D, const
       if (tid == 0) s_changed[(round + 1) & 1] = 0;
       if (!any) break;
     }
+    // 16 doubling rounds bring every vertex of a tree of height < 2^16 to
+    // the source.  A reachable vertex left elsewhere sits on a cycle or on a
+    // chain that stops short of the source: the row is not a tree (zero or
+    // fp32-absorbed weights can make recomputed predecessors point at each
+    // other), and the gradient would be wrong, so the call reports it.
+    bool broken = false;
+    for (int t = tid; t < n; t += NT) broken |= par[t] >= 0 && aux[t] != src;
+    if (__syncthreads_or(broken) && tid == 0) atomicOr(err, 1);
     // 2. counting sort by depth: off[d] counts level d, then exclusive prefix
     // sums (off[0] = 0 stays), then scatter into aux (order by level)
     for (int t = tid; t < n; t += NT) {
@@ -1333,6 +1341,105 @@ int check_diff(const void* a, const void* b, int32_t n, int32_t batch) {
 
 using namespace fwx;
 
+// The backward of fwx_diff_backward.  Recomputed predecessors (d_P null)
+// need a workspace of rows x n x batch int32 (and as many floats of g rows in
+// the column form), taken from hipMallocAsync with rows = 256 when
+// `scratch` is null, else from `scratch` (scratch_bytes long) with as many
+// sources per chunk as fit, at most 256.  Every workspace cell a kernel
+// reads is written earlier in the same chunk, so it needs no clearing.
+// Ends with `st` synchronised; a predecessor row that is not a tree rooted
+// at its source returns FWX_ERR_INVALID_ARG.
+static int backward_impl(const float* d_W, const float* d_D, const int32_t* d_P, const float* d_G, float* d_dW,
+                         int32_t n, int32_t batch, hipStream_t st, int32_t* scratch, size_t scratch_bytes) {
+  if (int rc = check_diff(d_D, d_G, n, batch)) return rc;
+  if (!d_dW) return fail(FWX_ERR_INVALID_ARG, "null gradient output");
+  if (!d_P && !d_W) return fail(FWX_ERR_INVALID_ARG, "recomputing predecessors needs W");
+  const size_t lds = (size_t)n * 14 + 8;
+  // Column form (default): k_fback writes each source's g row, k_fcolumns
+  // adds the rows into dW by predecessor.  With stored predecessors the g
+  // rows go into dW itself (one chunk of every source); recomputed
+  // predecessors come in chunks of at most 256 sources with a g buffer
+  // beside them.  FWX_DIFF_BACK=0 zeroes dW and adds to it with global
+  // atomics.
+  const bool columns = diff_back_columns();
+  const int32_t walk = diff_back_walk() ? 1 : 0;
+  int32_t rows = std::min<int32_t>(256, n);
+  const size_t per_row = (size_t)n * batch * (columns ? 8 : 4);
+  if (!d_P && scratch) {
+    const size_t fit = scratch_bytes / per_row;
+    if (fit == 0)
+      return fail(FWX_ERR_INVALID_ARG, "n x batch too large for the workspace: one source row of every graph "
+                                       "needs " + std::to_string(per_row) + " bytes, the workspace holds " +
+                                       std::to_string(scratch_bytes));
+    rows = (int32_t)std::min<size_t>((size_t)rows, fit);
+  }
+  DeviceFlag fl;  // set by k_fback when a predecessor row is not a tree
+  if (int rc = flag_acquire(st, &fl)) return rc;
+  int rc = FWX_OK;
+  if (!columns) {
+    const int64_t total = (int64_t)n * n * batch;
+    k_ffill<<<(int)std::min<int64_t>((total + 255) / 256, 65536), 256, 0, st>>>(d_dW, total);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) rc = hip_fail(e, "k_ffill", __FILE__, __LINE__);
+  }
+  // one workgroup per source: 256 threads up to n = 1024, 512 up to 4096,
+  // 1024 above (each thread holds at most PER vertices in the jumping rounds)
+  auto back = [&](const int32_t* pr, int64_t pstride, int32_t nrows, int32_t s0, float* gb,
+                  int64_t gstride) -> hipError_t {
+    const dim3 grid(nrows, batch);
+    if (n <= 1024) {
+      k_fback<256, 4><<<grid, 256, lds, st>>>(d_D, pr, pstride, d_G, d_dW, n, s0, gb, gstride, walk, fl.f);
+    } else if (n <= 4096) {
+      k_fback<512, 8><<<grid, 512, lds, st>>>(d_D, pr, pstride, d_G, d_dW, n, s0, gb, gstride, walk, fl.f);
+    } else {
+      if (lds > 65536) {
+        const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fback<1024, 8>),
+                                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+      }
+      k_fback<1024, 8><<<grid, 1024, lds, st>>>(d_D, pr, pstride, d_G, d_dW, n, s0, gb, gstride, walk, fl.f);
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess || !gb) return e;
+    return fcolumns(pr, pstride, gb, gstride, nrows, d_dW, n, batch, s0 == 0, st);
+  };
+  if (rc) {
+  } else if (d_P) {
+    const int64_t nn = (int64_t)n * n;
+    const hipError_t e = back(d_P, nn, n, 0, columns ? d_dW : nullptr, nn);
+    if (e != hipSuccess) rc = hip_fail(e, "k_fback", __FILE__, __LINE__);
+  } else {
+    const size_t cnt = (size_t)rows * n * batch;
+    int32_t* ws = scratch;
+    if (!ws) {
+      const hipError_t ea = hipMallocAsync(reinterpret_cast<void**>(&ws), cnt * (columns ? 8 : 4), st);
+      if (ea != hipSuccess) {
+        ws = nullptr;
+        rc = hip_fail(ea, "hipMallocAsync", __FILE__, __LINE__);
+      }
+    }
+    float* gb = columns && ws ? reinterpret_cast<float*>(ws + cnt) : nullptr;
+    for (int32_t s0 = 0; s0 < n && rc == FWX_OK; s0 += rows) {
+      const int32_t r = std::min(rows, n - s0);
+      const int nt = (n + FT - 1) / FT;
+      k_fpred<<<dim3(((r + FT - 1) / FT) * nt, batch), 256, 0, st>>>(d_W, d_D, ws, n, s0, r);
+      hipError_t e = hipGetLastError();
+      if (e == hipSuccess) e = back(ws, (int64_t)r * n, r, s0, gb, (int64_t)r * n);
+      if (e != hipSuccess) rc = hip_fail(e, "k_fpred/k_fback", __FILE__, __LINE__);
+    }
+    if (ws && !scratch) (void)hipFreeAsync(ws, st);
+  }
+  int bad = 0;
+  const int rc2 = flag_release(st, &fl, rc ? nullptr : &bad);
+  if (rc) return rc;
+  if (rc2) return rc2;
+  if (bad)
+    return fail(FWX_ERR_INVALID_ARG,
+                "a predecessor row is not a tree rooted at its source (a cycle or a broken chain: zero weights, "
+                "or weights too small to change an fp32 distance); dW is undefined");
+  return FWX_OK;
+}
+
 extern "C" {
 
 int fwx_diff_forward(const float* d_W, float* d_D, int32_t* d_P, int32_t n, int32_t batch, void* stream) {
@@ -1349,71 +1456,7 @@ int fwx_diff_forward(const float* d_W, float* d_D, int32_t* d_P, int32_t n, int3
 int fwx_diff_backward(const float* d_W, const float* d_D, const int32_t* d_P, const float* d_G, float* d_dW,
                       int32_t n, int32_t batch, void* stream) {
   clear_error();
-  if (int rc = check_diff(d_D, d_G, n, batch)) return rc;
-  if (!d_dW) return fail(FWX_ERR_INVALID_ARG, "null gradient output");
-  if (!d_P && !d_W) return fail(FWX_ERR_INVALID_ARG, "recomputing predecessors needs W");
-  hipStream_t st = as_stream(stream);
-  const size_t lds = (size_t)n * 14 + 8;
-  // Column form (default): k_fback writes each source's g row, k_fcolumns
-  // adds the rows into dW by predecessor.  With stored predecessors the g
-  // rows go into dW itself (one chunk of every source); recomputed
-  // predecessors come in chunks of 256 sources with a g buffer beside them.
-  // FWX_DIFF_BACK=0 zeroes dW and adds to it with global atomics.
-  const bool columns = diff_back_columns();
-  const int32_t walk = diff_back_walk() ? 1 : 0;
-  if (!columns) {
-    const int64_t total = (int64_t)n * n * batch;
-    k_ffill<<<(int)std::min<int64_t>((total + 255) / 256, 65536), 256, 0, st>>>(d_dW, total);
-    FWX_LAUNCHED("k_ffill");
-  }
-  // one workgroup per source: 256 threads up to n = 1024, 512 up to 4096,
-  // 1024 above (each thread holds at most PER vertices in the jumping rounds)
-  auto back = [&](const int32_t* pr, int64_t pstride, int32_t rows, int32_t s0, float* gb,
-                  int64_t gstride) -> hipError_t {
-    const dim3 grid(rows, batch);
-    if (n <= 1024) {
-      k_fback<256, 4><<<grid, 256, lds, st>>>(d_D, pr, pstride, d_G, d_dW, n, s0, gb, gstride, walk);
-    } else if (n <= 4096) {
-      k_fback<512, 8><<<grid, 512, lds, st>>>(d_D, pr, pstride, d_G, d_dW, n, s0, gb, gstride, walk);
-    } else {
-      if (lds > 65536) {
-        const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fback<1024, 8>),
-                                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        if (e != hipSuccess) return e;
-      }
-      k_fback<1024, 8><<<grid, 1024, lds, st>>>(d_D, pr, pstride, d_G, d_dW, n, s0, gb, gstride, walk);
-    }
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess || !gb) return e;
-    return fcolumns(pr, pstride, gb, gstride, rows, d_dW, n, batch, s0 == 0, st);
-  };
-  int rc = FWX_OK;
-  if (d_P) {
-    const int64_t nn = (int64_t)n * n;
-    const hipError_t e = back(d_P, nn, n, 0, columns ? d_dW : nullptr, nn);
-    if (e != hipSuccess) rc = hip_fail(e, "k_fback", __FILE__, __LINE__);
-  } else {
-    // source blocks of 256 rows; workspace 256 x n x batch int32 (and as
-    // many floats of g rows in the column form)
-    const int32_t rows = std::min<int32_t>(256, n);
-    const size_t cnt = (size_t)rows * n * batch;
-    int32_t* ws = nullptr;
-    const hipError_t ea = hipMallocAsync(reinterpret_cast<void**>(&ws), cnt * (columns ? 8 : 4), st);
-    if (ea != hipSuccess) rc = hip_fail(ea, "hipMallocAsync", __FILE__, __LINE__);
-    float* gb = columns && ws ? reinterpret_cast<float*>(ws + cnt) : nullptr;
-    for (int32_t s0 = 0; s0 < n && rc == FWX_OK; s0 += rows) {
-      const int32_t r = std::min(rows, n - s0);
-      const int nt = (n + FT - 1) / FT;
-      k_fpred<<<dim3(((r + FT - 1) / FT) * nt, batch), 256, 0, st>>>(d_W, d_D, ws, n, s0, r);
-      hipError_t e = hipGetLastError();
-      if (e == hipSuccess) e = back(ws, (int64_t)r * n, r, s0, gb, (int64_t)r * n);
-      if (e != hipSuccess) rc = hip_fail(e, "k_fpred/k_fback", __FILE__, __LINE__);
-    }
-    if (ws) (void)hipFreeAsync(ws, st);
-  }
-  if (rc) return rc;
-  FWX_HIP(hipStreamSynchronize(st));
-  return FWX_OK;
+  return backward_impl(d_W, d_D, d_P, d_G, d_dW, n, batch, as_stream(stream), nullptr, 0);
 }
 
 }  // extern "C"

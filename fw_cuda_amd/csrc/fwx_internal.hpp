@@ -52,6 +52,19 @@ int prof_chain(hipStream_t st, hipEvent_t* a, int cls, double work);
 // Records a fresh timing event on st (released at fwx_profile_end).
 int prof_mark(hipStream_t st, hipEvent_t* e);
 
+// A device int from a file-scope array (fwx_range.hip), held by one call:
+// flag_acquire takes a free slot of the current device and zeroes it on st;
+// kernels OR verdicts into *f; flag_release reads it into *value
+// (synchronising st) and frees the slot.  With value null it only waits for
+// st to drain and frees the slot (the path after a failed launch).
+struct DeviceFlag {
+  int* f = nullptr;
+  int dev = -1;
+  int slot = -1;
+};
+int flag_acquire(hipStream_t st, DeviceFlag* fl);
+int flag_release(hipStream_t st, DeviceFlag* fl, int* value);
+
 // Range check of count cells at d (fwx_range.hip): *ok = every cell in
 // [0, 2^30), i.e. the pk64 relaxation form is exact for the whole run.
 // Synchronises `st`.

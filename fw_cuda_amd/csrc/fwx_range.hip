@@ -90,29 +90,48 @@ void release_slot(int dev, int s) {
   g_freed.notify_one();
 }
 
-// Zero the slot, run `launch` (which ORs into it), read it back; the slot is
-// released on every path.
-template <class F>
-int flag_query(hipStream_t st, F launch, int* h) {
-  int* f = nullptr;
-  int dev = 0, slot = 0;
-  if (int rc = acquire_slot(&f, &dev, &slot)) return rc;
+}  // namespace
+
+int flag_acquire(hipStream_t st, DeviceFlag* fl) {
+  if (int rc = acquire_slot(&fl->f, &fl->dev, &fl->slot)) return rc;
+  k_flag_zero<<<1, 1, 0, st>>>(fl->f);
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    release_slot(fl->dev, fl->slot);
+    fl->f = nullptr;
+    return hip_fail(e, "k_flag_zero", __FILE__, __LINE__);
+  }
+  return FWX_OK;
+}
+
+int flag_release(hipStream_t st, DeviceFlag* fl, int* value) {
+  if (!fl->f) return FWX_OK;
   int rc = FWX_OK;
-  k_flag_zero<<<1, 1, 0, st>>>(f);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) rc = hip_fail(e, "k_flag_zero", __FILE__, __LINE__);
-  if (!rc) rc = launch(f);
-  if (!rc) {
-    e = hipMemcpyAsync(h, f, sizeof(int), hipMemcpyDeviceToHost, st);
+  if (value) {
+    hipError_t e = hipMemcpyAsync(value, fl->f, sizeof(int), hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e != hipSuccess) rc = hip_fail(e, "flag read-back", __FILE__, __LINE__);
   } else {
-    // the launch failed after the zero fill was enqueued: let it drain
-    // before the slot can be handed out again
+    // a launch failed after the zero fill was enqueued: let the stream
+    // drain before the slot can be handed out again
     (void)hipStreamSynchronize(st);
   }
-  release_slot(dev, slot);
+  release_slot(fl->dev, fl->slot);
+  fl->f = nullptr;
   return rc;
+}
+
+namespace {
+
+// Zero a slot, run `launch` (which ORs into it), read it back; the slot is
+// released on every path.
+template <class F>
+int flag_query(hipStream_t st, F launch, int* h) {
+  DeviceFlag fl;
+  if (int rc = flag_acquire(st, &fl)) return rc;
+  const int rc = launch(fl.f);
+  const int rc2 = flag_release(st, &fl, rc ? nullptr : h);
+  return rc ? rc : rc2;
 }
 
 }  // namespace

@@ -105,6 +105,10 @@ int fwx_copy_to_device(int32_t* d_D, const int32_t* h_D, int32_t n);   /* gpu_me
 int fwx_copy_to_host(int32_t* h_D, const int32_t* d_D, int32_t n);     /* gpu_memory.cu:18-21 */
 int fwx_device_free(int32_t* d_D);                                     /* gpu_memory.cu:23-27 */
 int fwx_device_synchronize(void);
+/* Free and total device memory of the current device in bytes (no
+ * reference counterpart; the differentiable binding sizes its predecessor
+ * storage from it). */
+int fwx_device_memory(uint64_t* free_bytes, uint64_t* total_bytes);
 /* Empty launch used to warm the context (reference Timer::warmupGPU, src/timer.cu:7-14). */
 int fwx_warmup(void);
 

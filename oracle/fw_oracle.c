@@ -489,3 +489,64 @@ void oracle_fw_f64(const double* W, double* D, int32_t n) {
       }
     }
 }
+
+/*
+ * oracle_dijkstra_f64_rows -- float64 single-source shortest path lengths of
+ * the fp32 graph the differentiable forward takes (fwx_diff_forward): W[u][v]
+ * is an edge weight, +inf means no edge, the diagonal is taken as 0.  Row q
+ * holds dist(src[q], .) in float64 (+inf where unreachable).  Dense O(n^2)
+ * Dijkstra per source, sources spread over threads; every present weight
+ * must be >= 0 (returns -1 otherwise, -2 without memory).  An algorithm
+ * independent of Floyd-Warshall, in higher precision than the fp32 kernels:
+ * it checks their distances at sizes the O(n^3) oracles cannot reach in a
+ * test (n = 4096).
+ * Pinned by: equality with oracle_fw_f64 (float64 Floyd-Warshall) on random
+ * graphs and on integer-valued weights (where both are exact), and a
+ * hand-worked graph in tests/test_oracle.py.
+ */
+int oracle_dijkstra_f64_rows(const float* W, int32_t n, const int32_t* src, int32_t nsrc, double* out,
+                             int32_t threads) {
+  const size_t N = (size_t)n;
+  for (size_t c = 0; c < N * N; ++c)
+    if (W[c] < 0.0f) return -1;
+  int bad = 0;
+#ifdef _OPENMP
+  if (threads <= 0) threads = omp_get_max_threads();
+#else
+  (void)threads;
+#endif
+#pragma omp parallel num_threads(threads)
+  {
+    double* dist = (double*)malloc(N * sizeof(double));
+    unsigned char* done = (unsigned char*)malloc(N);
+    if (!dist || !done) {
+#pragma omp atomic write
+      bad = 1;
+    } else {
+#pragma omp for schedule(dynamic, 1)
+      for (int32_t q = 0; q < nsrc; ++q) {
+        const size_t s = (size_t)src[q];
+        for (size_t v = 0; v < N; ++v) { dist[v] = INFINITY; done[v] = 0; }
+        dist[s] = 0.0;
+        for (size_t it = 0; it < N; ++it) {
+          size_t u = N;
+          double best = INFINITY;
+          for (size_t v = 0; v < N; ++v)
+            if (!done[v] && dist[v] < best) { best = dist[v]; u = v; }
+          if (u == N) break; /* the rest is unreachable */
+          done[u] = 1;
+          const float* wu = W + u * N;
+          for (size_t v = 0; v < N; ++v) {
+            if (done[v] || v == u) continue;
+            const double nd = best + (double)wu[v];
+            if (nd < dist[v]) dist[v] = nd;
+          }
+        }
+        memcpy(out + (size_t)q * N, dist, N * sizeof(double));
+      }
+    }
+    free(dist);
+    free(done);
+  }
+  return bad ? -2 : 0;
+}

@@ -55,6 +55,9 @@ def lib():
         _lib.oracle_grad_paths.argtypes = [_i32p, _f32p, _f32p, ctypes.c_int32]
         _lib.oracle_grad_paths.restype = ctypes.c_int
         _lib.oracle_fw_f64.argtypes = [_f64p, _f64p, ctypes.c_int32]
+        _lib.oracle_dijkstra_f64_rows.argtypes = [_f32p, ctypes.c_int32, _i32p, ctypes.c_int32, _f64p,
+                                                  ctypes.c_int32]
+        _lib.oracle_dijkstra_f64_rows.restype = ctypes.c_int
     return _lib
 
 
@@ -190,6 +193,58 @@ def fw_f64(W):
     D = np.empty(W.shape, dtype=np.float64)
     lib().oracle_fw_f64(_ptr(W, _f64p), _ptr(D, _f64p), n)
     return D
+
+
+def dijkstra_f64_rows(W, rows, threads=0):
+    """oracle_dijkstra_f64_rows: float64 shortest path lengths from each source
+    in `rows` over the fp32 graph W (+inf = no edge, diagonal taken as 0)."""
+    W = np.ascontiguousarray(W, dtype=np.float32)
+    n = W.shape[0]
+    assert W.shape == (n, n)
+    rows = np.ascontiguousarray(rows, dtype=np.int32)
+    out = np.empty((rows.size, n), dtype=np.float64)
+    rc = lib().oracle_dijkstra_f64_rows(_ptr(W, _f32p), n, _ptr(rows, _i32p), rows.size,
+                                        _ptr(out, _f64p), int(threads))
+    if rc != 0:
+        raise ValueError(f"oracle_dijkstra_f64_rows failed ({rc}): negative weights?")
+    return out
+
+
+def f32_row_failures(D_rows, P_rows, W, rows, ref, rtol=2e-6):
+    """Checks claimed fp32 distance rows D_rows and predecessor rows P_rows of
+    the sources `rows` against float64 reference rows `ref`
+    (dijkstra_f64_rows): reachability agrees, every finite D is within rtol
+    of the reference, and every predecessor is tight, i.e.
+    ref[P[t]] + W[P[t]][t] equals ref[t] within rtol (P[s] = -1, P of an
+    unreachable t = -1).  Returns a list of (source, target, reason); empty
+    when every row passes."""
+    W = np.asarray(W, dtype=np.float32)
+    bad = []
+    for q, s in enumerate(np.asarray(rows)):
+        d, p, r = np.asarray(D_rows[q]), np.asarray(P_rows[q]), ref[q]
+        fin = np.isfinite(r)
+        for t in np.flatnonzero(np.isfinite(d) != fin):
+            bad.append((int(s), int(t), "reachability"))
+        close = np.abs(d[fin].astype(np.float64) - r[fin]) <= rtol * np.abs(r[fin])
+        for t in np.flatnonzero(fin)[~close]:
+            bad.append((int(s), int(t), "distance"))
+        if p[s] != -1:
+            bad.append((int(s), int(s), "source predecessor"))
+        tgt = np.flatnonzero(fin)
+        tgt = tgt[tgt != s]
+        u = p[tgt]
+        valid = (u >= 0) & (u < len(d))
+        for t in tgt[~valid]:
+            bad.append((int(s), int(t), "missing predecessor"))
+        tgt, u = tgt[valid], u[valid]
+        via = r[u] + W[u, tgt].astype(np.float64)
+        tight = np.abs(via - r[tgt]) <= rtol * np.abs(r[tgt])
+        for t in tgt[~tight]:
+            bad.append((int(s), int(t), "predecessor not tight"))
+        for t in np.flatnonzero(~fin):
+            if p[t] != -1:
+                bad.append((int(s), int(t), "predecessor of an unreachable vertex"))
+    return bad
 
 
 def ref_fw_baseline_cpu(D):

@@ -182,6 +182,52 @@ def test_backward_recomputed_predecessors_match_stored():
     assert np.allclose(g1, g2, rtol=1e-5, atol=1e-4)
 
 
+def test_keep_pred_follows_free_memory(monkeypatch):
+    """With keep_pred left to the binding, the predecessor matrix is stored
+    when it fits in the set fraction of free device memory and recomputed in
+    the backward when it does not (the threshold forced both ways); the two
+    gradients agree, and against path enumeration over the stored P."""
+    import fw_cuda_amd.autograd as ag
+    W, G = graphs(4, 257, 0.3, 17)
+    grads = {}
+    for frac, expect in ((1.0, True), (0.0, False)):
+        monkeypatch.setattr(ag, "PRED_FREE_FRACTION", frac)
+        _, grads[expect] = gpu_fwd_bwd(W, G, keep_pred=None)
+        assert ag.last_kept_pred is expect
+    assert np.allclose(grads[True], grads[False], rtol=1e-5, atol=1e-4)
+    import torch
+    _, Pg = ag.forward_with_pred(torch.from_numpy(W).cuda())
+    Pg = Pg.cpu().numpy()
+    for b in range(W.shape[0]):
+        assert np.allclose(grads[False][b], po.grad_paths(Pg[b], G[b]), rtol=1e-4, atol=1e-3)
+    # the default threshold keeps P for small problems
+    monkeypatch.setattr(ag, "PRED_FREE_FRACTION", 0.25)
+    gpu_fwd_bwd(W, G, keep_pred=None)
+    assert ag.last_kept_pred is True
+
+
+def test_backward_rejects_predecessor_cycles():
+    """Zero-weight edges 1 <-> 2 reached from source 0 through 3 and 4: the
+    recomputed predecessors (smallest tight u) point 1 -> 2 and 2 -> 1, a
+    cycle that is not a shortest-path tree.  The backward reports it as an
+    error instead of returning a gradient with that cycle's terms dropped."""
+    from fw_cuda_amd import FwxError
+    n = 64
+    W = np.full((1, n, n), np.inf, dtype=np.float32)
+    np.fill_diagonal(W[0], 0.0)
+    for u, v, w in ((0, 3, 1.0), (0, 4, 1.0), (3, 1, 1.0), (4, 2, 1.0), (1, 2, 0.0), (2, 1, 0.0)):
+        W[0, u, v] = w
+    G = np.ones((1, n, n), dtype=np.float32)
+    # premise, on the oracle: D[0][1] == D[0][2], so both are tight for each other
+    D64 = po.fw_f64(W[0].astype(np.float64))
+    assert D64[0, 1] == D64[0, 2] == 2.0
+    with pytest.raises(FwxError, match="not a tree"):
+        gpu_fwd_bwd(W, G, keep_pred=False)
+    # the library is usable afterwards: a valid problem still runs
+    W2, G2 = graphs(1, 64, 0.3, 9)
+    gpu_fwd_bwd(W2, G2, keep_pred=False)
+
+
 @pytest.mark.parametrize("batch,n,keep", [(3, 129, True), (2, 700, True), (1100, 200, True), (1100, 200, False),
                                           (3, 300, False), (2, 700, False)])
 def test_backward_column_form_matches_atomics(monkeypatch, batch, n, keep):
@@ -261,9 +307,12 @@ def test_fd_64_graphs_n512():
 
 @pytest.mark.slow
 def test_fd_n4096():
-    """One graph at n=4096: D rows against float64 Dijkstra-free checks (path
-    consistency through the GPU's own predecessors) and dW against path
-    enumeration over those predecessors."""
+    """One graph at n=4096 (density 0.3, weights U[0.5, 10], seed 3): D rows of
+    16 sampled sources against the oracle's float64 Dijkstra within 2e-6
+    relative, every predecessor of those rows tight against the float64
+    distances (a path-consistent but non-shortest forward fails here: a
+    planted one is checked to fail), path consistency of walks through the
+    GPU's predecessors, and dW against path enumeration over them."""
     import torch
     from fw_cuda_amd.autograd import forward_with_pred
     W, G = graphs(1, 4096, 0.3, 3)
@@ -271,6 +320,17 @@ def test_fd_n4096():
     Dg, Pg = Dg.cpu().numpy()[0], Pg.cpu().numpy()[0]
     _, dW = gpu_fwd_bwd(W, G)
     dW = dW[0]
+    rows = np.random.default_rng(5).choice(4096, 16, replace=False).astype(np.int32)
+    ref = po.dijkstra_f64_rows(W[0], rows)
+    bad = po.f32_row_failures(Dg[rows], Pg[rows], W[0], rows, ref)
+    say(f"n=4096: {len(rows)} rows vs float64 Dijkstra, {len(bad)} failures {bad[:5]}")
+    assert bad == []
+    # planted: one cell set to a longer direct edge, its predecessor to the source
+    s = int(rows[0])
+    t = int(np.flatnonzero(np.isfinite(W[0, s]) & (W[0, s] > 1.5 * Dg[s]) & (np.arange(4096) != s))[0])
+    D2, P2 = Dg[rows].copy(), Pg[rows].copy()
+    D2[0, t], P2[0, t] = W[0, s, t], s
+    assert po.f32_row_failures(D2, P2, W[0], rows, ref) != []
     # walking P from sampled sources reproduces D within fp32 rounding
     rng = np.random.default_rng(4)
     for s in rng.choice(4096, 16, replace=False):

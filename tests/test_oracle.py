@@ -273,6 +273,61 @@ def test_f32_pred_matches_integer_oracle():
                 assert tot == D[s, t]
 
 
+def test_dijkstra_f64_matches_fw_f64():
+    """The float64 single-source rows equal float64 Floyd-Warshall bit for
+    bit: fp32 weights in [0.5, 10] are multiples of 2^-24, so every path sum
+    below 2^29 is exact in float64 and the two orders cannot round
+    differently.  Sparse graphs leave vertices unreachable (+inf)."""
+    rng = np.random.default_rng(41)
+    for n, p in [(1, 0.5), (2, 0.0), (60, 0.02), (200, 0.05), (257, 0.3)]:
+        W = np.where(rng.random((n, n)) < p, rng.uniform(0.5, 10.0, (n, n)), np.inf).astype(np.float32)
+        np.fill_diagonal(W, 0)
+        full = po.fw_f64(W.astype(np.float64))
+        rows = np.arange(n, dtype=np.int32)
+        got = po.dijkstra_f64_rows(W, rows)
+        assert np.array_equal(got, full), n
+    # hand-worked: 0->1 (1), 1->2 (2), 0->2 (5), 2->0 (1), vertex 3 unreachable
+    W = np.full((4, 4), np.inf, dtype=np.float32)
+    np.fill_diagonal(W, 0)
+    W[0, 1], W[1, 2], W[0, 2], W[2, 0] = 1, 2, 5, 1
+    got = po.dijkstra_f64_rows(W, np.array([0, 2, 3], dtype=np.int32))
+    inf = np.inf
+    assert np.array_equal(got, [[0, 1, 3, inf], [1, 2, 0, inf], [inf, inf, inf, 0]])
+    with pytest.raises(ValueError):
+        po.dijkstra_f64_rows(-W, np.array([0], dtype=np.int32))
+
+
+def test_f32_row_check_catches_planted_mistakes():
+    """f32_row_failures accepts the fp32 oracle's own rows and rejects a row
+    with a path-consistent but non-shortest distance (a direct edge kept
+    where a shorter detour exists, its predecessor pointing at the source),
+    a distance off by 1e-4 relative, and a non-tight predecessor."""
+    rng = np.random.default_rng(43)
+    n = 300
+    W = np.where(rng.random((n, n)) < 0.3, rng.uniform(0.5, 10.0, (n, n)), np.inf).astype(np.float32)
+    np.fill_diagonal(W, 0)
+    D, P = po.fw_f32_pred(W)
+    rows = rng.choice(n, 16, replace=False).astype(np.int32)
+    ref = po.dijkstra_f64_rows(W, rows)
+    assert po.f32_row_failures(D[rows], P[rows], W, rows, ref) == []
+    s = int(rows[0])
+    # a present direct edge much longer than the shortest distance
+    cand = np.flatnonzero(np.isfinite(W[s]) & (W[s] > 1.5 * D[s]) & (np.arange(n) != s))
+    t = int(cand[0])
+    D2, P2 = D[rows].copy(), P[rows].copy()
+    D2[0, t], P2[0, t] = W[s, t], s
+    kinds = {k for (_, tt, k) in po.f32_row_failures(D2, P2, W, rows, ref) if tt == t}
+    assert kinds == {"distance", "predecessor not tight"}
+    D3 = D[rows].copy()
+    D3[0, t] *= 1.0001
+    assert [k for (_, tt, k) in po.f32_row_failures(D3, P[rows], W, rows, ref)] == ["distance"]
+    # a predecessor u that is not even joined to t by an edge
+    P4 = P[rows].copy()
+    u = int(np.flatnonzero(~np.isfinite(W[:, t]))[0])
+    P4[0, t] = u
+    assert [k for (_, tt, k) in po.f32_row_failures(D[rows], P4, W, rows, ref)] == ["predecessor not tight"]
+
+
 def _fd_grad(W, G, edges, eps):
     g = []
     W = W.astype(np.float64)

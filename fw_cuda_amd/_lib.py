@@ -82,6 +82,7 @@ SIGNATURES = {
     "fwx_batch_loop": (ctypes.c_int, [_vp, _vp, _i32, _vp, _vp]),
     "fwx_diff_forward": (ctypes.c_int, [_vp, _vp, _vp, _i32, _i32, _vp]),
     "fwx_diff_backward": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _i32, _i32, _vp]),
+    "fwx_diff_backward_noalloc": (ctypes.c_int, [_vp, _vp, _vp, _vp, _i32, _i32, _vp]),
     "fwx_profile_begin": (ctypes.c_int, []),
     "fwx_profile_end": (ctypes.c_int, [_vp, _vp, _vp]),
 }

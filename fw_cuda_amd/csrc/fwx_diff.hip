@@ -25,6 +25,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
+#include <mutex>
 
 #include "fwx_internal.hpp"
 
@@ -1341,16 +1342,44 @@ int check_diff(const void* a, const void* b, int32_t n, int32_t batch) {
 
 using namespace fwx;
 
+// Workspace of fwx_diff_backward_noalloc: 16 MiB per device, in the code
+// object's own device memory (allocated when the library is loaded on a
+// device, held until the process ends).  Its name in host code is a
+// host-side stand-in, not a device address: host code uses only the
+// per-device address from scratch_address().  Every call on one device
+// shares it, so calls that overlap on one device race.
+constexpr size_t kScratchBytes = 16u << 20;
+static __device__ __align__(256) int32_t scratch[kScratchBytes / sizeof(int32_t)];
+
+static std::mutex g_scratch_mu;
+static int32_t* g_scratch_ptr[64] = {};
+
+// The current device's address of `scratch`, looked up once per device
+// (with that device current) on the first call there.
+static int scratch_address(int32_t** out) {
+  int dev = 0;
+  FWX_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 64) return fail(FWX_ERR_UNSUPPORTED, "device index above 63");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    int32_t* ptr = nullptr;
+    FWX_HIP(hipGetSymbolAddress(reinterpret_cast<void**>(&ptr), HIP_SYMBOL(scratch)));
+    g_scratch_ptr[dev] = ptr;
+  }
+  *out = g_scratch_ptr[dev];
+  return FWX_OK;
+}
+
 // The backward of fwx_diff_backward.  Recomputed predecessors (d_P null)
 // need a workspace of rows x n x batch int32 (and as many floats of g rows in
 // the column form), taken from hipMallocAsync with rows = 256 when
-// `scratch` is null, else from `scratch` (scratch_bytes long) with as many
+// `workspace` is null, else from `workspace` (workspace_bytes long) with as many
 // sources per chunk as fit, at most 256.  Every workspace cell a kernel
 // reads is written earlier in the same chunk, so it needs no clearing.
 // Ends with `st` synchronised; a predecessor row that is not a tree rooted
 // at its source returns FWX_ERR_INVALID_ARG.
 static int backward_impl(const float* d_W, const float* d_D, const int32_t* d_P, const float* d_G, float* d_dW,
-                         int32_t n, int32_t batch, hipStream_t st, int32_t* scratch, size_t scratch_bytes) {
+                         int32_t n, int32_t batch, hipStream_t st, int32_t* workspace, size_t workspace_bytes) {
   if (int rc = check_diff(d_D, d_G, n, batch)) return rc;
   if (!d_dW) return fail(FWX_ERR_INVALID_ARG, "null gradient output");
   if (!d_P && !d_W) return fail(FWX_ERR_INVALID_ARG, "recomputing predecessors needs W");
@@ -1365,12 +1394,12 @@ static int backward_impl(const float* d_W, const float* d_D, const int32_t* d_P,
   const int32_t walk = diff_back_walk() ? 1 : 0;
   int32_t rows = std::min<int32_t>(256, n);
   const size_t per_row = (size_t)n * batch * (columns ? 8 : 4);
-  if (!d_P && scratch) {
-    const size_t fit = scratch_bytes / per_row;
+  if (!d_P && workspace) {
+    const size_t fit = workspace_bytes / per_row;
     if (fit == 0)
       return fail(FWX_ERR_INVALID_ARG, "n x batch too large for the workspace: one source row of every graph "
                                        "needs " + std::to_string(per_row) + " bytes, the workspace holds " +
-                                       std::to_string(scratch_bytes));
+                                       std::to_string(workspace_bytes));
     rows = (int32_t)std::min<size_t>((size_t)rows, fit);
   }
   DeviceFlag fl;  // set by k_fback when a predecessor row is not a tree
@@ -1410,7 +1439,7 @@ static int backward_impl(const float* d_W, const float* d_D, const int32_t* d_P,
     if (e != hipSuccess) rc = hip_fail(e, "k_fback", __FILE__, __LINE__);
   } else {
     const size_t cnt = (size_t)rows * n * batch;
-    int32_t* ws = scratch;
+    int32_t* ws = workspace;
     if (!ws) {
       const hipError_t ea = hipMallocAsync(reinterpret_cast<void**>(&ws), cnt * (columns ? 8 : 4), st);
       if (ea != hipSuccess) {
@@ -1427,7 +1456,7 @@ static int backward_impl(const float* d_W, const float* d_D, const int32_t* d_P,
       if (e == hipSuccess) e = back(ws, (int64_t)r * n, r, s0, gb, (int64_t)r * n);
       if (e != hipSuccess) rc = hip_fail(e, "k_fpred/k_fback", __FILE__, __LINE__);
     }
-    if (ws && !scratch) (void)hipFreeAsync(ws, st);
+    if (ws && !workspace) (void)hipFreeAsync(ws, st);
   }
   int bad = 0;
   const int rc2 = flag_release(st, &fl, rc ? nullptr : &bad);
@@ -1457,6 +1486,20 @@ int fwx_diff_backward(const float* d_W, const float* d_D, const int32_t* d_P, co
                       int32_t n, int32_t batch, void* stream) {
   clear_error();
   return backward_impl(d_W, d_D, d_P, d_G, d_dW, n, batch, as_stream(stream), nullptr, 0);
+}
+
+int fwx_diff_backward_noalloc(const float* d_W, const float* d_D, const float* d_G, float* d_dW, int32_t n,
+                              int32_t batch, void* stream) {
+  clear_error();
+  if (int rc = check_diff(d_D, d_G, n, batch)) return rc;
+  if (!d_W) return fail(FWX_ERR_INVALID_ARG, "recomputing predecessors needs W");
+  const size_t per_row = (size_t)n * batch * (diff_back_columns() ? 8 : 4);
+  if (per_row > kScratchBytes)
+    return fail(FWX_ERR_INVALID_ARG, "fwx_diff_backward_noalloc: n x batch = " + std::to_string((size_t)n * batch) +
+                                         " exceeds the 16 MiB workspace (at most 2097152 in the column form)");
+  int32_t* ws = nullptr;
+  if (int rc = scratch_address(&ws)) return rc;
+  return backward_impl(d_W, d_D, nullptr, d_G, d_dW, n, batch, as_stream(stream), ws, kScratchBytes);
 }
 
 }  // extern "C"

@@ -274,6 +274,23 @@ int fwx_diff_forward(const float* d_W, float* d_D, int32_t* d_P, int32_t n, int3
 int fwx_diff_backward(const float* d_W, const float* d_D, const int32_t* d_P, const float* d_G,
                       float* d_dW, int32_t n, int32_t batch, void* stream);
 
+/*
+ * fwx_diff_backward with recomputed predecessors (d_P = NULL) that allocates
+ * nothing.  No counterpart in the reference: it is for callers that cannot
+ * allocate device memory -- code under HIP graph capture, tight training
+ * loops, harnesses that forbid hipMalloc.  The recomputation workspace is a
+ * 16 MiB array in the library's own device memory (one per device, held
+ * until the process ends) instead of a stream-ordered allocation per call;
+ * sources are processed in chunks of as many rows as fit (at most 256, as
+ * fwx_diff_backward), so n x batch may be at most 2,097,152 (1,048,576 with
+ * FWX_DIFF_BACK=0); larger sizes return FWX_ERR_INVALID_ARG.  Calls that
+ * overlap on one device share that workspace and race: run one at a time
+ * per device.  Same results as fwx_diff_backward(d_W, d_D, NULL, ...) for
+ * sizes where both take 256-source chunks; synchronises `stream`.
+ */
+int fwx_diff_backward_noalloc(const float* d_W, const float* d_D, const float* d_G, float* d_dW, int32_t n,
+                              int32_t batch, void* stream);
+
 /* ---------------- profiling (HIP events, no reference counterpart) ----- */
 /*
  * Between fwx_profile_begin() and fwx_profile_end(), every launch group is

@@ -183,6 +183,11 @@ def test_batched_and_diff_argument_errors():
     assert L.fwx_event_synchronize(None) == _lib.ERR_INVALID_ARG
     assert L.fwx_event_elapsed_ms(None, dummy, dummy) == _lib.ERR_INVALID_ARG
     assert L.fwx_event_destroy(None) == _lib.OK
+    # the allocation-free backward rejects sizes beyond its workspace before
+    # touching the device
+    assert L.fwx_diff_backward_noalloc(dummy, dummy, dummy, dummy, 8192, 300, None) == _lib.ERR_INVALID_ARG
+    assert b"16 MiB workspace" in L.fwx_last_error()
+    assert L.fwx_diff_backward_noalloc(None, dummy, dummy, dummy, 8, 1, None) == _lib.ERR_INVALID_ARG
 
 
 def test_batch_layout_alignment():

@@ -62,7 +62,8 @@ def parse():
 def bench_batched(steps):
     """512 graphs, n log-uniform in [64, 2048] (seed 11), density U[0.05, 0.9],
     weights U[1, 1000]: graphs/s of one hipGraph replay of the merged step
-    launches vs. the per-graph launch loop (one plain solve per graph)."""
+    launches, of the one persistent cooperative launch (from its hipGraph and
+    direct), and of the per-graph launch loop (one plain solve per graph)."""
     import numpy as np
     import torch
     import fw_cuda_amd as fx
@@ -90,6 +91,12 @@ def bench_batched(steps):
         return t / steps, work.clone()
 
     steps_s, steps_out = timed("graph")
+    pers = {}
+    for mode in ("persistent_graph", "persistent"):
+        p_s, p_out = timed(mode)
+        pers[mode] = {"graphs_per_s": len(ns) / p_s, "ms_per_batch": p_s * 1e3,
+                      "equals_step_replay": bool(torch.equal(p_out, steps_out))}
+        del p_out
     work.copy_(src)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -97,11 +104,14 @@ def bench_batched(steps):
     loop_s = time.perf_counter() - t0
     same = bool(torch.equal(steps_out, work))
     st = plan.stats()
+    pinfo = plan.persistent_info()
     plan.destroy()
     relax = float((ns.astype(np.float64) ** 3).sum())
+    pers["kernel"] = ("one cooperative launch of k_batch_coop: %d resident 512-thread blocks, %d grid-wide "
+                      "steps, %d tasks" % (pinfo["grid"], pinfo["steps"], pinfo["tasks"]))
     return {"graphs": int(len(ns)), "graphs_per_s": len(ns) / steps_s, "ms_per_batch": steps_s * 1e3,
-            "mode": "hipGraph replay of the merged step launches (the persistent single-kernel mode is "
-                    "switched off in this build)",
+            "mode": "hipGraph replay of the merged step launches (the fastest mode)",
+            "persistent": pers,
             "loop_graphs_per_s": len(ns) / loop_s, "loop_ms_per_batch": loop_s * 1e3,
             "speedup_vs_loop": loop_s / steps_s, "n3_per_s": relax / steps_s,
             "plan": st, "replay_equals_loop": same}

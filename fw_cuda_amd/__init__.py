@@ -177,8 +177,9 @@ class BatchPlan:
     `sizes[g]` and element `offsets[g]` locate graph g inside the int32 device
     buffer at address `d_data`.  run(mode) with mode one of BATCH_MODES:
     "graph" replays the captured hipGraph of the step launches, "direct"
-    enqueues them, "persistent" runs the whole batch as one persistent kernel
-    and "persistent_graph" replays that launch from a hipGraph.  A bool is
+    enqueues them, "persistent" runs the whole batch as one persistent
+    cooperative kernel (grid-wide barriers between steps) and
+    "persistent_graph" replays that one launch from a hipGraph.  A bool is
     accepted for the first two (True = "graph")."""
 
     def __init__(self, sizes, offsets, d_data, stream=None):
@@ -195,14 +196,12 @@ class BatchPlan:
             mode = "graph" if mode else "direct"
         check(lib().fwx_batch_plan_run(self.plan, BATCH_MODES[mode]))
 
-    def set_wait_limit(self, seconds):
-        """Limit of one device-side dependency wait of the persistent mode."""
-        check(lib().fwx_batch_plan_set_wait_limit(self.plan, float(seconds)))
-
-    def queues(self):
-        q, g, t = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int64()
-        check(lib().fwx_batch_plan_queues(self.plan, ctypes.byref(q), ctypes.byref(g), ctypes.byref(t)))
-        return {"queues": q.value, "grid": g.value, "tasks": t.value}
+    def persistent_info(self):
+        """The persistent mode's cooperative grid (blocks; 0 without
+        cooperative launches), grid-wide steps and tasks."""
+        g, st, t = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int64()
+        check(lib().fwx_batch_plan_persistent_info(self.plan, ctypes.byref(g), ctypes.byref(st), ctypes.byref(t)))
+        return {"grid": g.value, "steps": st.value, "tasks": t.value}
 
     def stats(self):
         st, la, it = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int64()

@@ -18,13 +18,20 @@
 // C, A and B windows, so one kernel serves panel, cross and rest work.
 // Items are ordered largest-graph first within a launch (longest processing
 // time first), so the big graphs' tiles start before the small ones' tail.
+//
+// The persistent mode runs the same steps inside ONE cooperative kernel
+// (k_batch_coop below): resident blocks take host-assigned task lists per
+// step, and grid-wide barriers stand in for the launch boundaries.
+#include <hip/hip_cooperative_groups.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <map>
+#include <queue>
 #include <vector>
 
 #include "fwx_device.hpp"
@@ -32,6 +39,8 @@
 #include "fwx_tile.hpp"
 
 namespace fwx {
+
+namespace cg = cooperative_groups;
 
 struct ProdItem {
   int64_t base;
@@ -90,151 +99,96 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
 }
 
 // ---------------------------------------------------------------------------
-// Persistent mode: the whole batch in ONE kernel launch.
+// Persistent mode: the whole batch in ONE cooperative launch.
 //
-// Tasks (a diagonal closure or a tile product of one graph) sit in one list
-// per XCD.  Each graph belongs to exactly one XCD's list, and a block takes
-// tasks only from the list of the XCD it is running on (read from the
-// hardware XCC_ID register), so every dependency is between blocks sharing
-// one L2: the producer's stores reach that L2 before it bumps the graph's
-// `done` counter (an L2 atomic), and the consumer invalidates its CU's L1
-// after seeing the count.  No cross-XCD cache maintenance is needed.
-//
-// A list is ordered by (step, graph), so a task waits only on tasks earlier
-// in its list; those were taken by blocks that are running, and the earliest
-// unfinished task always has its dependencies done -- the kernel cannot
-// deadlock.  Every wait is bounded in wall-clock time (wait_ticks): a wait
-// that exceeds it raises a flag that ends every block's loop, and the host
-// reports an error.  A list whose XCD got no block leaves tasks unrun, which
-// the host detects from the `finished` count.
-struct Task {
+// A grid of resident 512-thread blocks (as many as fit on the device, two per
+// CU) walks the plan's steps.  In step j the blocks take the step's units
+// (one task, or a side branch's product and closure) longest first from a
+// shared counter, then every block meets at a grid-wide barrier before step
+// j+1.  The steps and their side
+// branches are those of the step launches above -- a hoisted closure follows
+// its product in one block's list -- so the barriers carry exactly the
+// dependencies the launch boundaries carry there, and the barrier's own
+// device-scope fence makes each step's stores visible to every XCD.  The
+// cooperative launch guarantees that every block of the grid is resident at
+// once (the runtime refuses a grid that does not fit), so every barrier is
+// reached by the whole grid and completes; the kernel has no waits of its
+// own.  A launch without the cooperative attribute has no grid barrier: every
+// block sees that at entry, before any barrier, sets *status and returns.
+struct CoopTask {
   int64_t base;
   int32_t n;
-  int16_t kind;  // Kind below
+  int16_t kind;  // kDiag128 or kProd128
   int16_t vec;   // 16-byte global loads are legal for this graph
-  int32_t cr, cc;  // C (product) or diagonal (closure) tile origin
-  int32_t k0, depth;
-  int32_t graph;  // index into done[]
-  int32_t need;   // done[graph] must reach this before the task starts
+  int32_t cr, cc, ar, ac, br, bc;
+  int32_t depth;
 };
 
-// Every counter lives in a 128-byte line of its own that only one XCD
-// touches: the L2s of different XCDs are not coherent with each other during
-// a kernel, so two XCDs must never update the same word.
-constexpr int kMaxQueues = 16;
-constexpr int kLine = 32;  // int32 per 128-byte line
-// Spin cap of one dependency wait (each spin: two L2 atomics and an
-// s_sleep), a second bound next to the wall-clock limit.
-constexpr uint32_t kMaxSpins = 1u << 22;
-struct BatchCtl {          // one per queue
-  int32_t next;            // next task of the queue
-  int32_t error;           // a wait of this queue exceeded its limit
-  int32_t finished;        // tasks of this queue completed
-  int32_t pad[kLine - 3];
-};
-
-// The XCC id each wave of each 256-thread block reports (4 entries per block).
-__device__ __forceinline__ uint32_t xcc_id() {
-  uint32_t x;
-  asm volatile("s_getreg_b32 %0, hwreg(HW_REG_XCC_ID)" : "=s"(x));
-  return x & 15u;
-}
-
-__global__ __launch_bounds__(256) void k_xcc_probe(int32_t* out) {
-  if ((threadIdx.x & 63) == 0) out[blockIdx.x * 4 + (threadIdx.x >> 6)] = (int32_t)xcc_id();
-}
-
-// done[] holds one counter per graph, kLine ints apart.
-__global__ void k_batch_reset(int32_t* __restrict__ done, int32_t ngraphs, BatchCtl* __restrict__ ctl) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < ngraphs) done[i * kLine] = 0;
-  if (i < kMaxQueues) {
-    ctl[i].next = 0;
-    ctl[i].error = 0;
-    ctl[i].finished = 0;
-  }
-}
-
-// L2-coherent read of a counter: a compare-and-swap that never matches is a
-// read-modify-write performed in this XCD's L2, never served from the L1.
-__device__ __forceinline__ int32_t l2_read(int32_t* p) { return atomicCAS(p, INT32_MIN, INT32_MIN); }
-
-template <int R, bool VEC, bool PK>
-__device__ __forceinline__ void task_prod(TileSmem<64, 64, 32>& s, int32_t* g, const Task& t) {
-  const View A = make_view(g, t.n, t.cr, t.k0);
-  const View B = make_view(g, t.n, t.k0, t.cc);
-  const View C = make_view(g, t.n, t.cr, t.cc);
-  tile_update<R, R, 32, VEC, 16, PK>(*reinterpret_cast<TileSmem<R, R, 32>*>(&s), A, B, C, t.depth);
-}
-
-// 256 threads; tasks of tile width <= 64 (the persistent plan caps the tile
-// and the rest block at 64), so register pressure stays that of the 64-wide
-// product and four blocks share a CU.
+// Every persistent task is 128 wide (coop_config below): one product shape --
+// the 512-thread, 8 x 4 cells per thread product of the step launches -- and
+// one closure, within the same 128 VGPRs.  The products read the thread index
+// opaquely (block_tid<true>): with the plain index, their per-thread offsets
+// were hoisted out of the task loop, stayed live across it, and the kernel
+// spilled ~50 VGPRs inside the products' inner loops.
 template <bool PK>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) void k_batch_persistent(
-    int32_t* __restrict__ data, const Task* __restrict__ tasks, const int32_t* __restrict__ qoff,
-    const int32_t* __restrict__ xcc_q, int32_t* __restrict__ done, BatchCtl* __restrict__ ctl,
-    int64_t wait_ticks) {
-  __shared__ TileSmem<64, 64, 32> s;
-  __shared__ int32_t s_idx, s_ok, s_q;
-  // one reading of the XCC id for the whole block (thread 0's), so every
-  // wave takes the same queue and the same exit
-  if (threadIdx.x == 0) s_q = xcc_q[xcc_id()];
-  __syncthreads();
-  const int q = s_q;
-  if (q < 0) return;
-  const int begin = qoff[q], end = qoff[q + 1];
-  for (;;) {
-    if (threadIdx.x == 0) {
-      const int i = __hip_atomic_fetch_add(&ctl[q].next, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-      int ok = 1;
-      if (begin + i < end) {
-        const int32_t g = tasks[begin + i].graph, need = tasks[begin + i].need;
-        const int64_t t0 = wall_clock64();
-        uint32_t spins = 0;
-        while (l2_read(&done[g * kLine]) < need) {
-          if (l2_read(&ctl[q].error) != 0) {
-            ok = 0;
-            break;
-          }
-          // bounded twice: wall-clock time, and a spin count (each spin sleeps)
-          if (wall_clock64() - t0 > wait_ticks || ++spins > kMaxSpins) {
-            __hip_atomic_fetch_or(&ctl[q].error, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-            ok = 0;
-            break;
-          }
-          __builtin_amdgcn_s_sleep(2);
-        }
-      }
-      s_idx = i;
-      s_ok = ok;
-    }
-    __syncthreads();
-    const int i = s_idx;
-    if (begin + i >= end || !s_ok) break;
-    // the producers' stores are in this XCD's L2; drop stale L1 lines
-    asm volatile("buffer_inv sc0" ::: "memory");
-    const Task t = tasks[begin + i];
-    int32_t* g = data + t.base;
-    int32_t* lines = reinterpret_cast<int32_t*>(&s);
-    switch (t.kind) {
-      case kDiag32: close_tile_in<32, 256>(make_view(g, t.n, t.cr, t.cr), lines); break;
-      case kDiag64: close_tile_in<64, 256>(make_view(g, t.n, t.cr, t.cr), lines); break;
-      case kProd32:
-        if (t.vec) task_prod<32, true, PK>(s, g, t); else task_prod<32, false, PK>(s, g, t);
-        break;
-      default:
-        if (t.vec) task_prod<64, true, PK>(s, g, t); else task_prod<64, false, PK>(s, g, t);
-        break;
-    }
-    // every store of this block has reached the L2 before the count moves
-    __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      __hip_atomic_fetch_add(&done[t.graph * kLine], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-      __hip_atomic_fetch_add(&ctl[q].finished, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-    }
+__device__ __forceinline__ void coop_task(TileSmem<128, 128, 32>& s, int32_t* data, const CoopTask& t) {
+  int32_t* g = data + t.base;
+  if (t.kind == kDiag128) {
+    close_tile_in<128, 512>(make_view(g, t.n, t.cr, t.cr), reinterpret_cast<int32_t*>(&s));
+    return;
   }
+  const View A = make_view(g, t.n, t.ar, t.ac);
+  const View B = make_view(g, t.n, t.br, t.bc);
+  const View C = make_view(g, t.n, t.cr, t.cc);
+  if (t.vec)
+    tile_update<128, 128, 32, true, 32, PK, true>(s, A, B, C, t.depth);
+  else
+    tile_update<128, 128, 32, false, 32, PK, true>(s, A, B, C, t.depth);
+}
+
+// Step j's units are [sfirst[j], sfirst[j+1]), longest first; unit u is
+// tasks [ufirst[u], ufirst[u+1]) run in order by one block.  A block takes
+// the next unit of the step with one device-scope atomic on ctr[j] (zeroed
+// before the launch) until the step's units run out, then waits at the grid
+// barrier.  (Claiming the next unit while the current one runs, to hide the
+// atomic's round trip, measured slower: 19.5 vs 18.9 ms on the 512-graph
+// batch.)
+template <bool PK>
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_batch_coop(
+    int32_t* __restrict__ data, const CoopTask* __restrict__ tasks, const int32_t* __restrict__ ufirst,
+    const int32_t* __restrict__ sfirst, int32_t steps, int32_t* __restrict__ ctr, int32_t* __restrict__ status) {
+  __shared__ TileSmem<128, 128, 32> s;
+  __shared__ int32_t s_unit;
+  cg::grid_group grid = cg::this_grid();
+  if (!grid.is_valid()) {
+    if (threadIdx.x == 0) atomicOr(status, 1);
+    return;
+  }
+  for (int32_t j = 0; j < steps; ++j) {
+    const int32_t u0 = sfirst[j], u1 = sfirst[j + 1];
+    for (;;) {
+      // every thread read the previous s_unit before the barrier that ended
+      // the previous unit's last task (or the previous step's grid barrier)
+      if (threadIdx.x == 0) s_unit = u0 + atomicAdd(&ctr[j], 1);
+      __syncthreads();
+      const int32_t u = s_unit;
+      if (u >= u1) break;
+      const int32_t t1 = ufirst[u + 1];
+      for (int32_t t = ufirst[u]; t < t1; ++t) {
+        coop_task<PK>(s, data, tasks[t]);
+        __syncthreads();  // the next task restages the LDS the last one read
+      }
+    }
+    grid.sync();
+  }
+}
+
+// Clears the claim counters of every step and the status word before a
+// persistent run (a kernel, so that a captured graph holds no memset node).
+__global__ void k_coop_reset(int32_t* __restrict__ ctr, int32_t steps, int32_t* __restrict__ status) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < steps) ctr[i] = 0;
+  if (i == 0) *status = 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -415,18 +369,15 @@ struct fwx_batch_plan {
   int64_t items = 0;
   int32_t steps = 0;
   int64_t span = 0;  // elements from d_data covering every graph
-  // persistent mode
-  int32_t count = 0;           // graphs (entries of d_done)
-  fwx::Task* d_tasks = nullptr;
-  int32_t* d_qoff = nullptr;   // per-queue task ranges (nq + 1)
-  int32_t* d_xccq = nullptr;   // hardware XCC id -> queue (16 entries, -1 unused)
-  int32_t* d_done = nullptr;
-  fwx::BatchCtl* d_ctl = nullptr;
-  int64_t ntasks = 0;
-  int32_t nq = 0;
-  int32_t pgrid = 0;           // resident blocks of the persistent kernel
-  double wait_s = 10.0;        // limit of one dependency wait
-  int64_t wall_khz = 0;        // device wall-clock rate
+  // persistent mode (one cooperative launch)
+  fwx::CoopTask* d_ctasks = nullptr;
+  int32_t* d_ufirst = nullptr; // first task of each unit (units + 1)
+  int32_t* d_sfirst = nullptr; // first unit of each step (csteps + 1)
+  int32_t* d_ctr = nullptr;    // per-step claim counters (csteps)
+  int32_t* d_status = nullptr; // set by k_batch_coop when launched without a grid barrier
+  int64_t ctasks = 0;
+  int32_t csteps = 0;
+  int32_t cgrid = 0;           // resident blocks of k_batch_coop (0: no cooperative launch here)
   hipGraph_t pgraph[2] = {};
   hipGraphExec_t pexec[2] = {};
 };
@@ -485,13 +436,11 @@ static int enqueue_mode(fwx_batch_plan* p, bool pk) { return pk ? enqueue_plan<t
 
 template <bool PK>
 static int enqueue_persistent(fwx_batch_plan* p) {
-  const int nreset = std::max(p->count, kMaxQueues);
-  k_batch_reset<<<(nreset + 255) / 256, 256, 0, p->stream>>>(p->d_done, p->count, p->d_ctl);
-  FWX_LAUNCHED("k_batch_reset");
-  const int64_t ticks = (int64_t)(p->wait_s * (double)p->wall_khz * 1000.0);
-  k_batch_persistent<PK><<<p->pgrid, 256, 0, p->stream>>>(p->data, p->d_tasks, p->d_qoff, p->d_xccq, p->d_done,
-                                                           p->d_ctl, ticks);
-  FWX_LAUNCHED("k_batch_persistent");
+  k_coop_reset<<<(p->csteps + 255) / 256, 256, 0, p->stream>>>(p->d_ctr, p->csteps, p->d_status);
+  FWX_LAUNCHED("k_coop_reset");
+  void* args[] = {&p->data, &p->d_ctasks, &p->d_ufirst, &p->d_sfirst, &p->csteps, &p->d_ctr, &p->d_status};
+  FWX_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&k_batch_coop<PK>), dim3(p->cgrid), dim3(512),
+                                     args, 0, p->stream));
   return FWX_OK;
 }
 
@@ -506,106 +455,108 @@ static int upload(T** dst, const T* src, size_t count) {
   return FWX_OK;
 }
 
-// Which hardware XCC ids this device's workgroups run on: one probe launch
-// over the persistent grid, every wave of every block reporting.  Returns
-// the id -> queue map (16 entries), or nq = 0 when the waves of one block
-// disagree (then the persistent mode is unavailable on this device).
-static int probe_xcc(int grid, std::vector<int32_t>& xcc_q, int32_t* nq) {
-  int32_t* d = nullptr;
-  const size_t cnt = (size_t)grid * 4;
-  FWX_HIP(hipMalloc(reinterpret_cast<void**>(&d), cnt * sizeof(int32_t)));
-  k_xcc_probe<<<grid, 256>>>(d);
-  hipError_t e = hipGetLastError();
-  std::vector<int32_t> h(cnt, -1);
-  if (e == hipSuccess) e = hipMemcpy(h.data(), d, cnt * sizeof(int32_t), hipMemcpyDeviceToHost);
-  (void)hipFree(d);
-  if (e != hipSuccess) return hip_fail(e, "XCC probe", __FILE__, __LINE__);
-  xcc_q.assign(16, -1);
-  *nq = 0;
-  for (size_t b = 0; b < (size_t)grid; ++b) {
-    const int32_t x = h[b * 4];
-    for (int w = 1; w < 4; ++w)
-      if (h[b * 4 + w] != x) {
-        xcc_q.assign(16, -1);
-        *nq = 0;
-        return FWX_OK;
-      }
-    if (x >= 0 && x < 16 && xcc_q[x] < 0) xcc_q[x] = (*nq)++;
+// (tile, kappa) of a graph in the persistent plan: 128-wide tiles for every
+// graph (the kernel's one product shape); graphs with n >= 128 keep the step
+// plan's kappa, smaller ones are one 128-wide closure.
+static void coop_config(int32_t n, int32_t* T, int32_t* kappa) {
+  batch_config(n, T, kappa);
+  if (*T != 128) {
+    *T = 128;
+    *kappa = 1;
   }
-  return FWX_OK;
 }
 
-// Per-queue task lists: graphs (LPT order q) dealt to the least-loaded
-// queue by n^3; a queue lists its graphs' tasks by (step, graph), each with
-// the number of tasks of its graph in earlier steps.  The schedules are the
-// plain ones with the tile capped at 64 (kappa scaled to keep the stage
-// width, at most 16) and rest blocks of at most 64.
-static int build_queues(fwx_batch_plan* p, const std::vector<int32_t>& nsz, const std::vector<int64_t>& noff) {
-  int occ = 0, cus = 0, dev = 0;
+// Estimated microseconds of one task on one of two blocks sharing a CU, to
+// order each step's units longest first: 128-wide products at ~6e10
+// relaxations/s per block plus a fixed cost, 128-wide closures by their
+// measured latency-bound time (128 dependent steps).
+static double coop_cost(const CoopTask& t) {
+  if (t.kind == kDiag128) return 57.0;
+  return 3.0 + 128.0 * 128.0 * t.depth / 6.0e4;
+}
+
+static CoopTask coop_prod(int kind, const ProdItem& it) {
+  return CoopTask{it.base, it.n, (int16_t)kind, (int16_t)it.vec, it.cr, it.cc, it.ar, it.ac, it.br, it.bc, it.depth};
+}
+
+static CoopTask coop_diag(int kind, const DiagItem& d) {
+  return CoopTask{d.base, d.n, (int16_t)kind, 0, d.c0, d.c0, d.c0, d.c0, d.c0, d.c0, 0};
+}
+
+// The persistent plan: every graph's steps (coop_config, side branches as in
+// the step plan), merged by step.  A unit is one task, or a side branch's
+// product followed by its closure (kept in order in one block).  Each step's
+// units are stored longest first, for the blocks to claim in that order.
+// `order` lists the graphs largest first.  cgrid is the number of blocks
+// that fit on the device at once (the cooperative grid); it stays 0 when the
+// device has no cooperative launch (the persistent modes then return
+// FWX_ERR_UNSUPPORTED).
+static int build_coop(fwx_batch_plan* p, const int32_t* sizes, const int64_t* offsets,
+                      const std::vector<int32_t>& order) {
+  int dev = 0, coop = 0, cus = 0;
   FWX_HIP(hipGetDevice(&dev));
-  FWX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)k_batch_persistent<true>, 256, 0));
+  FWX_HIP(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev));
+  if (!coop) return FWX_OK;
   FWX_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-  int khz = 0;
-  FWX_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
-  p->wall_khz = khz > 0 ? khz : 100000;
-  p->pgrid = std::max(1, occ) * std::max(1, cus);
-  std::vector<int32_t> xcc_q;
-  if (int rc = probe_xcc(p->pgrid, xcc_q, &p->nq)) return rc;
-  if (p->nq == 0) return FWX_OK;  // waves of one block reported different XCCs
-  const int count = (int)nsz.size();
-  std::vector<std::vector<std::vector<std::pair<int, ProdItem>>>> P(count);
-  std::vector<std::vector<std::vector<std::pair<int, DiagItem>>>> Dg(count);
+  int o1 = 0, o0 = 0;
+  FWX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&o1, reinterpret_cast<const void*>(&k_batch_coop<true>), 512, 0));
+  FWX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&o0, reinterpret_cast<const void*>(&k_batch_coop<false>), 512, 0));
+  const int G = std::min(o0, o1) * cus;
+  if (G <= 0) return FWX_OK;
+  const int32_t count = (int32_t)order.size();
+  std::vector<ProdSteps> P(count), SP(count);
+  std::vector<DiagSteps> Dg(count), SD(count);
   int32_t steps = 0;
-  for (int q = 0; q < count; ++q) {
+  for (int32_t q = 0; q < count; ++q) {
+    const int32_t g = order[q];
     int32_t T, kappa;
-    tuned_config(nsz[q], &T, &kappa);
-    if (T > 64) {
-      kappa = std::min(16, kappa * (T / 64));
-      T = 64;
-    }
-    schedule(noff[q], nsz[q], T, kappa, P[q], Dg[q], 64);
+    coop_config(sizes[g], &T, &kappa);
+    schedule(offsets[g], sizes[g], T, kappa, P[q], Dg[q]);
+    hoist_closures(P[q], Dg[q], SP[q], SD[q], T);
     steps = std::max<int32_t>(steps, (int32_t)P[q].size());
   }
-  std::vector<int> qof(count);
-  std::vector<double> load(p->nq, 0.0);
-  for (int q = 0; q < count; ++q) {
-    int best = 0;
-    for (int k = 1; k < p->nq; ++k)
-      if (load[k] < load[best]) best = k;
-    qof[q] = best;
-    load[best] += (double)nsz[q] * nsz[q] * nsz[q];
-  }
-  std::vector<Task> tasks;
-  std::vector<int32_t> qoff(p->nq + 1, 0);
-  std::vector<int32_t> prefix(count, 0);
-  for (int k = 0; k < p->nq; ++k) {
-    qoff[k] = (int32_t)tasks.size();
-    for (int32_t j = 0; j < steps; ++j) {
-      for (int q = 0; q < count; ++q) {
-        if (qof[q] != k || j >= (int32_t)P[q].size()) continue;
-        const int32_t need = prefix[q];
-        for (const auto& kv : Dg[q][j]) {
-          if (kv.first != kDiag32 && kv.first != kDiag64) return fail(FWX_ERR_INVALID_ARG, "persistent plan: tile above 64");
-          const DiagItem& d = kv.second;
-          tasks.push_back(Task{d.base, d.n, (int16_t)kv.first, 0, d.c0, d.c0, d.c0, 0, q, need});
-        }
-        for (const auto& kv : P[q][j]) {
-          if (kv.first != kProd32 && kv.first != kProd64) return fail(FWX_ERR_INVALID_ARG, "persistent plan: block above 64");
-          const ProdItem& it = kv.second;
-          tasks.push_back(Task{it.base, it.n, (int16_t)kv.first, (int16_t)it.vec, it.cr, it.cc, it.ac, it.depth, q, need});
-        }
-        prefix[q] += (int32_t)(Dg[q][j].size() + P[q][j].size());
+  std::vector<CoopTask> tasks;
+  std::vector<int32_t> ufirst, sfirst;
+  for (int32_t j = 0; j < steps; ++j) {
+    std::vector<std::vector<CoopTask>> units;
+    for (int32_t q = 0; q < count; ++q) {
+      if (j >= (int32_t)P[q].size()) continue;
+      if (!SP[q][j].empty() || !SD[q][j].empty()) {  // side branch: its products, then its closures
+        std::vector<CoopTask> u;
+        for (auto& kv : SP[q][j]) u.push_back(coop_prod(kv.first, kv.second));
+        for (auto& kv : SD[q][j]) u.push_back(coop_diag(kv.first, kv.second));
+        units.push_back(u);
       }
+      for (auto& kv : Dg[q][j]) units.push_back({coop_diag(kv.first, kv.second)});
+      for (auto& kv : P[q][j]) units.push_back({coop_prod(kv.first, kv.second)});
     }
+    std::vector<double> cost(units.size(), 0.0);
+    for (size_t u = 0; u < units.size(); ++u)
+      for (const CoopTask& t : units[u]) {
+        if (t.kind != kDiag128 && t.kind != kProd128)
+          return fail(FWX_ERR_INVALID_ARG, "persistent plan: a task that is not 128 wide");
+        cost[u] += coop_cost(t);
+      }
+    std::vector<size_t> idx(units.size());
+    for (size_t u = 0; u < idx.size(); ++u) idx[u] = u;
+    std::stable_sort(idx.begin(), idx.end(), [&](size_t a, size_t b) { return cost[a] > cost[b]; });
+    sfirst.push_back((int32_t)ufirst.size());
+    for (size_t u : idx) {
+      ufirst.push_back((int32_t)tasks.size());
+      tasks.insert(tasks.end(), units[u].begin(), units[u].end());
+    }
+    if (tasks.size() > (size_t)INT32_MAX / 2) return fail(FWX_ERR_INVALID_ARG, "persistent plan: too many tasks");
   }
-  qoff[p->nq] = (int32_t)tasks.size();
-  p->ntasks = (int64_t)tasks.size();
-  p->count = count;
-  if (int rc = upload(&p->d_tasks, tasks.data(), tasks.size())) return rc;
-  if (int rc = upload(&p->d_qoff, qoff.data(), qoff.size())) return rc;
-  if (int rc = upload(&p->d_xccq, xcc_q.data(), xcc_q.size())) return rc;
-  FWX_HIP(hipMalloc(reinterpret_cast<void**>(&p->d_done), (size_t)std::max(1, count) * kLine * sizeof(int32_t)));
-  FWX_HIP(hipMalloc(reinterpret_cast<void**>(&p->d_ctl), kMaxQueues * sizeof(BatchCtl)));
+  sfirst.push_back((int32_t)ufirst.size());
+  ufirst.push_back((int32_t)tasks.size());
+  if (int rc = upload(&p->d_ctasks, tasks.data(), tasks.size())) return rc;
+  if (int rc = upload(&p->d_ufirst, ufirst.data(), ufirst.size())) return rc;
+  if (int rc = upload(&p->d_sfirst, sfirst.data(), sfirst.size())) return rc;
+  FWX_HIP(hipMalloc(reinterpret_cast<void**>(&p->d_ctr), (size_t)std::max(1, steps) * sizeof(int32_t)));
+  FWX_HIP(hipMalloc(reinterpret_cast<void**>(&p->d_status), sizeof(int32_t)));
+  p->ctasks = (int64_t)tasks.size();
+  p->csteps = steps;
+  p->cgrid = G;
   return FWX_OK;
 }
 
@@ -626,11 +577,11 @@ static void destroy_plan(fwx_batch_plan* p) {
     if (p->pexec[m]) (void)hipGraphExecDestroy(p->pexec[m]);
     if (p->pgraph[m]) (void)hipGraphDestroy(p->pgraph[m]);
   }
-  if (p->d_tasks) (void)hipFree(p->d_tasks);
-  if (p->d_qoff) (void)hipFree(p->d_qoff);
-  if (p->d_xccq) (void)hipFree(p->d_xccq);
-  if (p->d_done) (void)hipFree(p->d_done);
-  if (p->d_ctl) (void)hipFree(p->d_ctl);
+  if (p->d_ctasks) (void)hipFree(p->d_ctasks);
+  if (p->d_ufirst) (void)hipFree(p->d_ufirst);
+  if (p->d_sfirst) (void)hipFree(p->d_sfirst);
+  if (p->d_ctr) (void)hipFree(p->d_ctr);
+  if (p->d_status) (void)hipFree(p->d_status);
   delete p;
 }
 
@@ -745,16 +696,10 @@ int fwx_batch_plan_create(fwx_batch_plan** out, const int32_t* sizes, const int6
       p->items += (int64_t)diag[k].size();
     }
   }
-  std::vector<int32_t> nsz(count);
-  std::vector<int64_t> noff(count);
-  for (int32_t q = 0; q < count; ++q) {
-    nsz[q] = sizes[order[q]];
-    noff[q] = offsets[order[q]];
+  if (int rc = build_coop(p, sizes, offsets, order)) {
+    destroy_plan(p);
+    return rc;
   }
-  // persistent mode switched off (fwx_batch_plan_run): no queues, no probe
-  (void)nsz;
-  (void)noff;
-  (void)build_queues;
   *out = p;
   return FWX_OK;
 }
@@ -769,11 +714,8 @@ int fwx_batch_plan_run(fwx_batch_plan* p, int32_t mode) {
   const int m = pk ? 1 : 0;
   const bool persistent = mode == FWX_BATCH_PERSISTENT || mode == FWX_BATCH_PERSISTENT_GRAPH;
   const bool use_graph = mode == FWX_BATCH_GRAPH || mode == FWX_BATCH_PERSISTENT_GRAPH;
-  // The persistent kernel is switched off: on the GPU its first runs did not
-  // complete (a run held the device past the test's time limit although
-  // every dependency wait is bounded), and the cause is not established.
-  if (persistent)
-    return fail(FWX_ERR_UNSUPPORTED, "persistent batch mode is switched off in this build");
+  if (persistent && p->cgrid <= 0)
+    return fail(FWX_ERR_UNSUPPORTED, "persistent batch mode needs a cooperative launch, which this device lacks");
   hipGraph_t* graph = persistent ? &p->pgraph[m] : &p->graph[m];
   hipGraphExec_t* exec = persistent ? &p->pexec[m] : &p->exec[m];
   auto enqueue = [&](fwx_batch_plan* pl) { return persistent ? enqueue_persistent_mode(pl, pk) : enqueue_mode(pl, pk); };
@@ -795,6 +737,11 @@ int fwx_batch_plan_run(fwx_batch_plan* p, int32_t mode) {
       if (e != hipSuccess) return hip_fail(e, "hipStreamBeginCapture", __FILE__, __LINE__);
       if (rc) {
         if (g) (void)hipGraphDestroy(g);
+        if (persistent) {
+          (void)hipGetLastError();
+          return fail(FWX_ERR_UNSUPPORTED, std::string("the cooperative launch could not be captured into a "
+                                                       "hipGraph on this runtime: ") + fwx_last_error());
+        }
         return rc;
       }
       if (e2 != hipSuccess) return hip_fail(e2, "hipStreamEndCapture", __FILE__, __LINE__);
@@ -806,46 +753,24 @@ int fwx_batch_plan_run(fwx_batch_plan* p, int32_t mode) {
     if (int rc = enqueue(p)) return rc;
   }
   if (persistent) {
-    BatchCtl h[kMaxQueues];
-    FWX_HIP(hipMemcpyAsync(h, p->d_ctl, sizeof(h), hipMemcpyDeviceToHost, p->stream));
+    int32_t status = 0;
+    FWX_HIP(hipMemcpyAsync(&status, p->d_status, sizeof(status), hipMemcpyDeviceToHost, p->stream));
     FWX_HIP(hipStreamSynchronize(p->stream));
-    int64_t finished = 0;
-    int error = 0;
-    for (int q = 0; q < p->nq; ++q) {
-      finished += h[q].finished;
-      error |= h[q].error;
-    }
-    if (error) return fail(FWX_ERR_TIMEOUT, "batched persistent run: a dependency wait exceeded its limit");
-    if (finished != p->ntasks)
-      return fail(FWX_ERR_TIMEOUT, "batched persistent run: not every task ran (an XCD queue got no workgroup)");
+    if (status)
+      return fail(FWX_ERR_UNSUPPORTED, "the persistent kernel ran without a grid barrier (not a cooperative "
+                                       "launch); nothing was computed");
     return FWX_OK;
   }
   FWX_HIP(hipStreamSynchronize(p->stream));
   return FWX_OK;
 }
 
-int fwx_batch_plan_set_wait_limit(fwx_batch_plan* p, double seconds) {
-  clear_error();
-  if (!p || !(seconds >= 0.0)) return fail(FWX_ERR_INVALID_ARG, "null plan or negative limit");
-  if (seconds != p->wait_s) {
-    // the limit is a kernel argument: drop captured persistent graphs
-    for (int m = 0; m < 2; ++m) {
-      if (p->pexec[m]) (void)hipGraphExecDestroy(p->pexec[m]);
-      if (p->pgraph[m]) (void)hipGraphDestroy(p->pgraph[m]);
-      p->pexec[m] = nullptr;
-      p->pgraph[m] = nullptr;
-    }
-  }
-  p->wait_s = seconds;
-  return FWX_OK;
-}
-
-int fwx_batch_plan_queues(const fwx_batch_plan* p, int32_t* queues, int32_t* grid, int64_t* tasks) {
+int fwx_batch_plan_persistent_info(const fwx_batch_plan* p, int32_t* grid, int32_t* steps, int64_t* tasks) {
   clear_error();
   if (!p) return fail(FWX_ERR_INVALID_ARG, "null plan");
-  if (queues) *queues = p->nq;
-  if (grid) *grid = p->pgrid;
-  if (tasks) *tasks = p->ntasks;
+  if (grid) *grid = p->cgrid;
+  if (steps) *steps = p->csteps;
+  if (tasks) *tasks = p->ctasks;
   return FWX_OK;
 }
 

@@ -60,6 +60,20 @@ __device__ __forceinline__ T* at_bytes(T* base, uint32_t bytes) {
   return reinterpret_cast<T*>(reinterpret_cast<C*>(base) + bytes);
 }
 
+// The thread's index within its block.  OPAQUE reads it through an empty
+// volatile asm at each use site: values derived from it are then recomputed
+// where they are needed instead of being treated as invariant across an
+// enclosing loop.  The persistent batch kernel runs many products in a loop;
+// with the plain index the compiler hoisted every per-thread offset of the
+// product out of that loop, kept them live across it and spilled VGPRs
+// inside the product's inner loops.
+template <bool OPAQUE>
+__device__ __forceinline__ int block_tid() {
+  uint32_t t = threadIdx.x;
+  if (OPAQUE) asm volatile("" : "+v"(t));
+  return (int)t;
+}
+
 // LDS staging of one product: two buffers of an A chunk (row-major, row
 // stride KC+4) and a B chunk (pair-interleaved).
 template <int BM, int BN, int KC>
@@ -74,7 +88,9 @@ struct TileSmem {
 // EDGE=false: the caller guarantees that the C block and every operand cell
 // of the whole depth lie inside the matrix (tile_update checks it), so the
 // clamping and masking code is compiled out.
-template <int BM, int BN, int KC, bool VEC, int NTX, bool PK64, bool EDGE = true>
+// OPQ: the thread index is read with block_tid<true> (for products run in a
+// loop of a persistent kernel).
+template <int BM, int BN, int KC, bool VEC, int NTX, bool PK64, bool EDGE = true, bool OPQ = false>
 struct TileProduct {
   static constexpr int NT = NTX * 16;
   static constexpr int NW = NT / 64;  // waves
@@ -114,7 +130,7 @@ struct TileProduct {
   // is wholly inside the matrix (the common case), every address is a
   // uniform row base (SGPRs) plus one per-thread offset plus an immediate.
   __device__ __forceinline__ void fetch(const View& A, const View& B, int32_t kc0) {
-    const int tid = threadIdx.x;
+    const int tid = block_tid<OPQ>();
     static_assert(NT % (BN / 4) == 0, "B staging rows must be uniform per pass");
     constexpr int BROWS = NT / (BN / 4);  // k pairs per B pass
     full_ = !EDGE || (A.rlim >= BM && A.clim >= kc0 + KC && B.rlim >= kc0 + KC && B.clim >= BN);
@@ -204,7 +220,7 @@ struct TileProduct {
 
   // registers -> LDS buffer `buf`.  Cells that fetch() clamped become INF.
   __device__ __forceinline__ void stash(Smem& s, int buf, const View& A, const View& B) {
-    const int tid = threadIdx.x;
+    const int tid = block_tid<OPQ>();
     if (!full_) {
 #pragma unroll
       for (int p = 0; p < APASS; ++p) {
@@ -259,8 +275,12 @@ struct TileProduct {
   }
 
   __device__ __forceinline__ static void compute(const Smem& s, int buf, int32_t (&acc)[TM][TN]) {
-    const int tx = threadIdx.x % NTX;
-    const int ty = threadIdx.x / NTX;
+    compute(s, buf, acc, block_tid<OPQ>());
+  }
+
+  __device__ __forceinline__ static void compute(const Smem& s, int buf, int32_t (&acc)[TM][TN], int tid) {
+    const int tx = tid % NTX;
+    const int ty = tid / NTX;
     const int32_t* As = s.a[buf];
     const uint64_t* Bs = reinterpret_cast<const uint64_t*>(s.b[buf]);
 #pragma unroll FWX_QUNROLL
@@ -303,7 +323,7 @@ struct TileProduct {
     for (int ch = 0; ch < nch; ++ch) {
       const int buf = ch & 1;
       if (ch + 1 < nch) fetch(A, B, (ch + 1) * KC);
-      compute(s, buf, acc);
+      compute(s, buf, acc, block_tid<OPQ>());
       if (ch + 1 < nch) stash(s, buf ^ 1, A, B);
       __syncthreads();
     }
@@ -313,9 +333,9 @@ struct TileProduct {
     return !EDGE || (C.rlim >= BM && C.clim >= BN);
   }
 
-  __device__ __forceinline__ static void load_c(const View& C, int32_t (&acc)[TM][TN]) {
-    const int tx = threadIdx.x % NTX;
-    const int ty = threadIdx.x / NTX;
+  __device__ __forceinline__ static void load_c(const View& C, int32_t (&acc)[TM][TN], int tid) {
+    const int tx = tid % NTX;
+    const int ty = tid / NTX;
     if (full(C)) {
       const uint32_t oc = (uint32_t)(ty * C.ld + G * tx) * 4u;
 #pragma unroll
@@ -364,20 +384,20 @@ struct TileProduct {
 // C block and operands lie wholly inside the matrix (every block of an
 // n % BM == 0 matrix) run the edge-free instantiation, the others the
 // general one; the branch is uniform across the block.
-template <int BM, int BN, int KC, bool VEC, int NTX, bool PK64>
+template <int BM, int BN, int KC, bool VEC, int NTX, bool PK64, bool OPQ = false>
 __device__ __forceinline__ void tile_update(TileSmem<BM, BN, KC>& s, const View& A, const View& B,
                                             const View& C, int32_t depth) {
-  using PF = TileProduct<BM, BN, KC, VEC, NTX, PK64, false>;
-  using PE = TileProduct<BM, BN, KC, VEC, NTX, PK64, true>;
+  using PF = TileProduct<BM, BN, KC, VEC, NTX, PK64, false, OPQ>;
+  using PE = TileProduct<BM, BN, KC, VEC, NTX, PK64, true, OPQ>;
   int32_t acc[PF::TM][PF::TN];
   if (C.rlim >= BM && C.clim >= BN && A.rlim >= BM && A.clim >= depth && B.rlim >= depth &&
       B.clim >= BN) {
-    PF::load_c(C, acc);
+    PF::load_c(C, acc, block_tid<OPQ>());
     PF p;
     p.run(s, A, B, depth, acc);
     PF::store_c(C, acc);
   } else {
-    PE::load_c(C, acc);
+    PE::load_c(C, acc, block_tid<OPQ>());
     PE p;
     p.run(s, A, B, depth, acc);
     PE::store_c(C, acc);

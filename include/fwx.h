@@ -210,15 +210,16 @@ int fwx_certify_rows(const int32_t* d_W, int32_t n, const int32_t* d_src, int32_
  *   FWX_BATCH_DIRECT (0)   the step launches enqueued directly;
  *   FWX_BATCH_GRAPH (1)    the step launches captured into a hipGraph on first
  *                          use and replayed;
- *   FWX_BATCH_PERSISTENT (2) ONE persistent kernel: per-XCD task lists in
- *                          dependency order, per-graph completion counters,
- *                          every wait bounded (fwx_batch_plan_set_wait_limit,
- *                          default 10 s; exceeding it returns FWX_ERR_TIMEOUT
- *                          and leaves the batch's contents undefined);
- *   FWX_BATCH_PERSISTENT_GRAPH (3) the persistent launch (plus its counter
- *                          reset) captured into a hipGraph and replayed.
- * The two persistent modes are switched off in this build and return
- * FWX_ERR_UNSUPPORTED (their first GPU runs did not complete).
+ *   FWX_BATCH_PERSISTENT (2) ONE persistent cooperative kernel: a grid of
+ *                          resident 512-thread blocks walks the same steps
+ *                          (every graph on 128-wide tiles: graphs below
+ *                          n = 128 are one closure), each block running the
+ *                          tasks the plan assigned it, with a grid-wide
+ *                          barrier between steps;
+ *   FWX_BATCH_PERSISTENT_GRAPH (3) that one launch captured into a hipGraph
+ *                          and replayed.
+ * The persistent modes return FWX_ERR_UNSUPPORTED on a device without
+ * cooperative launches, or if the runtime cannot capture one into a graph.
  * Results equal a plain fwx_multi_layer_tiling_gpu call per graph in every
  * mode.  The plan owns device memory (its work lists), a side stream and two
  * events until fwx_batch_plan_destroy; side branches fork from the plan's
@@ -234,10 +235,9 @@ typedef struct fwx_batch_plan fwx_batch_plan;
 int fwx_batch_plan_create(fwx_batch_plan** plan, const int32_t* sizes, const int64_t* offsets,
                           int32_t count, int32_t* d_data, void* stream);
 int fwx_batch_plan_run(fwx_batch_plan* plan, int32_t mode);
-/* Limit of one device-side dependency wait of the persistent mode. */
-int fwx_batch_plan_set_wait_limit(fwx_batch_plan* plan, double seconds);
-/* Persistent mode: task queues (one per XCD the probe saw), grid, tasks. */
-int fwx_batch_plan_queues(const fwx_batch_plan* plan, int32_t* queues, int32_t* grid, int64_t* tasks);
+/* Persistent mode: blocks of the cooperative grid (0 when the device has no
+ * cooperative launch), grid-wide steps, and tasks. */
+int fwx_batch_plan_persistent_info(const fwx_batch_plan* plan, int32_t* grid, int32_t* steps, int64_t* tasks);
 /* Dependent steps, kernel launches per run, and work items of a plan. */
 int fwx_batch_plan_stats(const fwx_batch_plan* plan, int32_t* steps, int32_t* launches, int64_t* items);
 int fwx_batch_plan_destroy(fwx_batch_plan* plan);

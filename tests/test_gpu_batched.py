@@ -94,9 +94,9 @@ def test_batch_small_mixed(monkeypatch, deep):
     check_all(got, expect, flat, flat0, sizes, offs)
 
 
-PERSISTENT_OFF = ("persistent batch kernel switched off: its first two GPU runs did not complete "
-                  "(the run held the device past the test time limit); cause not established, "
-                  "fwx_batch_plan_run returns FWX_ERR_UNSUPPORTED for the persistent modes")
+PERSISTENT_OFF = ("exercised the per-XCD dependency-counter persistent kernel, which was removed after its "
+                  "first two GPU runs did not complete (cause not established); the persistent modes now run "
+                  "the cooperative kernel that test_batch_cooperative_* check")
 
 
 @pytest.mark.skip(reason=PERSISTENT_OFF)
@@ -123,18 +123,81 @@ def test_batch_persistent_modes():
     plan.destroy()
 
 
-def test_batch_persistent_switched_off():
-    """While switched off, the persistent modes fail cleanly (no launch)."""
+def _mixed_for_cooperative():
+    rng = np.random.default_rng(5)
+    ns = [1, 2, 3, 5, 17, 31, 32, 33, 63, 64, 65, 100, 127, 128, 129, 200, 255, 256, 257, 300, 399, 513, 640,
+          700, 1024]
+    ns = ns + list(rng.integers(1, 600, 15))
+    dens = rng.uniform(0.05, 0.9, len(ns))
+    seeds = list(rng.integers(1, 2**31, len(ns)))
+    return make_batch(ns, dens, seeds, neg={3, 9, 14, 20, 26})
+
+
+def test_batch_cooperative_direct():
+    """One cooperative launch solves a mixed batch (ragged sizes from 1 to
+    1024, graphs below n = 128 as one closure, negative weights in some
+    graphs: the min3 form) bit-exactly against the oracle, twice on restored
+    input, and again after a run in the pk64 form; bytes between graphs stay
+    untouched."""
+    import torch
     from fw_cuda_amd import _lib
-    graphs = make_batch([40, 70], [0.5, 0.5], [1, 2])
+    graphs = _mixed_for_cooperative()
+    expect = [po.fw_rowpar(g) for g in graphs]
     sizes, offs, t, flat0 = upload(graphs)
     plan = fx.BatchPlan(sizes, offs, _lib.ptr(t))
-    for mode in ("persistent", "persistent_graph"):
-        with pytest.raises(_lib.FwxError) as e:
-            plan.run(mode)
-        assert e.value.code == _lib.ERR_UNSUPPORTED
-    plan.run("graph")
+    info = plan.persistent_info()
+    say("cooperative plan", info, plan.stats())
+    assert info["grid"] > 0 and info["steps"] > 0 and info["tasks"] > 0
+    for rep in range(2):
+        t.copy_(torch.from_numpy(flat0).to("cuda:0"))
+        say("cooperative run", rep, "(min3 form: negative weights present)")
+        plan.run("persistent")
+        got, flat = unpack(t, sizes, offs)
+        check_all(got, expect, flat, flat0, sizes, offs)
     plan.destroy()
+    # non-negative weights only: the pk64 form
+    pos = [np.where(g < 0, 1, g).astype(np.int32) for g in graphs]
+    for g in pos:
+        np.fill_diagonal(g, 0)
+    expect = [po.fw_rowpar(g) for g in pos]
+    sizes, offs, t, flat0 = upload(pos)
+    plan = fx.BatchPlan(sizes, offs, _lib.ptr(t))
+    say("cooperative run (pk64 form)")
+    plan.run("persistent")
+    got, flat = unpack(t, sizes, offs)
+    check_all(got, expect, flat, flat0, sizes, offs)
+    plan.destroy()
+
+
+def test_batch_cooperative_graph():
+    """The cooperative launch captured into a hipGraph and replayed twice on
+    restored input: bit-exact against the oracle.  A runtime that cannot
+    capture a cooperative launch makes the mode return FWX_ERR_UNSUPPORTED
+    (checked, then the test is skipped with that message)."""
+    import torch
+    from fw_cuda_amd import _lib
+    graphs = _mixed_for_cooperative()
+    expect = [po.fw_rowpar(g) for g in graphs]
+    sizes, offs, t, flat0 = upload(graphs)
+    plan = fx.BatchPlan(sizes, offs, _lib.ptr(t))
+    try:
+        for rep in range(2):
+            t.copy_(torch.from_numpy(flat0).to("cuda:0"))
+            say("cooperative graph replay", rep)
+            try:
+                plan.run("persistent_graph")
+            except _lib.FwxError as e:
+                assert e.code == _lib.ERR_UNSUPPORTED, e
+                pytest.skip(f"cooperative launch not capturable here: {e}")
+            got, flat = unpack(t, sizes, offs)
+            check_all(got, expect, flat, flat0, sizes, offs)
+        # the step-launch replay of the same plan afterwards is unaffected
+        t.copy_(torch.from_numpy(flat0).to("cuda:0"))
+        plan.run("graph")
+        got, flat = unpack(t, sizes, offs)
+        check_all(got, expect, flat, flat0, sizes, offs)
+    finally:
+        plan.destroy()
 
 
 @pytest.mark.skip(reason=PERSISTENT_OFF)
@@ -167,7 +230,9 @@ def test_batch_persistent_wait_limit():
 @pytest.mark.slow
 def test_batch_512_mixed_sizes():
     """512 graphs, n log-uniform in [64, 2048] (seed 11), density U[0.05, 0.9],
-    weights U[1, 1000]: one hipGraph replay, every graph bit-exact."""
+    weights U[1, 1000]: one hipGraph replay of the step launches, every graph
+    bit-exact against the oracle; a second replay and the persistent
+    cooperative launch (from its hipGraph) give the same matrices."""
     import torch
     from fw_cuda_amd import _lib
     ns, dens, seeds = fx.mixed_batch()
@@ -179,8 +244,9 @@ def test_batch_512_mixed_sizes():
     say("512 graphs: gpu done; oracle", plan.stats())
     for g in range(len(graphs)):
         assert (got[g] == po.fw_rowpar(graphs[g])).all(), g
-    # a second replay on restored input gives the same matrices
-    for mode in ("graph",):
+    # a second replay on restored input, and the one persistent cooperative
+    # launch from its hipGraph, give the same matrices
+    for mode in ("graph", "persistent_graph"):
         t.copy_(torch.from_numpy(flat0).to("cuda:0"))
         say("512 graphs", mode)
         plan.run(mode)

@@ -168,8 +168,7 @@ def test_batched_and_diff_argument_errors():
     assert L.fwx_certify_rows(dummy, 32770, dummy, 1, dummy, dummy, ctypes.byref(r), None) == _lib.ERR_INVALID_ARG
     assert L.fwx_certify_rows(dummy, 4, dummy, 1, dummy, dummy, None, None) == _lib.ERR_INVALID_ARG
     assert L.fwx_batch_plan_run(None, 3) == _lib.ERR_INVALID_ARG
-    assert L.fwx_batch_plan_set_wait_limit(None, 1.0) == _lib.ERR_INVALID_ARG
-    assert L.fwx_batch_plan_queues(None, None, None, None) == _lib.ERR_INVALID_ARG
+    assert L.fwx_batch_plan_persistent_info(None, None, None, None) == _lib.ERR_INVALID_ARG
     assert L.fwx_batch_loop(_lib.ptr(sizes), _lib.ptr(offs), 0, dummy, None) == _lib.ERR_INVALID_ARG
     assert L.fwx_diff_forward(dummy, dummy, None, 0, 1, None) == _lib.ERR_INVALID_ARG
     assert L.fwx_diff_forward(dummy, dummy, None, 8193, 1, None) == _lib.ERR_INVALID_ARG

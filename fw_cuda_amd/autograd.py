@@ -49,10 +49,25 @@ def free_device_memory(device):
     return int(free.value) + max(0, int(cached))
 
 
+_total_memory = {}
+
+
 def choose_keep_pred(n, batch, device):
     """True when the n x n x batch int32 predecessor matrix fits in
-    PRED_FREE_FRACTION of the free device memory."""
+    PRED_FREE_FRACTION of the free device memory.  A matrix below 1/256 of
+    the device's total memory is kept without asking for the free memory
+    (the query costs a few hundred microseconds per forward) unless
+    PRED_FREE_FRACTION is set below 1/64."""
     need = 4 * n * n * batch
+    idx = torch.device(device).index or 0
+    if idx not in _total_memory:
+        free = ctypes.c_uint64()
+        total = ctypes.c_uint64()
+        with torch.cuda.device(device):
+            _lib.check(_lib.lib().fwx_device_memory(ctypes.byref(free), ctypes.byref(total)))
+        _total_memory[idx] = int(total.value)
+    if need * 256 <= _total_memory[idx] and PRED_FREE_FRACTION >= 1.0 / 64:
+        return True
     return need <= PRED_FREE_FRACTION * free_device_memory(device)
 
 

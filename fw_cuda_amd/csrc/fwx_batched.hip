@@ -56,8 +56,19 @@ struct DiagItem {
   int32_t c0;
 };
 
-// Work kinds: a diagonal closure or a tile product, by tile size.
-enum Kind { kDiag64 = 0, kProd64 = 1, kProd128 = 2, kProd32 = 3, kDiag32 = 4, kDiag128 = 5, kKinds = 6 };
+// Work kinds: a diagonal closure or a tile product, by tile size; 128-wide
+// products on 512 threads of 8x4 cells (kProd128) or 256 threads of 8x8
+// cells (kProd128w), the register micro-tile tuned per size class.
+enum Kind {
+  kDiag64 = 0,
+  kProd64 = 1,
+  kProd128 = 2,
+  kProd32 = 3,
+  kDiag32 = 4,
+  kDiag128 = 5,
+  kProd128w = 6,
+  kKinds = 7
+};
 
 template <int T, int NTH>
 __global__ __launch_bounds__(NTH) void k_batch_diag(int32_t* __restrict__ data,
@@ -204,33 +215,30 @@ struct Launch {
   bool side;      // on the side stream, beside the step's other launches
 };
 
-// (tile, kappa) of one graph inside a batch.  Graphs with n >= 128 take
-// (128, 3): 128-wide rest items of depth 384 on the 512-thread product and
-// 128-wide closures, with far fewer dependent steps than the plain path's
-// 64-wide entries for these sizes (tuned for one graph alone, where
-// parallelism per launch matters more).  Measured on the 512-graph
-// configuration (n log-uniform in [64, 2048]): 18.3 ms per batch on the
-// tuned entries, 16.5 with (128, 2); with the closures on side branches
-// 15.84 for (128, 2), 15.77 for (128, 3), 15.88 for (128, 4).
-// Smaller graphs keep the plain path's tuned entry.  FWX_BATCH_DEEP="n0,T,
-// kappa" replaces the rule (graphs with n >= n0 take (T, kappa)); "off" uses
-// the tuned entries for every graph.
-void batch_config(int32_t n, int32_t* T, int32_t* kappa) {
-  tuned_config(n, T, kappa);
-  int n0 = 128, t = 128, k = 3;
+// (tile, kappa) of one graph inside a batch and the threads of its 128-wide
+// products, from the batched table the autotuner writes by timing whole
+// batches of the 512-graph mix (fwx_tune.cpp; FWX_BATCH_TABLE overrides it).
+// The hand rule it replaced gave graphs with n >= 128 (128, 3) on 512
+// threads and smaller ones the plain path's entries.  FWX_BATCH_DEEP="n0,T,
+// kappa" (graphs with n >= n0 take (T, kappa), the others the plain table's
+// entries) and "off" (the plain table for every graph), both on 512-thread
+// 128-wide products, override the table; the tests use them to cover other
+// plan shapes.
+void batch_config(int32_t n, int32_t* T, int32_t* kappa, int32_t* rest_threads) {
   if (const char* e = std::getenv("FWX_BATCH_DEEP")) {
+    tuned_config(n, T, kappa);
+    *rest_threads = 512;
     if (std::strcmp(e, "off") == 0) return;
     int a = 0, b = 0, c = 0;
     if (std::sscanf(e, "%d,%d,%d", &a, &b, &c) == 3 && (b == 32 || b == 64 || b == 128) && c >= 1 && c <= 16) {
-      n0 = a;
-      t = b;
-      k = c;
+      if (n >= a) {
+        *T = b;
+        *kappa = c;
+      }
+      return;
     }
   }
-  if (n >= n0) {
-    *T = t;
-    *kappa = k;
-  }
+  batch_tuned_config(n, T, kappa, rest_threads);
 }
 
 // FWX_BATCH_DIAG=256 closes 64-wide diagonal tiles with 256 threads
@@ -241,11 +249,13 @@ bool batch_diag_wide() {
 }
 
 int diag_kind(int T) { return T == 32 ? kDiag32 : T == 64 ? kDiag64 : kDiag128; }
-int prod_kind(int R) { return R == 32 ? kProd32 : R == 64 ? kProd64 : kProd128; }
+int prod_kind(int R, int rest_threads) {
+  return R == 32 ? kProd32 : R == 64 ? kProd64 : rest_threads == 256 ? kProd128w : kProd128;
+}
 
 // The schedule of one graph, step by step (mirrors run_blocked).
 void schedule(int64_t base, int32_t n, int32_t T, int32_t kappa, std::vector<std::vector<std::pair<int, ProdItem>>>& psteps,
-              std::vector<std::vector<std::pair<int, DiagItem>>>& dsteps, int maxR = 128) {
+              std::vector<std::vector<std::pair<int, DiagItem>>>& dsteps, int rest_threads = 512, int maxR = 128) {
   const bool vec = (n % 4 == 0) && (base % 4 == 0);
   const int N = (n + T - 1) / T;
   const int span = kappa * T;
@@ -256,7 +266,7 @@ void schedule(int64_t base, int32_t n, int32_t T, int32_t kappa, std::vector<std
   };
   auto prod = [&](int RR, int cr, int cc, int ar, int ac, int br, int bc, int depth) {
     ProdItem it{base, n, cr, cc, ar, ac, br, bc, depth, vec ? 1 : 0};
-    psteps.back().push_back({prod_kind(RR), it});
+    psteps.back().push_back({prod_kind(RR, rest_threads), it});
   };
   for (int kb = 0; kb < N; kb += kappa) {
     const int B = std::min(kappa, N - kb);
@@ -303,7 +313,7 @@ void schedule(int64_t base, int32_t n, int32_t T, int32_t kappa, std::vector<std
 using ProdSteps = std::vector<std::vector<std::pair<int, ProdItem>>>;
 using DiagSteps = std::vector<std::vector<std::pair<int, DiagItem>>>;
 
-int prod_block(int kind) { return kind == kProd32 ? 32 : kind == kProd64 ? 64 : 128; }
+int prod_block(int kind) { return kind == kProd32 ? 32 : kind == kProd64 ? 64 : 128; }  // kProd128, kProd128w
 
 // Side branches of one graph's schedule.  A closure step that follows a
 // product step needs, from that step, only the item whose C block holds the
@@ -421,6 +431,7 @@ static int enqueue_plan(fwx_batch_plan* p) {
       case kProd32: k_batch_prod<32, PK><<<L.count, 256, 0, st>>>(p->data, p->d_prod[kProd32] + L.first); break;
       case kProd64: k_batch_prod<64, PK><<<L.count, 256, 0, st>>>(p->data, p->d_prod[kProd64] + L.first); break;
       case kProd128: k_batch_prod512<PK><<<L.count, 512, 0, st>>>(p->data, p->d_prod[kProd128] + L.first); break;
+      case kProd128w: k_batch_prod<128, PK><<<L.count, 256, 0, st>>>(p->data, p->d_prod[kProd128w] + L.first); break;
       default: return fail(FWX_ERR_INVALID_ARG, "corrupt batch plan");
     }
     FWX_LAUNCHED("k_batch");
@@ -456,10 +467,12 @@ static int upload(T** dst, const T* src, size_t count) {
 }
 
 // (tile, kappa) of a graph in the persistent plan: 128-wide tiles for every
-// graph (the kernel's one product shape); graphs with n >= 128 keep the step
-// plan's kappa, smaller ones are one 128-wide closure.
+// graph (the kernel's one product shape, on 512 threads); graphs whose step
+// plan entry is 128 wide keep its kappa, the others are one 128-wide closure
+// per 128 x 128 diagonal block with kappa 1.
 static void coop_config(int32_t n, int32_t* T, int32_t* kappa) {
-  batch_config(n, T, kappa);
+  int32_t threads = 512;
+  batch_config(n, T, kappa, &threads);
   if (*T != 128) {
     *T = 128;
     *kappa = 1;
@@ -615,9 +628,9 @@ int fwx_batch_plan_create(fwx_batch_plan** out, const int32_t* sizes, const int6
   int32_t steps = 0;
   for (int32_t q = 0; q < count; ++q) {
     const int32_t g = order[q];
-    int32_t T, kappa;
-    batch_config(sizes[g], &T, &kappa);
-    schedule(offsets[g], sizes[g], T, kappa, P[q], Dg[q]);
+    int32_t T, kappa, threads;
+    batch_config(sizes[g], &T, &kappa, &threads);
+    schedule(offsets[g], sizes[g], T, kappa, P[q], Dg[q], threads);
     if (hoist) {
       hoist_closures(P[q], Dg[q], SP[q], SD[q], T);
     } else {

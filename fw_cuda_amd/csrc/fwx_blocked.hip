@@ -290,13 +290,14 @@ static int lead_round(int32_t* d, int32_t n, int32_t c, int32_t kb, int32_t B, i
   return FWX_OK;
 }
 
-// Shape of the 128-wide rest kernel (FWX_REST_VARIANT): 2 = 512 threads,
-// 8x4 cells each, 126 VGPRs, two blocks (4 waves/SIMD) per CU -- the default,
-// fastest measured; 0 = 256 threads, 8x8 cells, 32-deep chunks (2 waves/SIMD);
-// 1 = 256 threads with 16-deep chunks (3 waves/SIMD, twice the barriers).
-static int rest_variant() {
+// Shape of the 128-wide rest kernel -- the register micro-tile: 2 = 512
+// threads, 8x4 cells each, 126 VGPRs, two blocks (4 waves/SIMD) per CU; 0 =
+// 256 threads, 8x8 cells, 32-deep chunks (2 waves/SIMD); 1 = 256 threads with
+// 16-deep chunks (3 waves/SIMD, twice the barriers).  The tuned table picks
+// it per size class (fwx_tune.cpp); FWX_REST_VARIANT=0/1/2 overrides it.
+static int rest_variant(int32_t n) {
   const char* e = std::getenv("FWX_REST_VARIANT");
-  return (e && e[0] >= '0' && e[0] <= '2') ? e[0] - '0' : 2;
+  return (e && e[0] >= '0' && e[0] <= '2') ? e[0] - '0' : tuned_rest_variant(n);
 }
 
 // Rest tiles outside the R-tile band [s0/R, s0/R + excl): excl covers stage
@@ -310,7 +311,7 @@ static int rest_pass(int32_t* d, int32_t n, int32_t s0, int32_t S, int32_t excl,
   if (m <= 0) return FWX_OK;
   const int64_t total = (int64_t)m * m;
   const int per = (int)((total + 7) / 8);
-  const int v = R == 128 ? rest_variant() : 0;
+  const int v = R == 128 ? rest_variant(n) : 0;
   if (v == 1)
     k_rest<R, VEC, PK, 16><<<8 * per, 256, 0, st>>>(d, n, s0, S, sr0, srn, m, per);
   else if (v == 2)
@@ -652,7 +653,7 @@ int enqueue_blocked(int32_t* d, int32_t n, int32_t tile, int32_t kappa, hipStrea
   if (n > kGraphMaxN || !graphs_enabled() || profiling()) return enqueue_direct(d, n, tile, kappa, vec, pk, st);
   int dev = 0;
   FWX_HIP(hipGetDevice(&dev));
-  const int knobs = (fused_dpanel() ? 1 : 0) | (rest_variant() << 1) | (band_narrow() ? 8 : 0) | (diag_threads() << 4) |
+  const int knobs = (fused_dpanel() ? 1 : 0) | (rest_variant(n) << 1) | (band_narrow() ? 8 : 0) | (diag_threads() << 4) |
                     (lead_priority_enabled() ? 1 << 20 : 0);
   const GraphKey key{d, n, tile, kappa, vec, pk, lookahead_mode(), knobs, st, dev};
   std::lock_guard<std::mutex> lk(g_graph_mu);

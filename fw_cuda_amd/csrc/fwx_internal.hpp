@@ -74,7 +74,12 @@ int side_stream(hipStream_t* out);
 // Any D[i][i] < 0 on a device matrix (fwx_range.hip); synchronises `st`.
 int device_negative_cycle(const int32_t* d, int32_t n, hipStream_t st, bool* neg);
 
-// Tuned (tile, kappa) for a plain run of size n.
+// Tuned (tile, kappa) for a plain run of size n, and the shape of its
+// 128-wide rest kernel (0, 1 or 2, see fwx_blocked.hip).
 void tuned_config(int32_t n, int32_t* tile, int32_t* kappa);
+int tuned_rest_variant(int32_t n);
+// Tuned (tile, kappa) of a graph of size n inside a batch, and the threads of
+// its 128-wide products (512: 8x4 cells each; 256: 8x8).
+void batch_tuned_config(int32_t n, int32_t* tile, int32_t* kappa, int32_t* rest_threads);
 
 }  // namespace fwx

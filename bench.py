@@ -546,10 +546,22 @@ def main():
         "parity": check,
         "comparators": compare,
     }
-    # The two host-only legs (the CPU baseline on one core, and the
-    # single-threaded host generator of the n=32768 input) run on threads of
-    # their own beside the GPU legs below; both spend their time inside one
-    # ctypes call, which releases the interpreter lock.
+    # The legs bound by launch latency or host work run first, alone: the
+    # differentiable leg, the n=1024 leg and the input pipeline (whose host
+    # generator it times).  Then the two host-only legs (the CPU baseline on
+    # one core, and the single-threaded host generator of the n=32768 input)
+    # start on threads of their own beside the batched leg, which is bound by
+    # GPU throughput; both spend their time inside one ctypes call, which
+    # releases the interpreter lock.  (Run beside the host threads, the
+    # differentiable leg measured 10 % slower.)
+    if not a.no_diff:
+        out["differentiable"] = bench_diff(3)
+        print("bench: differentiable leg done", file=sys.stderr, flush=True)
+    if not a.no_n1024:
+        out["n1024"] = bench_n1024()
+        print("bench: n=1024 leg done", file=sys.stderr, flush=True)
+    if not a.no_pipeline:
+        out["input_pipeline"] = bench_pipeline(n)
     import concurrent.futures as cf
     pool = cf.ThreadPoolExecutor(max_workers=2)
     cpu_job = None if a.no_cpu else pool.submit(cpu_baseline, a.cpu_n, a.cpu_n_warm)
@@ -557,14 +569,6 @@ def main():
     if not a.no_batched:
         out["batched"] = bench_batched(3)
         print("bench: batched leg done", file=sys.stderr, flush=True)
-    if not a.no_diff:
-        out["differentiable"] = bench_diff(3)
-        print("bench: differentiable leg done", file=sys.stderr, flush=True)
-    if not a.no_pipeline:
-        out["input_pipeline"] = bench_pipeline(n)
-    if not a.no_n1024:
-        out["n1024"] = bench_n1024()
-        print("bench: n=1024 leg done", file=sys.stderr, flush=True)
     if gen_job is not None:
         h32, gen_s = gen_job.result()
         out["n32768"] = bench_n32768(h32, gen_s)

@@ -5,9 +5,11 @@
 // One workgroup of NTX x 16 threads (256 or 512: one or two wave64s per
 // SIMD) owns the C block in VGPRs: thread (tx, ty) = (tid % NTX, tid / NTX)
 // holds cells in rows ty + 16 r and in groups of up to four adjacent columns
-// (colmap below).  The LDS reads are conflict-free: a wave's tx lanes read
-// consecutive 64-bit B pairs and its ty rows hit distinct 16-byte bank groups
-// of A (row stride KC+4 dwords); the rest are broadcasts.
+// (colmap below).  LDS accesses are bank-conflict-free under gfx950's bank
+// model as measured per access pattern (fw_cuda_amd/microbench/lds_banks.hip;
+// tests/test_lds_banks.py walks every access of every shape): a wave's tx
+// lanes read consecutive 64-bit B pairs, its ty rows read A as 16-byte
+// broadcasts, and the stash writes whole 16-byte A groups and 8-byte B pairs.
 //
 // K is walked in chunks of KC, double-buffered through LDS with one barrier
 // per chunk.  A is staged row-major (k contiguous), B pair-interleaved
@@ -74,12 +76,24 @@ __device__ __forceinline__ int block_tid() {
   return (int)t;
 }
 
+// Row stride (dwords) of the staged A chunk.  gfx950's LDS has 32 banks of
+// 4 bytes; a 16-byte write serves 8 lanes per cycle over 128 bytes, which the
+// A stash fills with one 32-deep row (8 groups of 4 k) or two 16-deep rows:
+// 36 dwords keep every 32-deep row start 16-byte aligned, and 16-deep rows
+// stay unpadded so that two consecutive rows are 128 contiguous bytes.
+template <int KC>
+constexpr int tile_lda() {
+  return KC == 16 ? 16 : KC + 4;
+}
+
 // LDS staging of one product: two buffers of an A chunk (row-major, row
-// stride KC+4) and a B chunk (pair-interleaved).
+// stride tile_lda) and a B chunk (pair-interleaved).  Both are 16-byte
+// aligned so that the stash stores 16- and 8-byte words (4-byte pairs on
+// 8-byte slots conflict 2-way).
 template <int BM, int BN, int KC>
-struct TileSmem {
-  int32_t a[2][BM * (KC + 4)];
-  int32_t b[2][(KC / 2) * BN * 2];
+struct alignas(16) TileSmem {
+  alignas(16) int32_t a[2][BM * tile_lda<KC>()];
+  alignas(16) int32_t b[2][(KC / 2) * BN * 2];
 };
 
 // VEC: 16-byte global loads (needs ld % 4 == 0, 16-byte aligned base and
@@ -96,7 +110,7 @@ struct TileProduct {
   static constexpr int NW = NT / 64;  // waves
   static constexpr int TM = BM / 16;
   static constexpr int TN = BN / NTX;
-  static constexpr int LDA = KC + 4;
+  static constexpr int LDA = tile_lda<KC>();
   // Column placement.  Thread tx owns groups of G = min(4, TN) adjacent
   // columns: cell slot c is column colmap(tx, c) = G*(tx + NTX*(c/G)) + c%G.
   // In LDS, B's pair row stores column j at pair slot bpos(j) =
@@ -106,6 +120,14 @@ struct TileProduct {
   static constexpr int G = TN >= 4 ? 4 : TN;
   __device__ __forceinline__ static int colmap(int tx, int c) { return G * (tx + NTX * (c / G)) + c % G; }
   __device__ __forceinline__ static int bpos(int j) { return (j % G) * (BN / G) + j / G; }
+  // 8-byte slot of pair (pr, j) in the B buffer.  An 8-byte write serves 16
+  // lanes per cycle over 128 bytes; when a pair row holds fewer than 16
+  // staging units (BN = 32), the 16 lanes span two pair rows 256 bytes apart,
+  // which would hit the same banks, so odd pair rows are rotated by one slot.
+  static constexpr int SKEW = (BN / 4 < 16) ? 1 : 0;
+  __device__ __forceinline__ static int bslot(int pr, int j) {
+    return pr * BN + (SKEW ? ((bpos(j) + (pr & 1)) & (BN - 1)) : bpos(j));
+  }
   static_assert(BM % 16 == 0 && BN % NTX == 0, "block must divide over the threads");
   static_assert(KC % 4 == 0, "chunk must hold whole k quads");
   // A staging: KC/4 int4 groups per row.
@@ -253,7 +275,7 @@ struct TileProduct {
       const int r = p * AROWS + tid / AG;
       const int k = (tid % AG) * 4;
       if (BM % AROWS == 0 || r < BM)
-        *reinterpret_cast<int4*>(&s.a[buf][r * LDA + k]) = ra[p];
+        reinterpret_cast<int4*>(s.a[buf])[(r * LDA + k) / 4] = ra[p];
     }
 #pragma unroll
     for (int u = 0; u < BPASS; ++u) {
@@ -261,15 +283,11 @@ struct TileProduct {
       if (BU % NT == 0 || idx < BU) {
         const int pr = idx / (BN / 4);
         const int j = (idx % (BN / 4)) * 4;
-        int32_t* row = &s.b[buf][pr * BN * 2];
-        row[bpos(j + 0) * 2 + 0] = rb0[u].x;
-        row[bpos(j + 0) * 2 + 1] = rb1[u].x;
-        row[bpos(j + 1) * 2 + 0] = rb0[u].y;
-        row[bpos(j + 1) * 2 + 1] = rb1[u].y;
-        row[bpos(j + 2) * 2 + 0] = rb0[u].z;
-        row[bpos(j + 2) * 2 + 1] = rb1[u].z;
-        row[bpos(j + 3) * 2 + 0] = rb0[u].w;
-        row[bpos(j + 3) * 2 + 1] = rb1[u].w;
+        int2* slots = reinterpret_cast<int2*>(s.b[buf]);
+        slots[bslot(pr, j + 0)] = make_int2(rb0[u].x, rb1[u].x);
+        slots[bslot(pr, j + 1)] = make_int2(rb0[u].y, rb1[u].y);
+        slots[bslot(pr, j + 2)] = make_int2(rb0[u].z, rb1[u].z);
+        slots[bslot(pr, j + 3)] = make_int2(rb0[u].w, rb1[u].w);
       }
     }
   }
@@ -292,8 +310,8 @@ struct TileProduct {
         a[r] = *reinterpret_cast<const int4*>(&As[(ty + 16 * r) * LDA + 4 * q]);
 #pragma unroll
       for (int c = 0; c < TN; ++c) {
-        b0[c] = Bs[(2 * q) * BN + bpos(colmap(tx, c))];
-        b1[c] = Bs[(2 * q + 1) * BN + bpos(colmap(tx, c))];
+        b0[c] = Bs[bslot(2 * q, colmap(tx, c))];
+        b1[c] = Bs[bslot(2 * q + 1, colmap(tx, c))];
       }
 #pragma unroll
       for (int r = 0; r < TM; ++r) {

@@ -489,7 +489,7 @@ static int run_blocked_sideband(int32_t* d, int32_t n, int32_t kappa, hipStream_
   FWX_HIP(hipStreamWaitEvent(side, evR, 0));
   if (int rc = lead_stage<T, VEC, PK>(d, n, 0, min(kappa, N), N, prio, side)) return rc;
   FWX_HIP(hipEventRecord(evL, side));
-  hipEvent_t ev = nullptr;  // profiling chain on the caller's stream
+  hipEvent_t ev = nullptr;  // profiling markers of the rest passes on the caller's stream
   for (int kb = 0; kb < N; kb += kappa) {
     const int B = min(kappa, N - kb);
     const int s0 = kb * T, S = B * T;
@@ -497,8 +497,9 @@ static int run_blocked_sideband(int32_t* d, int32_t n, int32_t kappa, hipStream_
     const int en = (S + R - 1) / R;
     const int kb2 = kb + kappa;
     FWX_HIP(hipStreamWaitEvent(st, evL, 0));
-    if (kb == 0)
-      if (int rc = prof_open(st, &ev)) return rc;
+    // a fresh start marker after the wait, so that a profiled rest group
+    // times the rest kernel alone, not the wait for the side stream
+    if (int rc = prof_open(st, &ev)) return rc;
     if (kb2 < N) {
       const int B2 = min(kappa, N - kb2);
       const int fn = (B2 * T + R - 1) / R;

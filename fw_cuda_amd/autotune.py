@@ -40,7 +40,7 @@ from . import _lib
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 
-CLASSES = [64, 128, 256, 512, 1024, 2048, 4096, 8192, 16384]
+CLASSES = [64, 128, 256, 512, 1024, 1536, 2048, 4096, 8192, 16384]
 MAX_N = 32769
 TILES = (32, 64, 128)
 KAPPAS = (1, 2, 3, 4, 6, 8, 12, 16)

@@ -598,6 +598,29 @@ static int enqueue_direct(int32_t* d, int32_t n, int32_t tile, int32_t kappa, bo
 }
 
 // ---------------------------------------------------------------------------
+// Small n: tile 32 with kappa 1 runs as one fused launch per k-tile round
+// (fwx_small.hip), each block picking the relaxation form from the operands
+// it loaded, so there is no range pass and the host never waits.  Up to
+// kSmallMaxN; FWX_SMALL=0 keeps the three launches per round.
+constexpr int32_t kSmallMaxN = 4096;
+
+static bool small_path(int32_t n, int32_t tile, int32_t kappa) {
+  if (tile != 32 || kappa != 1 || n > kSmallMaxN) return false;
+  const char* e = std::getenv("FWX_SMALL");
+  return !(e && e[0] == '0');
+}
+
+// The rounds, bracketed for the profiler as one lead group of n^3
+// relaxations.
+static int enqueue_small(int32_t* d, int32_t n, hipStream_t st) {
+  hipEvent_t ev;
+  if (int rc = prof_open(st, &ev)) return rc;
+  if (int rc = enqueue_small_rounds(d, n, st)) return rc;
+  const double nn = (double)n;
+  return prof_close(st, ev, FWX_PROF_LEAD, nn * nn * nn);
+}
+
+// ---------------------------------------------------------------------------
 // Replay cache.  A solve is a fixed sequence of a few hundred launches for a
 // given (buffer, n, tile, kappa, form, stream); the first call captures it
 // (both streams of the look-ahead schedule, joined by events) into a
@@ -617,13 +640,14 @@ struct GraphKey {
   int32_t* d;
   int32_t n, tile, kappa;
   bool vec, pk;
+  bool small;  // the fused small-n rounds (each block then picks the form itself)
   int la;
   int knobs;  // FWX_DPANEL / FWX_REST_VARIANT / FWX_BAND / FWX_DIAG_NTH / FWX_LEAD_PRIO choices
   hipStream_t st;
   int dev;
   bool operator==(const GraphKey& o) const {
     return d == o.d && n == o.n && tile == o.tile && kappa == o.kappa && vec == o.vec && pk == o.pk &&
-           la == o.la && knobs == o.knobs && st == o.st && dev == o.dev;
+           small == o.small && la == o.la && knobs == o.knobs && st == o.st && dev == o.dev;
   }
 };
 
@@ -649,14 +673,18 @@ bool graphs_enabled() {
 
 int enqueue_blocked(int32_t* d, int32_t n, int32_t tile, int32_t kappa, hipStream_t st) {
   const bool vec = (n % 4 == 0) && ((reinterpret_cast<uintptr_t>(d) & 15u) == 0);
+  const bool small = small_path(n, tile, kappa);
+  const bool direct = n > kGraphMaxN || !graphs_enabled() || profiling();
+  if (small && direct) return enqueue_small(d, n, st);
   bool pk = false;
-  if (int rc = values_in_pk64_range(d, (int64_t)n * n, st, &pk)) return rc;
-  if (n > kGraphMaxN || !graphs_enabled() || profiling()) return enqueue_direct(d, n, tile, kappa, vec, pk, st);
+  if (!small)
+    if (int rc = values_in_pk64_range(d, (int64_t)n * n, st, &pk)) return rc;
+  if (direct) return enqueue_direct(d, n, tile, kappa, vec, pk, st);
   int dev = 0;
   FWX_HIP(hipGetDevice(&dev));
   const int knobs = (fused_dpanel() ? 1 : 0) | (rest_variant(n) << 1) | (band_narrow() ? 8 : 0) | (diag_threads() << 4) |
-                    (lead_priority_enabled() ? 1 << 20 : 0);
-  const GraphKey key{d, n, tile, kappa, vec, pk, lookahead_mode(), knobs, st, dev};
+                    (lead_priority_enabled() ? 1 << 20 : 0) | (small ? small_region_knob(n) << 21 : 0);
+  const GraphKey key{d, n, tile, kappa, vec, pk, small, lookahead_mode(), knobs, st, dev};
   std::lock_guard<std::mutex> lk(g_graph_mu);
   auto replay = [&](GraphEntry& e) -> int {
     e.used = ++g_graph_clock;
@@ -677,7 +705,7 @@ int enqueue_blocked(int32_t* d, int32_t n, int32_t tile, int32_t kappa, hipStrea
   hipError_t e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
   int rc = FWX_OK;
   t_capture_side = cap_side;
-  if (e == hipSuccess) rc = enqueue_direct(d, n, tile, kappa, vec, pk, cap);
+  if (e == hipSuccess) rc = small ? enqueue_small(d, n, cap) : enqueue_direct(d, n, tile, kappa, vec, pk, cap);
   t_capture_side = nullptr;
   hipGraph_t graph = nullptr;
   const hipError_t e2 = hipStreamEndCapture(cap, &graph);

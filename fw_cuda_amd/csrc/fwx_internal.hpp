@@ -36,6 +36,10 @@ inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s);
 int enqueue_baseline(int32_t* d, int32_t n, hipStream_t st);
 int enqueue_blocked(int32_t* d, int32_t n, int32_t tile, int32_t kappa, hipStream_t st);
 int enqueue_refshape(int32_t* d, int32_t n, hipStream_t st);
+// The fused rounds of a small solve on 32-wide k-tiles (fwx_small.hip), and
+// the region width they use for size n (a graph-cache key).
+int enqueue_small_rounds(int32_t* d, int32_t n, hipStream_t st);
+int small_region_knob(int32_t n);
 
 // Event profiling (fwx_profile.hip): prof_open records a start event when
 // profiling is on (else leaves *a null); prof_close records the end event.

@@ -438,8 +438,10 @@ __device__ __forceinline__ void tile_update(TileSmem<BM, BN, KC>& s, const View&
 // the T dependent steps.
 // `lines` is LDS scratch of 8 T ints (the caller's, so a kernel that also
 // runs tile products can lend their staging buffer).
+// close_regs runs the steps on a tile already in the threads' registers
+// (acc[r][q] = cell (ty + RS r, tx + 16 q)), for k < kmax.
 template <int T, int NTH = 256>
-__device__ __forceinline__ void close_tile_in(const View V, int32_t* lines) {
+__device__ __forceinline__ void close_regs(int32_t (&acc)[T / (NTH / 16)][T / 16], int32_t* lines, int kmax) {
   constexpr int RS = NTH / 16;  // thread rows
   constexpr int TR = T / RS;    // register rows per thread
   constexpr int TQ = T / 16;    // register columns per thread
@@ -448,13 +450,6 @@ __device__ __forceinline__ void close_tile_in(const View V, int32_t* lines) {
   int32_t(*cols)[2][T] = reinterpret_cast<int32_t(*)[2][T]>(lines + 4 * T);  // [buf][k, k+1][i]
   const int tx = threadIdx.x & 15;
   const int ty = threadIdx.x >> 4;
-  int32_t acc[TR][TQ];
-#pragma unroll
-  for (int r = 0; r < TR; ++r)
-#pragma unroll
-    for (int q = 0; q < TQ; ++q)
-      acc[r][q] = V.in(ty + RS * r, tx + 16 * q) ? *V.at(ty + RS * r, tx + 16 * q) : kInf;
-  const int kmax = min(T, V.rlim);
 #pragma unroll
   for (int qk = 0; qk < TQ; ++qk) {
     // steps k = 16 qk + k16: column k is register column qk of threads
@@ -506,6 +501,22 @@ __device__ __forceinline__ void close_tile_in(const View V, int32_t* lines) {
       }
     }
   }
+}
+
+template <int T, int NTH = 256>
+__device__ __forceinline__ void close_tile_in(const View V, int32_t* lines) {
+  constexpr int RS = NTH / 16;
+  constexpr int TR = T / RS;
+  constexpr int TQ = T / 16;
+  const int tx = threadIdx.x & 15;
+  const int ty = threadIdx.x >> 4;
+  int32_t acc[TR][TQ];
+#pragma unroll
+  for (int r = 0; r < TR; ++r)
+#pragma unroll
+    for (int q = 0; q < TQ; ++q)
+      acc[r][q] = V.in(ty + RS * r, tx + 16 * q) ? *V.at(ty + RS * r, tx + 16 * q) : kInf;
+  close_regs<T, NTH>(acc, lines, min(T, V.rlim));
 #pragma unroll
   for (int r = 0; r < TR; ++r)
 #pragma unroll

@@ -402,3 +402,76 @@ def test_graph_replay_cache(monkeypatch, lookahead):
         assert abs(total - 3 * float(n) ** 3) < 1e-6 * 3e9, total
         assert ms[_lib.PROF_REST] > 0 and grp[_lib.PROF_REST] > 0
         solve(g, expect, "replay after profiling")
+
+
+@pytest.mark.parametrize("region,threads", [("64", "512"), ("128", "512"), ("64", "256"), ("128", "256")])
+def test_small_rounds_every_form_and_mode(monkeypatch, region, threads):
+    """Tile 32 with kappa 1 runs the fused small-n rounds (one launch per
+    k-tile round, each block picking the relaxation form from its operands),
+    with 64- and 128-wide regions on 256 or 512 threads per block, and the
+    next round's diagonal tile closed ahead by one more block per launch.  On one buffer per size, inputs
+    that select pk64 and min3 (negative weights; operands >= 2^30)
+    alternate, replayed from a graph and launched directly, plus the
+    three-launch rounds (FWX_SMALL=0); every result equals the oracle.
+    Sizes: one tile, ragged edges, up to 65 k-tiles."""
+    monkeypatch.setenv("FWX_SMALL_R", region)
+    monkeypatch.setenv("FWX_SMALL_NT", threads)
+    rng = np.random.default_rng(321)
+    for n in (1, 2, 31, 32, 33, 64, 100, 255, 513, 1024, 1500, 2049):
+        pos = fx.generate_graph(n, float(rng.uniform(0.05, 0.9)), int(rng.integers(1, 2**31)), 1, 1000)
+        neg = reweighted(rng, n, 0.4) if n > 2 else pos
+        big = pos.copy()
+        if n > 2:
+            col = n // 2
+            sel = (rng.random(n) < 0.5) & (big[:, col] == INF)
+            big[sel, col] = 1_100_000_000
+            np.fill_diagonal(big, 0)
+        cases = [("pk64", pos, po.fw_rowpar(pos)), ("min3", neg, po.fw_rowpar(neg)),
+                 ("big", big, po.fw_rowpar(big))]
+        with fx.DeviceMatrix(n) as dm:
+            for small, graph in (("1", "1"), ("1", "1"), ("1", "0"), ("0", "1"), ("0", "0"), ("1", "1")):
+                monkeypatch.setenv("FWX_SMALL", small)
+                monkeypatch.setenv("FWX_GRAPH", graph)
+                for tag, src, exp in cases:
+                    say("small rounds", region, threads, n, small, graph, tag)
+                    dm.upload(src)
+                    fx.run_device("tiling_gpu", dm.ptr, n, 32, 1)
+                    assert (dm.download() == exp).all(), (region, threads, n, small, graph, tag)
+
+
+def test_small_rounds_profiled_and_graph_eviction():
+    """Profiled small solves launch directly and account n^3 relaxations each;
+    more distinct buffers than the graph cache holds evict small-n graphs
+    (each holding a range-flag slot), and solves after the evictions stay
+    exact."""
+    import ctypes
+    from fw_cuda_amd import _lib
+    n = 300
+    g = fx.generate_graph(n, 0.5, 17, 1, 1000)
+    gneg = reweighted(np.random.default_rng(6), n, 0.5)
+    expect, expect_neg = po.fw_rowpar(g), po.fw_rowpar(gneg)
+    L = _lib.lib()
+    with fx.DeviceMatrix(n) as dm:
+        _lib.check(L.fwx_profile_begin())
+        for src, exp in ((g, expect), (gneg, expect_neg)):
+            dm.upload(src)
+            fx.run_device("tiling_gpu", dm.ptr, n, 32, 1)
+            assert (dm.download() == exp).all()
+        ms = (ctypes.c_double * _lib.PROF_NCLASS)()
+        wk = (ctypes.c_double * _lib.PROF_NCLASS)()
+        grp = (ctypes.c_int64 * _lib.PROF_NCLASS)()
+        _lib.check(L.fwx_profile_end(ms, wk, grp))
+        assert abs(wk[_lib.PROF_LEAD] - 2 * float(n) ** 3) < 1e-3, wk[_lib.PROF_LEAD]
+        assert ms[_lib.PROF_LEAD] > 0 and grp[_lib.PROF_LEAD] == 2
+    mats = [fx.DeviceMatrix(n) for _ in range(12)]
+    try:
+        for rep in range(3):
+            for k, dm in enumerate(mats):
+                src, exp = ((g, expect), (gneg, expect_neg))[(k + rep) % 2]
+                say("eviction", rep, k)
+                dm.upload(src)
+                fx.run_device("tiling_gpu", dm.ptr, n, 32, 1)
+                assert (dm.download() == exp).all(), (rep, k)
+    finally:
+        for dm in mats:
+            dm.free()

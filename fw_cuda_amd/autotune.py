@@ -63,7 +63,9 @@ def candidates(n):
         for k in KAPPAS:
             if k > max(1, N):
                 continue
-            if t == 32 and k * t < 64 and n > 256:
+            if t == 32 and k == 1 and n > 4096:
+                continue  # (32, 1) runs the fused small-n rounds, up to n = 4096
+            if t == 32 and k * t < 64 and k > 1 and n > 256:
                 continue  # 32-wide rest blocks only pay off on tiny graphs
             for v in (REST_SHAPES if rest_block(t, k) == 128 and N > k else (2,)):
                 out.append((t, k, v))
@@ -215,7 +217,7 @@ def tune_batched(start, passes=2, reps=3):
     return cur, record
 
 
-def read_batch_table(path):
+def read_table(path):
     out = []
     for line in open(path):
         line = line.strip()
@@ -242,18 +244,24 @@ def main():
     json_path = os.path.join(a.out_dir or os.path.join(ROOT, "profiles"), "autotune_mi355x.json")
     os.makedirs(out_dir, exist_ok=True)
     record = {}
-    if os.path.exists(json_path):
-        with open(json_path) as f:
-            record = json.load(f)
+    prev = os.path.join(ROOT, "profiles", "autotune_mi355x.json")
+    for path in (prev, json_path):
+        if os.path.exists(path):
+            with open(path) as f:
+                record = json.load(f)
     if "plain" in what:
         table, rec = tune_plain(a.max_n)
-        record["plain"] = rec
+        # classes above --max-n keep their current entries and timings
+        kept = [e for e in read_table(os.path.join(src_dir, "fwx_tune_table.inc")) if e[0] > a.max_n]
+        if kept:
+            table = [e for e in table if e[0] <= a.max_n] + kept
+        record.setdefault("plain", {}).update(rec)
         with open(os.path.join(out_dir, "fwx_tune_table.inc"), "w") as f:
             f.write("// Generated by fw_cuda_amd/autotune.py on the target GPU: {n_max, tile, kappa, rest}.\n")
             for e in table:
                 f.write("{%d, %d, %d, %d},\n" % e)
     if "batched" in what:
-        start = read_batch_table(os.path.join(src_dir, "fwx_batch_table.inc"))
+        start = read_table(os.path.join(src_dir, "fwx_batch_table.inc"))
         start = [next((e for e in start if e[0] >= c), start[-1]) for c in BATCH_CLASSES]
         start = [(c,) + tuple(e[1:]) for c, e in zip(BATCH_CLASSES, start)]
         table, rec = tune_batched(start)

@@ -119,13 +119,13 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
 // j+1.  The steps and their side
 // branches are those of the step launches above -- a hoisted closure follows
 // its product in one block's list -- so the barriers carry exactly the
-// dependencies the launch boundaries carry there, and the barrier's own
-// device-scope fence makes each step's stores visible to every XCD.  The
-// cooperative launch guarantees that every block of the grid is resident at
-// once (the runtime refuses a grid that does not fit), so every barrier is
-// reached by the whole grid and completes; the kernel has no waits of its
-// own.  A launch without the cooperative attribute has no grid barrier: every
-// block sees that at entry, before any barrier, sets *status and returns.
+// dependencies the launch boundaries carry there, and the barrier's
+// device-scope release and acquire make each step's stores visible to every
+// XCD.  The cooperative launch guarantees that every block of the grid is
+// resident at once (the runtime refuses a grid that does not fit), so every
+// barrier is reached by the whole grid and completes.  A launch without the
+// cooperative attribute is detected at entry, before any barrier: every block
+// sets *status and returns.
 struct CoopTask {
   int64_t base;
   int32_t n;
@@ -134,6 +134,46 @@ struct CoopTask {
   int32_t cr, cc, ar, ac, br, bc;
   int32_t depth;
 };
+
+// The grid barrier (bar[0] = arrivals, bar[64] = generation, zeroed before
+// the launch; *gen, in LDS, is the block's own count of barriers passed).  A device-
+// scope fence writes back the XCD's whole L2, about 0.2 us per fence and
+// serialised per XCD, so only thread 0 of a block fences: every wave first
+// waits for its own stores to complete (into its XCD's L2), the block barrier
+// orders that before thread 0's release-increment, whose write-back covers
+// them; the last arrival resets the count and publishes the next generation
+// with a release store; the others poll it with relaxed loads that bypass the
+// L2 (an acquire load per poll would invalidate the L2 each time) and then
+// acquire once, invalidating the L2 and the CU's L1 -- which every wave of
+// the block shares -- before the block barrier lets them load.  Measured on
+// MI355X (fw_cuda_amd/microbench/sync_cost.hip), 512 blocks: 12.6 us per
+// barrier, against 52 us for cooperative_groups' grid sync, whose every
+// thread fences.  Every block passes the same barriers, and the cooperative
+// launch keeps them all resident, so each wait ends; a poll that still
+// gives up after 2^24 tries sets bit 2 of *status.
+__device__ __forceinline__ void coop_barrier(uint32_t* bar, uint32_t blocks, uint32_t* gen, int32_t* status) {
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const uint32_t g = *gen;
+    if (__hip_atomic_fetch_add(bar, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) == blocks - 1) {
+      __hip_atomic_store(bar, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(bar + 64, g + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+      uint32_t spins = 0;
+      while (__hip_atomic_load(bar + 64, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == g) {
+        __builtin_amdgcn_s_sleep(1);
+        if (++spins > (1u << 24)) {
+          atomicOr(status, 2);
+          break;
+        }
+      }
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    }
+    *gen = g + 1;
+  }
+  __syncthreads();
+}
 
 // Every persistent task is 128 wide (coop_config below): one product shape --
 // the 512-thread, 8 x 4 cells per thread product of the step launches -- and
@@ -167,14 +207,16 @@ __device__ __forceinline__ void coop_task(TileSmem<128, 128, 32>& s, int32_t* da
 template <bool PK>
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_batch_coop(
     int32_t* __restrict__ data, const CoopTask* __restrict__ tasks, const int32_t* __restrict__ ufirst,
-    const int32_t* __restrict__ sfirst, int32_t steps, int32_t* __restrict__ ctr, int32_t* __restrict__ status) {
+    const int32_t* __restrict__ sfirst, int32_t steps, int32_t* __restrict__ ctr, uint32_t* __restrict__ bar,
+    int32_t* __restrict__ status) {
   __shared__ TileSmem<128, 128, 32> s;
   __shared__ int32_t s_unit;
-  cg::grid_group grid = cg::this_grid();
-  if (!grid.is_valid()) {
+  __shared__ uint32_t s_gen;
+  if (!cg::this_grid().is_valid()) {
     if (threadIdx.x == 0) atomicOr(status, 1);
     return;
   }
+  if (threadIdx.x == 0) s_gen = 0;
   for (int32_t j = 0; j < steps; ++j) {
     const int32_t u0 = sfirst[j], u1 = sfirst[j + 1];
     for (;;) {
@@ -190,15 +232,18 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
         __syncthreads();  // the next task restages the LDS the last one read
       }
     }
-    grid.sync();
+    coop_barrier(bar, gridDim.x, &s_gen, status);
   }
 }
 
-// Clears the claim counters of every step and the status word before a
-// persistent run (a kernel, so that a captured graph holds no memset node).
-__global__ void k_coop_reset(int32_t* __restrict__ ctr, int32_t steps, int32_t* __restrict__ status) {
+// Clears the claim counters of every step, the barrier words and the status
+// word before a persistent run (a kernel, so that a captured graph holds no
+// memset node).
+__global__ void k_coop_reset(int32_t* __restrict__ ctr, int32_t steps, uint32_t* __restrict__ bar,
+                             int32_t* __restrict__ status) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < steps) ctr[i] = 0;
+  if (i < 128) bar[i] = 0;
   if (i == 0) *status = 0;
 }
 
@@ -384,7 +429,8 @@ struct fwx_batch_plan {
   int32_t* d_ufirst = nullptr; // first task of each unit (units + 1)
   int32_t* d_sfirst = nullptr; // first unit of each step (csteps + 1)
   int32_t* d_ctr = nullptr;    // per-step claim counters (csteps)
-  int32_t* d_status = nullptr; // set by k_batch_coop when launched without a grid barrier
+  uint32_t* d_bar = nullptr;   // k_batch_coop's grid barrier: arrivals, generation (256 bytes apart)
+  int32_t* d_status = nullptr; // bit 1: launched without the cooperative attribute; bit 2: a barrier wait gave up
   int64_t ctasks = 0;
   int32_t csteps = 0;
   int32_t cgrid = 0;           // resident blocks of k_batch_coop (0: no cooperative launch here)
@@ -447,9 +493,10 @@ static int enqueue_mode(fwx_batch_plan* p, bool pk) { return pk ? enqueue_plan<t
 
 template <bool PK>
 static int enqueue_persistent(fwx_batch_plan* p) {
-  k_coop_reset<<<(p->csteps + 255) / 256, 256, 0, p->stream>>>(p->d_ctr, p->csteps, p->d_status);
+  k_coop_reset<<<(std::max(p->csteps, 128) + 255) / 256, 256, 0, p->stream>>>(p->d_ctr, p->csteps, p->d_bar,
+                                                                             p->d_status);
   FWX_LAUNCHED("k_coop_reset");
-  void* args[] = {&p->data, &p->d_ctasks, &p->d_ufirst, &p->d_sfirst, &p->csteps, &p->d_ctr, &p->d_status};
+  void* args[] = {&p->data, &p->d_ctasks, &p->d_ufirst, &p->d_sfirst, &p->csteps, &p->d_ctr, &p->d_bar, &p->d_status};
   FWX_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&k_batch_coop<PK>), dim3(p->cgrid), dim3(512),
                                      args, 0, p->stream));
   return FWX_OK;
@@ -567,6 +614,7 @@ static int build_coop(fwx_batch_plan* p, const int32_t* sizes, const int64_t* of
   if (int rc = upload(&p->d_sfirst, sfirst.data(), sfirst.size())) return rc;
   FWX_HIP(hipMalloc(reinterpret_cast<void**>(&p->d_ctr), (size_t)std::max(1, steps) * sizeof(int32_t)));
   FWX_HIP(hipMalloc(reinterpret_cast<void**>(&p->d_status), sizeof(int32_t)));
+  FWX_HIP(hipMalloc(reinterpret_cast<void**>(&p->d_bar), 128 * sizeof(uint32_t)));
   p->ctasks = (int64_t)tasks.size();
   p->csteps = steps;
   p->cgrid = G;
@@ -595,6 +643,7 @@ static void destroy_plan(fwx_batch_plan* p) {
   if (p->d_sfirst) (void)hipFree(p->d_sfirst);
   if (p->d_ctr) (void)hipFree(p->d_ctr);
   if (p->d_status) (void)hipFree(p->d_status);
+  if (p->d_bar) (void)hipFree(p->d_bar);
   delete p;
 }
 
@@ -769,9 +818,11 @@ int fwx_batch_plan_run(fwx_batch_plan* p, int32_t mode) {
     int32_t status = 0;
     FWX_HIP(hipMemcpyAsync(&status, p->d_status, sizeof(status), hipMemcpyDeviceToHost, p->stream));
     FWX_HIP(hipStreamSynchronize(p->stream));
-    if (status)
+    if (status & 1)
       return fail(FWX_ERR_UNSUPPORTED, "the persistent kernel ran without a grid barrier (not a cooperative "
                                        "launch); nothing was computed");
+    if (status)
+      return fail(FWX_ERR_TIMEOUT, "a grid-barrier wait of the persistent kernel gave up; the result is incomplete");
     return FWX_OK;
   }
   FWX_HIP(hipStreamSynchronize(p->stream));

@@ -641,13 +641,15 @@ struct GraphKey {
   int32_t n, tile, kappa;
   bool vec, pk;
   bool small;  // the fused small-n rounds (each block then picks the form itself)
+  int small_cfg;  // their region width and threads (small_region_knob)
   int la;
   int knobs;  // FWX_DPANEL / FWX_REST_VARIANT / FWX_BAND / FWX_DIAG_NTH / FWX_LEAD_PRIO choices
   hipStream_t st;
   int dev;
   bool operator==(const GraphKey& o) const {
     return d == o.d && n == o.n && tile == o.tile && kappa == o.kappa && vec == o.vec && pk == o.pk &&
-           small == o.small && la == o.la && knobs == o.knobs && st == o.st && dev == o.dev;
+           small == o.small && small_cfg == o.small_cfg && la == o.la && knobs == o.knobs && st == o.st &&
+           dev == o.dev;
   }
 };
 
@@ -683,8 +685,9 @@ int enqueue_blocked(int32_t* d, int32_t n, int32_t tile, int32_t kappa, hipStrea
   int dev = 0;
   FWX_HIP(hipGetDevice(&dev));
   const int knobs = (fused_dpanel() ? 1 : 0) | (rest_variant(n) << 1) | (band_narrow() ? 8 : 0) | (diag_threads() << 4) |
-                    (lead_priority_enabled() ? 1 << 20 : 0) | (small ? small_region_knob(n) << 21 : 0);
-  const GraphKey key{d, n, tile, kappa, vec, pk, small, lookahead_mode(), knobs, st, dev};
+                    (lead_priority_enabled() ? 1 << 20 : 0);
+  const GraphKey key{d, n, tile, kappa, vec, pk, small, small ? small_region_knob(n) : 0, lookahead_mode(), knobs, st,
+                     dev};
   std::lock_guard<std::mutex> lk(g_graph_mu);
   auto replay = [&](GraphEntry& e) -> int {
     e.used = ++g_graph_clock;

@@ -524,6 +524,85 @@ static int run_blocked_sideband(int32_t* d, int32_t n, int32_t kappa, hipStream_
   return FWX_OK;
 }
 
+// Side-band-ahead schedule (FWX_LOOKAHEAD=3): the side-band schedule with
+// the side stream one stage ahead.  Measured at n=8192 on (64, 4): 17.31-17.34
+// vs 17.39-17.40 ms per step for the side-band schedule -- the step is
+// already within ~1 % of n^3 at the rest kernel's own rate, so removing the
+// caller's waits for the side stream gains little.  Stage s's rest pass outside bands s and s+1 is split once
+// more: X(s), its tiles with a row or a column in band s+2, runs first on the
+// caller's stream (on 64-wide tiles, like the band), then Y(s), every other
+// tile.  band(s+1) -- the rest(s+1) tiles with a row or a column in band s+2
+// -- reads and writes only cells that rest(s) wrote in X(s), so it waits for
+// X(s) (event X) instead of the whole rest pass; the side stream then runs
+// band(s+1) and lead(s+2) while the caller's stream still runs Y(s), and the
+// event the caller waits for before stage s+2 (L) has completed by then.
+// Read/write sets: Y(s) writes cells outside bands s, s+1, s+2 and reads
+// band-s panel cells outside band s+2; band(s+1) writes cells with a row or
+// column in band s+2 outside band s+1 and reads band s+1; lead(s+2) reads
+// and writes band s+2 only; X(s) writes cells with a row or column in band
+// s+2 outside bands s and s+1, which the side stream touches only after
+// event X.  Ordering: X(s), Y(s) wait for lead(s) (event L); band(s+1) waits
+// for X(s) (event X).
+template <int T, bool VEC, bool PK>
+static int run_blocked_ahead(int32_t* d, int32_t n, int32_t kappa, hipStream_t st, hipStream_t side,
+                             hipEvent_t evX, hipEvent_t evL) {
+  const int N = (n + T - 1) / T;
+  const int span = kappa * T;
+  const int R = (span % 128 == 0) ? 128 : (span % 64 == 0) ? 64 : 32;
+  const double nn = (double)n;
+  const int prio = lead_priority_enabled() ? 1 : 0;
+  FWX_HIP(hipEventRecord(evX, st));  // the side stream starts after the caller's prior work
+  FWX_HIP(hipStreamWaitEvent(side, evX, 0));
+  if (int rc = lead_stage<T, VEC, PK>(d, n, 0, min(kappa, N), N, prio, side)) return rc;
+  FWX_HIP(hipEventRecord(evL, side));
+  hipEvent_t ev = nullptr;  // profiling markers of the caller's stream
+  for (int kb = 0; kb < N; kb += kappa) {
+    const int B = min(kappa, N - kb);
+    const int s0 = kb * T, S = B * T;
+    const double L = (double)min(S, n - s0);
+    const int en = (S + R - 1) / R;  // band s in R tiles
+    const int kb2 = kb + kappa, kb3 = kb2 + kappa;
+    FWX_HIP(hipStreamWaitEvent(st, evL, 0));
+    if (int rc = prof_open(st, &ev)) return rc;
+    if (kb2 < N) {
+      // side: band(s) -- it waited for X(s-1) when that was enqueued -- and lead(s+1)
+      const int B2 = min(kappa, N - kb2);
+      const int fn = (B2 * T + R - 1) / R;
+      const double L2 = (double)min(B2 * T, n - kb2 * T);
+      hipEvent_t es;
+      if (int rc = prof_open(side, &es)) return rc;
+      if (int rc = band_dispatch<VEC, PK>(R, d, n, s0, S, en, fn, side)) return rc;
+      if (int rc = prof_chain(side, &es, FWX_PROF_BAND, ((nn - L) * (nn - L) - (nn - L - L2) * (nn - L - L2)) * L))
+        return rc;
+      if (int rc = lead_rounds<T, VEC, PK>(d, n, kb2, B2, N, prio, side)) return rc;
+      if (int rc = prof_close(side, es, FWX_PROF_LEAD, lead_work(n, T, kb2, B2))) return rc;
+      FWX_HIP(hipEventRecord(evL, side));
+      if (kb3 < N) {
+        // caller: X(s), the rest tiles with a row or column in band s+2, first
+        const int B3 = min(kappa, N - kb3);
+        const int gn = (B3 * T + R - 1) / R;
+        const double L3 = (double)min(B3 * T, n - kb3 * T);
+        const double outside2 = nn - L - L2;  // rows outside bands s and s+1
+        if (int rc = band_dispatch<VEC, PK>(R, d, n, s0, S, en + fn, gn, st)) return rc;
+        if (int rc = prof_chain(st, &ev, FWX_PROF_BAND, (outside2 * outside2 - (outside2 - L3) * (outside2 - L3)) * L))
+          return rc;
+        FWX_HIP(hipEventRecord(evX, st));
+        // the side stream's band(s+1) needs X(s) only
+        FWX_HIP(hipStreamWaitEvent(side, evX, 0));
+        if (int rc = rest_dispatch<VEC, PK>(R, d, n, s0, S, en + fn + gn, st)) return rc;
+        if (int rc = prof_chain(st, &ev, FWX_PROF_REST, (outside2 - L3) * (outside2 - L3) * L)) return rc;
+      } else {
+        if (int rc = rest_dispatch<VEC, PK>(R, d, n, s0, S, en + fn, st)) return rc;
+        if (int rc = prof_chain(st, &ev, FWX_PROF_REST, (nn - L - L2) * (nn - L - L2) * L)) return rc;
+      }
+    } else {
+      if (int rc = rest_dispatch<VEC, PK>(R, d, n, s0, S, en, st)) return rc;
+      if (int rc = prof_chain(st, &ev, FWX_PROF_REST, (nn - L) * (nn - L) * L)) return rc;
+    }
+  }
+  return FWX_OK;
+}
+
 // One non-blocking side stream per device (default priority), created on
 // first use and kept for the life of the process.
 // While this thread captures a solve into a graph, its side stream is a
@@ -549,10 +628,10 @@ int side_stream(hipStream_t* out) {
 
 // FWX_LOOKAHEAD selects the schedule (read on every call): 0 = single
 // stream, 1 = look-ahead with the band on the caller's stream, 2 =
-// side-band (the default).
+// side-band (the default), 3 = side-band with the side stream a stage ahead.
 static int lookahead_mode() {
   const char* e = std::getenv("FWX_LOOKAHEAD");
-  return (e && e[0] >= '0' && e[0] <= '2') ? e[0] - '0' : 2;
+  return (e && e[0] >= '0' && e[0] <= '3') ? e[0] - '0' : 2;
 }
 
 template <int T, bool VEC, bool PK>
@@ -569,8 +648,9 @@ static int run_any(int32_t* d, int32_t n, int32_t kappa, hipStream_t st) {
     (void)hipEventDestroy(evA);
     return hip_fail(e, "hipEventCreateWithFlags", __FILE__, __LINE__);
   }
-  const int rc = mode == 2 ? run_blocked_sideband<T, VEC, PK>(d, n, kappa, st, side, evA, evB)
-                           : run_blocked_lookahead<T, VEC, PK>(d, n, kappa, st, side, evA, evB);
+  const int rc = mode == 3   ? run_blocked_ahead<T, VEC, PK>(d, n, kappa, st, side, evA, evB)
+                 : mode == 2 ? run_blocked_sideband<T, VEC, PK>(d, n, kappa, st, side, evA, evB)
+                             : run_blocked_lookahead<T, VEC, PK>(d, n, kappa, st, side, evA, evB);
   (void)hipEventDestroy(evA);
   (void)hipEventDestroy(evB);
   return rc;

@@ -157,11 +157,12 @@ def test_large_values_take_exact_path():
         assert (gpu(g, algo, tile, kappa) == expect).all(), (algo, tile, kappa)
 
 
-@pytest.mark.parametrize("lookahead,prio", [("0", "1"), ("1", "1"), ("1", "0"), ("2", "1"), ("2", "0")])
+@pytest.mark.parametrize("lookahead,prio", [("0", "1"), ("1", "1"), ("1", "0"), ("2", "1"), ("2", "0"), ("3", "1"),
+                                            ("3", "0")])
 def test_schedules_agree(monkeypatch, lookahead, prio):
-    """The single-stream, look-ahead and side-band (two-stream) schedules,
-    with and without the lead kernels' raised wave priority, graph-replayed
-    and launched directly."""
+    """The single-stream, look-ahead, side-band and side-band-ahead
+    (two-stream) schedules, with and without the lead kernels' raised wave
+    priority, graph-replayed and launched directly."""
     monkeypatch.setenv("FWX_LOOKAHEAD", lookahead)
     monkeypatch.setenv("FWX_LEAD_PRIO", prio)
     rng = np.random.default_rng(8)
@@ -364,7 +365,7 @@ def test_max_n_32769():
     assert (got == dij).all()
 
 
-@pytest.mark.parametrize("lookahead", ["0", "1", "2"])
+@pytest.mark.parametrize("lookahead", ["0", "1", "2", "3"])
 def test_graph_replay_cache(monkeypatch, lookahead):
     """Repeated solves on one buffer replay a captured hipGraph; every replay
     must equal the oracle, whichever relaxation form (pk64 / min3) the input

@@ -10,23 +10,22 @@
 #include "algorithm_runner.h"
 #include "constants.h"
 
+// Usage text in this project's words; the argv rules and the error lines
+// below are the reference's contract.
 static void printUsage(const char* programName) {
   std::cerr << "Usage: " << programName
-            << " <algorithm_name> <n_size> <density> [seed] [tile_size] [kappa] [wmin wmax]\n";
-  std::cerr << "\n";
-  std::cerr << "Arguments:\n";
-  std::cerr << "  algorithm_name  Name of the algorithm to run\n";
-  std::cerr << "  n_size          Graph size (number of vertices, 1 to " << MAX_N << ")\n";
-  std::cerr << "  density         Edge density (0.0 to 1.0)\n";
-  std::cerr << "  seed            Random seed (optional, 0 for random)\n";
-  std::cerr << "  tile_size       Tile size for tiling (optional; 0 = tuned; 32, 64, 128)\n";
-  std::cerr << "  kappa           Number of tile-layers to group (optional, used by multi_layer_tiling_gpu)\n";
-  std::cerr << "  wmin wmax       Edge weight range (optional, default 0 " << MAX_EDGE_WEIGHT << ")\n";
-  std::cerr << "\n";
-  std::cerr << "Examples:\n";
-  std::cerr << "  " << programName << " baseline_cpu 1024 0.8\n";
-  std::cerr << "  " << programName << " baseline_gpu 2048 0.9 42 16\n";
-  std::cerr << "  " << programName << " multi_layer_tiling_gpu 4096 0.9 42 32 4\n";
+            << " <algorithm_name> <n_size> <density> [seed] [tile_size] [kappa] [wmin wmax]\n"
+            << "\n"
+            << "Times one solve of a generated graph after one warmup call.\n"
+            << "  algorithm_name  baseline_cpu, baseline_gpu, tiling_gpu or multi_layer_tiling_gpu\n"
+            << "  n_size          vertices, 1 to " << MAX_N << "\n"
+            << "  density         probability of an edge i -> j (i != j), 0.0 to 1.0\n"
+            << "  seed            generator seed; 0 or omitted draws one at random\n"
+            << "  tile_size       32, 64 or 128; 0 or omitted picks the tuned tile for n_size\n"
+            << "  kappa           k-tiles per stage (multi_layer_tiling_gpu); 0 picks the tuned value\n"
+            << "  wmin wmax       edge weights drawn from [wmin, wmax] (default 0 " << MAX_EDGE_WEIGHT << ")\n"
+            << "\n"
+            << "e.g. " << programName << " multi_layer_tiling_gpu 8192 0.9 42 0 0\n";
 }
 
 int main(int argc, char* argv[]) {

@@ -69,7 +69,7 @@ namespace fwx {
 namespace {
 
 constexpr int kT = 32;        // k-tile
-constexpr int kLda = kT + 2;  // A rows in LDS: 8-byte aligned k pairs
+constexpr int kLda = kT + 4;  // A rows in LDS: 16-byte aligned groups of 4 k
 
 template <int R>
 struct SmallSmem {
@@ -154,25 +154,38 @@ __device__ __forceinline__ void to_b(int2* B, const Cells<NT, kT, BN>& t, int tx
 }
 
 // acc <- min(acc, A (x) B) over the 32 k of the round; A is BM x 32
-// row-major, B 32 x BN in k pairs, both in LDS.
+// row-major, B 32 x BN in k pairs, both in LDS.  Eight groups of 4 k: each
+// group's operands (one 16-byte A read per row, two 8-byte B reads per
+// column) are loaded while the previous group is computed, so the LDS
+// latency overlaps the relaxations instead of stalling the wave.
 template <int NT, int BM, int BN, bool PK>
 __device__ __forceinline__ void prod_lds(const int32_t* A, const int2* B, Cells<NT, BM, BN>& acc, int tx, int ty) {
-  constexpr int RS = NT / 16;
-#pragma unroll 1
-  for (int kp = 0; kp < kT / 2; ++kp) {
-    uint64_t pb[BN / 16];
+  constexpr int RS = NT / 16, TR = BM / RS, TQ = BN / 16;
+  int4 a[2][TR];
+  int2 b[2][2][TQ];
+  auto load = [&](int buf, int g) {
 #pragma unroll
-    for (int q = 0; q < BN / 16; ++q) {
-      const int2 b = B[kp * BN + tx + 16 * q];
-      pb[q] = pack2(b.x, b.y);
+    for (int r = 0; r < TR; ++r) a[buf][r] = *reinterpret_cast<const int4*>(&A[(ty + RS * r) * kLda + 4 * g]);
+#pragma unroll
+    for (int q = 0; q < TQ; ++q) {
+      b[buf][0][q] = B[(2 * g) * BN + tx + 16 * q];
+      b[buf][1][q] = B[(2 * g + 1) * BN + tx + 16 * q];
     }
+  };
+  load(0, 0);
+#pragma unroll 2
+  for (int g = 0; g < kT / 4; ++g) {
+    const int cur = g & 1;
+    if (g + 1 < kT / 4) load(cur ^ 1, g + 1);
 #pragma unroll
-    for (int r = 0; r < BM / RS; ++r) {
-      const int2 a = *reinterpret_cast<const int2*>(&A[(ty + RS * r) * kLda + 2 * kp]);
-      const uint64_t pa = pack2(a.x, a.y);
+    for (int r = 0; r < TR; ++r) {
+      const uint64_t a0 = pack2(a[cur][r].x, a[cur][r].y), a1 = pack2(a[cur][r].z, a[cur][r].w);
 #pragma unroll
-      for (int q = 0; q < BN / 16; ++q)
-        acc[r][q] = PK ? relax2_pk64(acc[r][q], pa, pb[q]) : relax2_min3(acc[r][q], pa, pb[q]);
+      for (int q = 0; q < TQ; ++q) {
+        const uint64_t b0 = pack2(b[cur][0][q].x, b[cur][0][q].y), b1 = pack2(b[cur][1][q].x, b[cur][1][q].y);
+        acc[r][q] = PK ? relax2_pk64(acc[r][q], a0, b0) : relax2_min3(acc[r][q], a0, b0);
+        acc[r][q] = PK ? relax2_pk64(acc[r][q], a1, b1) : relax2_min3(acc[r][q], a1, b1);
+      }
     }
   }
 }

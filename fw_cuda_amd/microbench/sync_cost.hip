@@ -348,6 +348,7 @@ int main() {
     run("closure 64x64, 512 threads", [&](int r) { k_closures<64, 512><<<1, 512, 0, st>>>(D, r); });
     run("closure 64x64, 1024 threads", [&](int r) { k_closures<64, 1024><<<1, 1024, 0, st>>>(D, r); });
     run("closure 32x32, 256 threads", [&](int r) { k_closures<32, 256><<<1, 256, 0, st>>>(D, r); });
+    run("closure 32x32, 512 threads", [&](int r) { k_closures<32, 512><<<1, 512, 0, st>>>(D, r); });
     run("closure 128x128, 512 threads", [&](int r) { k_closures<128, 512><<<1, 512, 0, st>>>(D, r); });
     run("product 64x64x64, 256 threads", [&](int r) { k_products<16><<<1, 256, 0, st>>>(D, r); });
     CK(hipFree(D));

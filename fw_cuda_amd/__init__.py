@@ -138,6 +138,16 @@ class DeviceMatrix:
         self.free()
 
 
+def last_solve_form():
+    """(form, exact_finish) of this thread's last plain blocked solve
+    (fwx_last_solve_form): form is FORM_MIN3, FORM_PK64, FORM_PK16 or -1 for
+    the fused small-n rounds; exact_finish is 1 when the pk16 run left cells
+    at or above the cap and an exact pk64 run finished the solve."""
+    f, e = ctypes.c_int32(), ctypes.c_int32()
+    check(lib().fwx_last_solve_form(ctypes.byref(f), ctypes.byref(e)))
+    return f.value, e.value
+
+
 def run_device(algo, d_ptr, n, tile=0, kappa=0, stream=None):
     """In-place FW on a device buffer through the registry (fwx_run)."""
     s = ctypes.c_void_p(stream) if stream else None

@@ -50,6 +50,7 @@ SIGNATURES = {
     "fwx_multi_layer_tiling_gpu": (ctypes.c_int, [_vp, _i32, _i32, _i32, _vp]),
     "fwx_tuned_config": (ctypes.c_int, [_i32, _i32p, _i32p]),
     "fwx_relaxation_form": (ctypes.c_int, [_vp, _i32, _i32p, _vp]),
+    "fwx_last_solve_form": (ctypes.c_int, [_i32p, _i32p]),
     "fwx_event_create": (ctypes.c_int, [ctypes.POINTER(_vp)]),
     "fwx_event_record": (ctypes.c_int, [_vp, _vp]),
     "fwx_event_synchronize": (ctypes.c_int, [_vp]),
@@ -87,7 +88,8 @@ SIGNATURES = {
 }
 
 PROF_REST, PROF_LEAD, PROF_BASELINE, PROF_BAND, PROF_NCLASS = 0, 1, 2, 3, 4
-FORM_MIN3, FORM_PK64 = 0, 1
+FORM_MIN3, FORM_PK64, FORM_PK16 = 0, 1, 2
+PK16_CAP = 15359  # include/fwx.h FWX_PK16_CAP
 
 _lib = None
 

@@ -37,6 +37,7 @@
 #include "fwx_device.hpp"
 #include "fwx_internal.hpp"
 #include "fwx_tile.hpp"
+#include "fwx_tile16.hpp"
 
 namespace fwx {
 
@@ -204,6 +205,22 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
   tile_update<R, R, kChunk, VEC, 32, PK>(s, A, B, C, S);
 }
 
+// pk16 shape (fwx_tile16.hpp): 256 threads, 8 x 8 cells each in 32 packed
+// VGPRs.  KC = 32: 52 KB of LDS, three blocks per CU; KC = 16: 26 KB, four
+// (held to 128 VGPRs).
+template <int R, bool VEC, int KC>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(KC == 16 ? 4 : 3, 4))) void k_rest16(
+    int32_t* __restrict__ D, int32_t n, int32_t s0, int32_t S, int32_t sr0, int32_t srn, int32_t m,
+    int32_t per) {
+  __shared__ TileSmem16<R, R, KC> s;
+  int I, J;
+  if (!rest_tile_of_block(m, per, sr0, srn, &I, &J)) return;
+  const View A = make_view(D, n, I * R, s0);
+  const View B = make_view(D, n, s0, J * R);
+  const View C = make_view(D, n, I * R, J * R);
+  tile_update16<R, R, KC, VEC>(s, A, B, C, S);
+}
+
 // ---------------------------------------------------------------------------
 // rest_band: the part of stage s's rest pass that the next stage's lead rounds
 // read -- R x R tiles outside stage s's band [e0, e0+en) with a row or a
@@ -338,9 +355,34 @@ static int band_pass(int32_t* d, int32_t n, int32_t s0, int32_t S, int32_t en, i
   return FWX_OK;
 }
 
-template <bool VEC, bool PK>
+// Chunk depth of the pk16 rest kernel: 32 (default) or FWX_P16_KC=16.
+static int p16_chunk() {
+  const char* e = std::getenv("FWX_P16_KC");
+  return (e && std::atoi(e) == 16) ? 16 : 32;
+}
+
+template <int R, bool VEC>
+static int rest_pass16(int32_t* d, int32_t n, int32_t s0, int32_t S, int32_t excl, hipStream_t st) {
+  const int nr = (n + R - 1) / R;
+  const int sr0 = s0 / R;
+  const int srn = min(excl, nr - sr0);
+  const int m = nr - srn;
+  if (m <= 0) return FWX_OK;
+  const int64_t total = (int64_t)m * m;
+  const int per = (int)((total + 7) / 8);
+  if (p16_chunk() == 16)
+    k_rest16<R, VEC, 16><<<8 * per, 256, 0, st>>>(d, n, s0, S, sr0, srn, m, per);
+  else
+    k_rest16<R, VEC, 32><<<8 * per, 256, 0, st>>>(d, n, s0, S, sr0, srn, m, per);
+  FWX_LAUNCHED("k_rest16");
+  return FWX_OK;
+}
+
+// P16: the 128-wide rest passes run in the pk16 form (the caller checks the
+// result for saturated cells afterwards).
+template <bool VEC, bool PK, bool P16>
 static int rest_dispatch(int R, int32_t* d, int32_t n, int32_t s0, int32_t S, int32_t excl, hipStream_t st) {
-  if (R == 128) return rest_pass<128, VEC, PK>(d, n, s0, S, excl, st);
+  if (R == 128) return P16 ? rest_pass16<128, VEC>(d, n, s0, S, excl, st) : rest_pass<128, VEC, PK>(d, n, s0, S, excl, st);
   if (R == 64) return rest_pass<64, VEC, PK>(d, n, s0, S, excl, st);
   return rest_pass<32, VEC, PK>(d, n, s0, S, excl, st);
 }
@@ -388,7 +430,7 @@ static int lead_stage(int32_t* d, int32_t n, int32_t kb, int32_t B, int32_t N, i
 
 // Plain schedule: lead(s), rest(s), lead(s+1), ... on one stream.  While
 // profiling, one event between adjacent groups ends one and starts the next.
-template <int T, bool VEC, bool PK>
+template <int T, bool VEC, bool PK, bool P16>
 static int run_blocked(int32_t* d, int32_t n, int32_t kappa, hipStream_t st) {
   const int N = (n + T - 1) / T;
   const int span = kappa * T;
@@ -401,7 +443,7 @@ static int run_blocked(int32_t* d, int32_t n, int32_t kappa, hipStream_t st) {
     const double L = (double)min(S, n - s0), nn = (double)n;
     if (int rc = lead_rounds<T, VEC, PK>(d, n, kb, B, N, 0, st)) return rc;
     if (int rc = prof_chain(st, &ev, FWX_PROF_LEAD, lead_work(n, T, kb, B))) return rc;
-    if (int rc = rest_dispatch<VEC, PK>(R, d, n, s0, S, (S + R - 1) / R, st)) return rc;
+    if (int rc = rest_dispatch<VEC, PK, P16>(R, d, n, s0, S, (S + R - 1) / R, st)) return rc;
     if (int rc = prof_chain(st, &ev, FWX_PROF_REST, (nn - L) * (nn - L) * L)) return rc;
   }
   return FWX_OK;
@@ -422,7 +464,7 @@ static bool lead_priority_enabled() {
 // neither band s+1 nor anything lead(s+1) writes (lead(s+1) writes only band
 // s+1 rows/columns; rest(s) reads only band s rows/columns).  The caller's
 // stream waits for lead(s+1) before stage s+1's rest.
-template <int T, bool VEC, bool PK>
+template <int T, bool VEC, bool PK, bool P16>
 static int run_blocked_lookahead(int32_t* d, int32_t n, int32_t kappa, hipStream_t st, hipStream_t side,
                                  hipEvent_t evA, hipEvent_t evB) {
   const int N = (n + T - 1) / T;
@@ -456,11 +498,11 @@ static int run_blocked_lookahead(int32_t* d, int32_t n, int32_t kappa, hipStream
       FWX_HIP(hipStreamWaitEvent(side, ev ? ev : evA, 0));
       if (int rc = lead_stage<T, VEC, PK>(d, n, kb2, B2, N, prio, side)) return rc;
       FWX_HIP(hipEventRecord(evB, side));
-      if (int rc = rest_dispatch<VEC, PK>(R, d, n, s0, S, en + fn, st)) return rc;
+      if (int rc = rest_dispatch<VEC, PK, P16>(R, d, n, s0, S, en + fn, st)) return rc;
       if (int rc = prof_chain(st, &ev, FWX_PROF_REST, (nn - L - L2) * (nn - L - L2) * L)) return rc;
       FWX_HIP(hipStreamWaitEvent(st, evB, 0));
     } else {
-      if (int rc = rest_dispatch<VEC, PK>(R, d, n, s0, S, en, st)) return rc;
+      if (int rc = rest_dispatch<VEC, PK, P16>(R, d, n, s0, S, en, st)) return rc;
       if (int rc = prof_chain(st, &ev, FWX_PROF_REST, (nn - L) * (nn - L) * L)) return rc;
     }
   }
@@ -477,7 +519,7 @@ static int run_blocked_lookahead(int32_t* d, int32_t n, int32_t kappa, hipStream
 // band s and band s+1.  Ordering: rest_other(s) waits for lead(s) (event L);
 // band(s+1) waits for rest_other(s) (event R), which wrote the cells of the
 // band s+2 cross that band(s+1) updates next.
-template <int T, bool VEC, bool PK>
+template <int T, bool VEC, bool PK, bool P16>
 static int run_blocked_sideband(int32_t* d, int32_t n, int32_t kappa, hipStream_t st, hipStream_t side,
                                 hipEvent_t evR, hipEvent_t evL) {
   const int N = (n + T - 1) / T;
@@ -512,12 +554,12 @@ static int run_blocked_sideband(int32_t* d, int32_t n, int32_t kappa, hipStream_
       if (int rc = lead_rounds<T, VEC, PK>(d, n, kb2, B2, N, prio, side)) return rc;
       if (int rc = prof_close(side, es, FWX_PROF_LEAD, lead_work(n, T, kb2, B2))) return rc;
       FWX_HIP(hipEventRecord(evL, side));
-      if (int rc = rest_dispatch<VEC, PK>(R, d, n, s0, S, en + fn, st)) return rc;
+      if (int rc = rest_dispatch<VEC, PK, P16>(R, d, n, s0, S, en + fn, st)) return rc;
       if (int rc = prof_chain(st, &ev, FWX_PROF_REST, (nn - L - L2) * (nn - L - L2) * L)) return rc;
       if (!ev) FWX_HIP(hipEventRecord(evR, st));
       FWX_HIP(hipStreamWaitEvent(side, ev ? ev : evR, 0));
     } else {
-      if (int rc = rest_dispatch<VEC, PK>(R, d, n, s0, S, en, st)) return rc;
+      if (int rc = rest_dispatch<VEC, PK, P16>(R, d, n, s0, S, en, st)) return rc;
       if (int rc = prof_chain(st, &ev, FWX_PROF_REST, (nn - L) * (nn - L) * L)) return rc;
     }
   }
@@ -543,7 +585,7 @@ static int run_blocked_sideband(int32_t* d, int32_t n, int32_t kappa, hipStream_
 // s+2 outside bands s and s+1, which the side stream touches only after
 // event X.  Ordering: X(s), Y(s) wait for lead(s) (event L); band(s+1) waits
 // for X(s) (event X).
-template <int T, bool VEC, bool PK>
+template <int T, bool VEC, bool PK, bool P16>
 static int run_blocked_ahead(int32_t* d, int32_t n, int32_t kappa, hipStream_t st, hipStream_t side,
                              hipEvent_t evX, hipEvent_t evL) {
   const int N = (n + T - 1) / T;
@@ -589,14 +631,14 @@ static int run_blocked_ahead(int32_t* d, int32_t n, int32_t kappa, hipStream_t s
         FWX_HIP(hipEventRecord(evX, st));
         // the side stream's band(s+1) needs X(s) only
         FWX_HIP(hipStreamWaitEvent(side, evX, 0));
-        if (int rc = rest_dispatch<VEC, PK>(R, d, n, s0, S, en + fn + gn, st)) return rc;
+        if (int rc = rest_dispatch<VEC, PK, P16>(R, d, n, s0, S, en + fn + gn, st)) return rc;
         if (int rc = prof_chain(st, &ev, FWX_PROF_REST, (outside2 - L3) * (outside2 - L3) * L)) return rc;
       } else {
-        if (int rc = rest_dispatch<VEC, PK>(R, d, n, s0, S, en + fn, st)) return rc;
+        if (int rc = rest_dispatch<VEC, PK, P16>(R, d, n, s0, S, en + fn, st)) return rc;
         if (int rc = prof_chain(st, &ev, FWX_PROF_REST, (nn - L - L2) * (nn - L - L2) * L)) return rc;
       }
     } else {
-      if (int rc = rest_dispatch<VEC, PK>(R, d, n, s0, S, en, st)) return rc;
+      if (int rc = rest_dispatch<VEC, PK, P16>(R, d, n, s0, S, en, st)) return rc;
       if (int rc = prof_chain(st, &ev, FWX_PROF_REST, (nn - L) * (nn - L) * L)) return rc;
     }
   }
@@ -634,11 +676,11 @@ static int lookahead_mode() {
   return (e && e[0] >= '0' && e[0] <= '3') ? e[0] - '0' : 2;
 }
 
-template <int T, bool VEC, bool PK>
+template <int T, bool VEC, bool PK, bool P16>
 static int run_any(int32_t* d, int32_t n, int32_t kappa, hipStream_t st) {
   const int N = (n + T - 1) / T;
   const int mode = lookahead_mode();
-  if (mode == 0 || N <= kappa) return run_blocked<T, VEC, PK>(d, n, kappa, st);
+  if (mode == 0 || N <= kappa) return run_blocked<T, VEC, PK, P16>(d, n, kappa, st);
   hipStream_t side;
   if (int rc = side_stream(&side)) return rc;
   hipEvent_t evA, evB;
@@ -648,33 +690,37 @@ static int run_any(int32_t* d, int32_t n, int32_t kappa, hipStream_t st) {
     (void)hipEventDestroy(evA);
     return hip_fail(e, "hipEventCreateWithFlags", __FILE__, __LINE__);
   }
-  const int rc = mode == 3   ? run_blocked_ahead<T, VEC, PK>(d, n, kappa, st, side, evA, evB)
-                 : mode == 2 ? run_blocked_sideband<T, VEC, PK>(d, n, kappa, st, side, evA, evB)
-                             : run_blocked_lookahead<T, VEC, PK>(d, n, kappa, st, side, evA, evB);
+  const int rc = mode == 3   ? run_blocked_ahead<T, VEC, PK, P16>(d, n, kappa, st, side, evA, evB)
+                 : mode == 2 ? run_blocked_sideband<T, VEC, PK, P16>(d, n, kappa, st, side, evA, evB)
+                             : run_blocked_lookahead<T, VEC, PK, P16>(d, n, kappa, st, side, evA, evB);
   (void)hipEventDestroy(evA);
   (void)hipEventDestroy(evB);
   return rc;
 }
 
-template <bool VEC, bool PK>
+template <bool VEC, bool PK, bool P16>
 static int by_tile(int32_t* d, int32_t n, int32_t tile, int32_t kappa, hipStream_t st) {
   switch (tile) {
     case 32:
-      return run_any<32, VEC, PK>(d, n, kappa, st);
+      return run_any<32, VEC, PK, P16>(d, n, kappa, st);
     case 64:
-      return run_any<64, VEC, PK>(d, n, kappa, st);
+      return run_any<64, VEC, PK, P16>(d, n, kappa, st);
     case 128:
-      return run_any<128, VEC, PK>(d, n, kappa, st);
+      return run_any<128, VEC, PK, P16>(d, n, kappa, st);
     default:
       return fail(FWX_ERR_INVALID_ARG, "tile must be 32, 64 or 128");
   }
 }
 
-static int enqueue_direct(int32_t* d, int32_t n, int32_t tile, int32_t kappa, bool vec, bool pk,
+// form: 0 = min3, 1 = pk64, 2 = pk64 with the 128-wide rest passes in pk16.
+static int enqueue_direct(int32_t* d, int32_t n, int32_t tile, int32_t kappa, bool vec, int form,
                           hipStream_t st) {
-  if (vec)
-    return pk ? by_tile<true, true>(d, n, tile, kappa, st) : by_tile<true, false>(d, n, tile, kappa, st);
-  return pk ? by_tile<false, true>(d, n, tile, kappa, st) : by_tile<false, false>(d, n, tile, kappa, st);
+  if (vec) {
+    if (form == 2) return by_tile<true, true, true>(d, n, tile, kappa, st);
+    return form ? by_tile<true, true, false>(d, n, tile, kappa, st) : by_tile<true, false, false>(d, n, tile, kappa, st);
+  }
+  if (form == 2) return by_tile<false, true, true>(d, n, tile, kappa, st);
+  return form ? by_tile<false, true, false>(d, n, tile, kappa, st) : by_tile<false, false, false>(d, n, tile, kappa, st);
 }
 
 // ---------------------------------------------------------------------------
@@ -719,7 +765,8 @@ namespace {
 struct GraphKey {
   int32_t* d;
   int32_t n, tile, kappa;
-  bool vec, pk;
+  bool vec;
+  int form;  // 0 min3, 1 pk64, 2 pk64 with pk16 rest passes
   bool small;  // the fused small-n rounds (each block then picks the form itself)
   int small_cfg;  // their region width and threads (small_region_knob)
   int la;
@@ -727,7 +774,7 @@ struct GraphKey {
   hipStream_t st;
   int dev;
   bool operator==(const GraphKey& o) const {
-    return d == o.d && n == o.n && tile == o.tile && kappa == o.kappa && vec == o.vec && pk == o.pk &&
+    return d == o.d && n == o.n && tile == o.tile && kappa == o.kappa && vec == o.vec && form == o.form &&
            small == o.small && small_cfg == o.small_cfg && la == o.la && knobs == o.knobs && st == o.st &&
            dev == o.dev;
   }
@@ -753,20 +800,28 @@ bool graphs_enabled() {
 
 }  // namespace
 
-int enqueue_blocked(int32_t* d, int32_t n, int32_t tile, int32_t kappa, hipStream_t st) {
-  const bool vec = (n % 4 == 0) && ((reinterpret_cast<uintptr_t>(d) & 15u) == 0);
-  const bool small = small_path(n, tile, kappa);
-  const bool direct = n > kGraphMaxN || !graphs_enabled() || profiling();
-  if (small && direct) return enqueue_small(d, n, st);
-  bool pk = false;
-  if (!small)
-    if (int rc = values_in_pk64_range(d, (int64_t)n * n, st, &pk)) return rc;
-  if (direct) return enqueue_direct(d, n, tile, kappa, vec, pk, st);
+// The pk16 rest passes (fwx_tile16.hpp) run whenever the pk64 form is exact
+// and the rest pass is 128 wide; FWX_PK16=0 keeps the pk64 form throughout.
+static bool pk16_enabled() {
+  const char* e = std::getenv("FWX_PK16");
+  return !(e && e[0] == '0');
+}
+
+static int rest_width(int32_t tile, int32_t kappa) {
+  const int span = kappa * tile;
+  return (span % 128 == 0) ? 128 : (span % 64 == 0) ? 64 : 32;
+}
+
+// One solve in the given form: launched directly, or replayed from the graph
+// cache (captured on first use).
+static int run_form(int32_t* d, int32_t n, int32_t tile, int32_t kappa, bool vec, int form, bool small,
+                    bool direct, hipStream_t st) {
+  if (direct) return small ? enqueue_small(d, n, st) : enqueue_direct(d, n, tile, kappa, vec, form, st);
   int dev = 0;
   FWX_HIP(hipGetDevice(&dev));
   const int knobs = (fused_dpanel() ? 1 : 0) | (rest_variant(n) << 1) | (band_narrow() ? 8 : 0) | (diag_threads() << 4) |
-                    (lead_priority_enabled() ? 1 << 20 : 0);
-  const GraphKey key{d, n, tile, kappa, vec, pk, small, small ? small_region_knob(n) : 0, lookahead_mode(), knobs, st,
+                    (lead_priority_enabled() ? 1 << 20 : 0) | (p16_chunk() == 16 ? 1 << 21 : 0);
+  const GraphKey key{d, n, tile, kappa, vec, form, small, small ? small_region_knob(n) : 0, lookahead_mode(), knobs, st,
                      dev};
   std::lock_guard<std::mutex> lk(g_graph_mu);
   auto replay = [&](GraphEntry& e) -> int {
@@ -788,7 +843,7 @@ int enqueue_blocked(int32_t* d, int32_t n, int32_t tile, int32_t kappa, hipStrea
   hipError_t e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
   int rc = FWX_OK;
   t_capture_side = cap_side;
-  if (e == hipSuccess) rc = small ? enqueue_small(d, n, cap) : enqueue_direct(d, n, tile, kappa, vec, pk, cap);
+  if (e == hipSuccess) rc = small ? enqueue_small(d, n, cap) : enqueue_direct(d, n, tile, kappa, vec, form, cap);
   t_capture_side = nullptr;
   hipGraph_t graph = nullptr;
   const hipError_t e2 = hipStreamEndCapture(cap, &graph);
@@ -815,6 +870,38 @@ int enqueue_blocked(int32_t* d, int32_t n, int32_t tile, int32_t kappa, hipStrea
   }
   g_graphs.push_back(GraphEntry{key, graph, exec, 0});
   return replay(g_graphs.back());
+}
+
+static thread_local int t_last_form = -1;
+static thread_local int t_last_exact_finish = 0;
+
+void last_solve_form(int32_t* form, int32_t* exact_finish) {
+  *form = t_last_form;
+  *exact_finish = t_last_exact_finish;
+}
+
+int enqueue_blocked(int32_t* d, int32_t n, int32_t tile, int32_t kappa, hipStream_t st) {
+  const bool vec = (n % 4 == 0) && ((reinterpret_cast<uintptr_t>(d) & 15u) == 0);
+  const bool small = small_path(n, tile, kappa);
+  const bool direct = n > kGraphMaxN || !graphs_enabled() || profiling();
+  t_last_form = -1;
+  t_last_exact_finish = 0;
+  if (small) return run_form(d, n, tile, kappa, vec, 0, true, direct, st);
+  bool pk = false;
+  if (int rc = values_in_pk64_range(d, (int64_t)n * n, st, &pk)) return rc;
+  const int form = !pk ? 0 : (pk16_enabled() && rest_width(tile, kappa) == 128) ? 2 : 1;
+  t_last_form = form;
+  if (int rc = run_form(d, n, tile, kappa, vec, form, false, direct, st)) return rc;
+  if (form != 2) return FWX_OK;
+  // Every cell below the pk16 cap is exact.  If any cell is at or above it,
+  // the matrix (walk lengths of the input graph, each at most its input
+  // value) is solved once more in the exact pk64 form: its closure equals
+  // the input's.
+  bool below = false;
+  if (int rc = values_below(d, (int64_t)n * n, (int32_t)kP16Cap, st, &below)) return rc;
+  if (below) return FWX_OK;
+  t_last_exact_finish = 1;
+  return run_form(d, n, tile, kappa, vec, 1, false, direct, st);
 }
 
 }  // namespace fwx

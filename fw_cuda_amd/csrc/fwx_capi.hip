@@ -229,6 +229,13 @@ int fwx_relaxation_form(const int32_t* d_D, int32_t n, int32_t* form, void* stre
   return FWX_OK;
 }
 
+int fwx_last_solve_form(int32_t* form, int32_t* exact_finish) {
+  clear_error();
+  if (!form || !exact_finish) return fail(FWX_ERR_INVALID_ARG, "null output pointer");
+  last_solve_form(form, exact_finish);
+  return FWX_OK;
+}
+
 // ---- events (the cudaEvent calls of src/timer.cu:48-63) ---------------------
 int fwx_event_create(void** ev) {
   clear_error();

@@ -35,6 +35,8 @@ inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s);
 // Plain-FW drivers (enqueue only; the C entry points add the final sync).
 int enqueue_baseline(int32_t* d, int32_t n, hipStream_t st);
 int enqueue_blocked(int32_t* d, int32_t n, int32_t tile, int32_t kappa, hipStream_t st);
+// The form of this thread's last enqueue_blocked (fwx_last_solve_form).
+void last_solve_form(int32_t* form, int32_t* exact_finish);
 int enqueue_refshape(int32_t* d, int32_t n, hipStream_t st);
 // The fused rounds of a small solve on 32-wide k-tiles (fwx_small.hip), and
 // the region width they use for size n (a graph-cache key).
@@ -73,6 +75,9 @@ int flag_release(hipStream_t st, DeviceFlag* fl, int* value);
 // [0, 2^30), i.e. the pk64 relaxation form is exact for the whole run.
 // Synchronises `st`.
 int values_in_pk64_range(const int32_t* d, int64_t count, hipStream_t st, bool* ok);
+// *ok = every one of count cells at d is below cap (fwx_range.hip); the
+// check after a solve with pk16 rest passes.  Synchronises `st`.
+int values_below(const int32_t* d, int64_t count, int32_t cap, hipStream_t st, bool* ok);
 // The per-device second stream of the look-ahead schedules (fwx_blocked.hip).
 int side_stream(hipStream_t* out);
 // Any D[i][i] < 0 on a device matrix (fwx_range.hip); synchronises `st`.

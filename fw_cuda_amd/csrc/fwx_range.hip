@@ -156,6 +156,48 @@ int values_in_pk64_range(const int32_t* d, int64_t count, hipStream_t st, bool* 
 
 namespace {
 
+// Any cell >= cap -> *f = 1 (signed compare).
+__global__ __launch_bounds__(256) void k_at_least(const int32_t* __restrict__ d, int64_t count, int vec4,
+                                                  int32_t cap, int* __restrict__ f) {
+  int32_t mx = INT32_MIN;
+  const int64_t tid = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int64_t stride = (int64_t)gridDim.x * 256;
+  int64_t done = 0;
+  if (vec4) {
+    const int4* d4 = reinterpret_cast<const int4*>(d);
+    const int64_t n4 = count / 4;
+    for (int64_t i = tid; i < n4; i += stride) {
+      const int4 v = d4[i];
+      mx = max(mx, max(max(v.x, v.y), max(v.z, v.w)));
+    }
+    done = n4 * 4;
+  }
+  for (int64_t i = done + tid; i < count; i += stride) mx = max(mx, d[i]);
+  if (__ballot(mx >= cap) != 0ull && (threadIdx.x & 63) == 0) atomicOr(f, 1);
+}
+
+}  // namespace
+
+int values_below(const int32_t* d, int64_t count, int32_t cap, hipStream_t st, bool* ok) {
+  const int64_t work = (count + 3) / 4;
+  const int blocks = (int)std::min<int64_t>((work + 255) / 256, 4096);
+  const int vec4 = (reinterpret_cast<uintptr_t>(d) & 15u) == 0;
+  int h = 1;
+  const int rc = flag_query(
+      st,
+      [&](int* f) -> int {
+        k_at_least<<<blocks > 0 ? blocks : 1, 256, 0, st>>>(d, count, vec4, cap, f);
+        FWX_LAUNCHED("k_at_least");
+        return FWX_OK;
+      },
+      &h);
+  if (rc) return rc;
+  *ok = (h == 0);
+  return FWX_OK;
+}
+
+namespace {
+
 // Any D[i][i] < 0 -> *f = 1.  One diagonal cell per thread.
 __global__ __launch_bounds__(256) void k_negative_diagonal(const int32_t* __restrict__ d, int32_t n,
                                                             int* __restrict__ f) {

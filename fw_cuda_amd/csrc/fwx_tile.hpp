@@ -240,8 +240,8 @@ struct TileProduct {
 
   __device__ __forceinline__ static int32_t keep(bool ok, int32_t v) { return ok ? v : kInf; }
 
-  // registers -> LDS buffer `buf`.  Cells that fetch() clamped become INF.
-  __device__ __forceinline__ void stash(Smem& s, int buf, const View& A, const View& B) {
+  // Cells that fetch() clamped into the matrix become INF.
+  __device__ __forceinline__ void mask_edges(const View& A, const View& B) {
     const int tid = block_tid<OPQ>();
     if (!full_) {
 #pragma unroll
@@ -270,6 +270,12 @@ struct TileProduct {
         rb1[u].w = keep(r1 && j + 3 < B.clim, rb1[u].w);
       }
     }
+  }
+
+  // registers -> LDS buffer `buf`.  Cells that fetch() clamped become INF.
+  __device__ __forceinline__ void stash(Smem& s, int buf, const View& A, const View& B) {
+    const int tid = block_tid<OPQ>();
+    mask_edges(A, B);
 #pragma unroll
     for (int p = 0; p < APASS; ++p) {
       const int r = p * AROWS + tid / AG;

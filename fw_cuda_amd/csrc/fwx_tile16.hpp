@@ -1,0 +1,251 @@
+// fwx_tile16.hpp -- the packed 16-bit (pk16) min-plus tile product for gfx950.
+//
+//   C[BM x BN] <- min(C, A[BM x K] (x) B[K x BN])   on values clamped at CAP
+//
+// Encoding.  A value v >= 0 is carried as e(v) = min(v, CAP) in a 16-bit lane
+// (CAP = 0x3BFF); A operands and C cells carry e(v) + BIAS (BIAS = 0x0400).
+// Per k pair (k0, k1) and cell pair (j0, j1) of row i, ONE 64-bit add
+//   (a_k0, a_k0, a_k1, a_k1) + (b_k0j0, b_k0j1, b_k1j0, b_k1j1)   (16-bit lanes)
+// forms four sums (each below 0x7BFF: no lane carries), and ONE
+// v_pk_minimum3_f16 folds them into the cell pair:
+//   c = min3(c, sums(k0), sums(k1)) per lane.
+// Every compared pattern lies in [BIAS, 0x7BFF], where binary16 numbers are
+// positive, normal and finite and order exactly as their unsigned bit
+// patterns, so the f16 minimum is the unsigned minimum (checked on the device
+// by fw_cuda_amd/microbench/pk16_forms.hip over every pattern).  That is 2
+// VALU ops per 4 relaxations, half of the pk64 form's 1 op per relaxation.
+//
+// Exactness.  Let h(x) = min(x, CAP).  For x, y >= 0, h(min(x, y)) =
+// min(h(x), h(y)) and h(x + y) = h(h(x) + h(y)), so a product on encoded
+// values yields h of the exact product: a result below CAP is the exact
+// relaxation result; a result at CAP means the exact result is >= CAP, and
+// the cell then keeps its old value (>= CAP as well, since its own encoding
+// was CAP) -- store_c skips it.  So the matrix always holds walk lengths,
+// and h(matrix) evolves exactly as h(FW state): after the run every cell
+// below CAP holds its exact distance.  The caller checks for cells >= CAP
+// afterwards and, if any, finishes with an exact int32 run on the matrix
+// (fwx_blocked.hip).  Inputs must be >= 0 (the pk64 range check holds).
+//
+// Layout (256 threads, 16 x 16; thread (tx, ty) holds rows ty + 16 r and the
+// 4-column groups 4 (tx + 16 g) .. +3, i.e. cell pairs 2 (tx + 16 g) + {0, 1}):
+//   A chunk: row-major u32 e+BIAS duplicated into both lanes, row stride
+//            tile_lda<KC> (the int32 product's A layout, same bytes);
+//   B chunk: per k pair, the BN/2 column pairs in order, one 8-byte slot each
+//            (low word: row k0's pair, high word: row k1's pair).
+// A compute read is one 16-byte read per row (4 k, broadcast across the 16
+// lanes of a row), a B read one 16-byte read per group and k pair (16 lanes
+// on 256 consecutive bytes); the stash writes 16 bytes per lane on
+// consecutive addresses -- conflict-free under gfx950's LDS bank model
+// (tests/test_lds_banks.py).
+#pragma once
+
+#include "fwx_tile.hpp"
+
+namespace fwx {
+
+constexpr uint32_t kP16Bias = 0x0400u;
+constexpr uint32_t kP16Cap = 0x3BFFu;
+constexpr uint32_t kP16Bias2 = kP16Bias | (kP16Bias << 16);
+
+// v_pk_minimum3_f16 (IEEE minimum of three, per 16-bit lane)
+__device__ __forceinline__ uint32_t pkmin3_16(uint32_t a, uint32_t b, uint32_t c) {
+  uint32_t r;
+  asm("v_pk_minimum3_f16 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+  return r;
+}
+
+// two k steps on a cell pair: a = (a_k0, a_k0, a_k1, a_k1), b = (b_k0j0,
+// b_k0j1, b_k1j0, b_k1j1), 16-bit lanes low to high
+__device__ __forceinline__ uint32_t relax4_pk16(uint32_t c, uint64_t a, uint64_t b) {
+  const uint64_t s = a + b;
+  return pkmin3_16(c, (uint32_t)s, (uint32_t)(s >> 32));
+}
+
+// A operand: e(v) + BIAS in both lanes (v >= 0)
+__device__ __forceinline__ uint32_t p16_dup(int32_t v) {
+  return __umul24(min((uint32_t)v, kP16Cap), 0x10001u) + kP16Bias2;
+}
+
+// B operand pair: e(x) | e(y) << 16
+__device__ __forceinline__ uint32_t p16_pair(int32_t x, int32_t y) {
+  return min((uint32_t)x, kP16Cap) | (min((uint32_t)y, kP16Cap) << 16);
+}
+
+template <int BM, int BN, int KC>
+struct alignas(16) TileSmem16 {
+  alignas(16) uint32_t a[2][BM * tile_lda<KC>()];
+  alignas(16) uint64_t b[2][(KC / 2) * (BN / 2)];
+};
+
+template <int BM, int BN, int KC, bool VEC, bool EDGE>
+struct TileProduct16 : TileProduct<BM, BN, KC, VEC, 16, true, EDGE> {
+  using Base = TileProduct<BM, BN, KC, VEC, 16, true, EDGE>;
+  using Base::AG;
+  using Base::APASS;
+  using Base::AROWS;
+  using Base::BPASS;
+  using Base::BU;
+  using Base::LDA;
+  using Base::NT;
+  using Base::ra;
+  using Base::rb0;
+  using Base::rb1;
+  static constexpr int TM = BM / 16;  // rows per thread
+  static constexpr int TG = BN / 64;  // 4-column groups per thread
+  static constexpr int TP = 2 * TG;   // cell pairs per thread
+  static_assert(BN % 64 == 0 && BM % 16 == 0, "pk16 block: 16 x 16 threads, 4-column groups");
+  using Smem = TileSmem16<BM, BN, KC>;
+
+  __device__ __forceinline__ void stash16(Smem& s, int buf, const View& A, const View& B) {
+    const int tid = threadIdx.x;
+    this->mask_edges(A, B);
+#pragma unroll
+    for (int p = 0; p < APASS; ++p) {
+      const int r = p * AROWS + tid / AG;
+      const int k = (tid % AG) * 4;
+      if (BM % AROWS == 0 || r < BM) {
+        const uint4 e = make_uint4(p16_dup(ra[p].x), p16_dup(ra[p].y), p16_dup(ra[p].z), p16_dup(ra[p].w));
+        reinterpret_cast<uint4*>(s.a[buf])[(r * LDA + k) / 4] = e;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < BPASS; ++u) {
+      const int idx = u * NT + tid;
+      if (BU % NT == 0 || idx < BU) {
+        const int pr = idx / (BN / 4);
+        const int j = (idx % (BN / 4)) * 4;
+        ulonglong2 v;
+        v.x = (uint64_t)p16_pair(rb0[u].x, rb0[u].y) | ((uint64_t)p16_pair(rb1[u].x, rb1[u].y) << 32);
+        v.y = (uint64_t)p16_pair(rb0[u].z, rb0[u].w) | ((uint64_t)p16_pair(rb1[u].z, rb1[u].w) << 32);
+        reinterpret_cast<ulonglong2*>(s.b[buf])[(pr * (BN / 2) + j / 2) / 2] = v;
+      }
+    }
+  }
+
+  __device__ __forceinline__ static void compute16(const Smem& s, int buf, uint32_t (&acc)[TM][TP]) {
+    const int tid = threadIdx.x;
+    const int tx = tid % 16;
+    const int ty = tid / 16;
+    const uint32_t* As = s.a[buf];
+    const uint64_t* Bs = s.b[buf];
+#pragma unroll 1
+    for (int q = 0; q < KC / 4; ++q) {
+      uint64_t b[2][TP];
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+#pragma unroll
+        for (int g = 0; g < TG; ++g) {
+          const ulonglong2 v =
+              *reinterpret_cast<const ulonglong2*>(&Bs[(2 * q + h) * (BN / 2) + 2 * (tx + 16 * g)]);
+          b[h][2 * g] = v.x;
+          b[h][2 * g + 1] = v.y;
+        }
+#pragma unroll
+      for (int r = 0; r < TM; ++r) {
+        const uint4 a = *reinterpret_cast<const uint4*>(&As[(ty + 16 * r) * LDA + 4 * q]);
+        const uint64_t a01 = (uint64_t)a.x | ((uint64_t)a.y << 32);
+        const uint64_t a23 = (uint64_t)a.z | ((uint64_t)a.w << 32);
+#pragma unroll
+        for (int p = 0; p < TP; ++p) {
+          acc[r][p] = relax4_pk16(acc[r][p], a01, b[0][p]);
+          acc[r][p] = relax4_pk16(acc[r][p], a23, b[1][p]);
+        }
+      }
+    }
+  }
+
+  // acc <- min(acc, A (x) B) over k in [0, depth), depth a multiple of KC.
+  __device__ __forceinline__ void run16(Smem& s, const View& A, const View& B, int32_t depth,
+                                        uint32_t (&acc)[TM][TP]) {
+    const int nch = depth / KC;
+    this->fetch(A, B, 0);
+    stash16(s, 0, A, B);
+    __syncthreads();
+    for (int ch = 0; ch < nch; ++ch) {
+      const int buf = ch & 1;
+      if (ch + 1 < nch) this->fetch(A, B, (ch + 1) * KC);
+      compute16(s, buf, acc);
+      if (ch + 1 < nch) stash16(s, buf ^ 1, A, B);
+      __syncthreads();
+    }
+  }
+
+  // C cells: e(v) + BIAS per lane; out-of-range cells are INF (e = CAP).
+  __device__ __forceinline__ static void load_c16(const View& C, uint32_t (&acc)[TM][TP]) {
+    const int tid = threadIdx.x;
+    const int tx = tid % 16;
+    const int ty = tid / 16;
+#pragma unroll
+    for (int r = 0; r < TM; ++r)
+#pragma unroll
+      for (int g = 0; g < TG; ++g) {
+        const int i = ty + 16 * r, j = 4 * (tx + 16 * g);
+        int4 v;
+        if (!EDGE && VEC) {
+          v = *reinterpret_cast<const int4*>(C.at(i, j));
+        } else if (!EDGE) {
+          const int32_t* q = C.at(i, j);
+          v = make_int4(q[0], q[1], q[2], q[3]);
+        } else {
+          v.x = C.in(i, j + 0) ? *C.at(i, j + 0) : kInf;
+          v.y = C.in(i, j + 1) ? *C.at(i, j + 1) : kInf;
+          v.z = C.in(i, j + 2) ? *C.at(i, j + 2) : kInf;
+          v.w = C.in(i, j + 3) ? *C.at(i, j + 3) : kInf;
+        }
+        acc[r][2 * g] = p16_pair(v.x, v.y) + kP16Bias2;
+        acc[r][2 * g + 1] = p16_pair(v.z, v.w) + kP16Bias2;
+      }
+  }
+
+  // Stores every cell whose result is below CAP; a cell at CAP keeps its old
+  // value (which is >= CAP, see the header).
+  __device__ __forceinline__ static void store_c16(const View& C, const uint32_t (&acc)[TM][TP]) {
+    uint32_t tid = threadIdx.x;
+    asm("" : "+v"(tid));
+    const int tx = tid % 16;
+    const int ty = tid / 16;
+#pragma unroll
+    for (int r = 0; r < TM; ++r)
+#pragma unroll
+      for (int g = 0; g < TG; ++g) {
+        const int i = ty + 16 * r, j = 4 * (tx + 16 * g);
+        const uint32_t p0 = acc[r][2 * g] - kP16Bias2, p1 = acc[r][2 * g + 1] - kP16Bias2;
+        const int32_t v0 = (int32_t)(p0 & 0xFFFFu), v1 = (int32_t)(p0 >> 16);
+        const int32_t v2 = (int32_t)(p1 & 0xFFFFu), v3 = (int32_t)(p1 >> 16);
+        const bool k0 = v0 < (int32_t)kP16Cap, k1 = v1 < (int32_t)kP16Cap;
+        const bool k2 = v2 < (int32_t)kP16Cap, k3 = v3 < (int32_t)kP16Cap;
+        if (!EDGE && VEC && k0 && k1 && k2 && k3) {
+          *reinterpret_cast<int4*>(C.at(i, j)) = make_int4(v0, v1, v2, v3);
+        } else {
+          if (k0 && (!EDGE || C.in(i, j + 0))) *C.at(i, j + 0) = v0;
+          if (k1 && (!EDGE || C.in(i, j + 1))) *C.at(i, j + 1) = v1;
+          if (k2 && (!EDGE || C.in(i, j + 2))) *C.at(i, j + 2) = v2;
+          if (k3 && (!EDGE || C.in(i, j + 3))) *C.at(i, j + 3) = v3;
+        }
+      }
+  }
+};
+
+// C <- min(C, A (x) B) in the pk16 form by one 256-thread workgroup; blocks
+// wholly inside the matrix run the edge-free instantiation.
+template <int BM, int BN, int KC, bool VEC>
+__device__ __forceinline__ void tile_update16(TileSmem16<BM, BN, KC>& s, const View& A, const View& B,
+                                              const View& C, int32_t depth) {
+  using PF = TileProduct16<BM, BN, KC, VEC, false>;
+  using PE = TileProduct16<BM, BN, KC, VEC, true>;
+  uint32_t acc[PF::TM][PF::TP];
+  if (C.rlim >= BM && C.clim >= BN && A.rlim >= BM && A.clim >= depth && B.rlim >= depth &&
+      B.clim >= BN) {
+    PF::load_c16(C, acc);
+    PF p;
+    p.run16(s, A, B, depth, acc);
+    PF::store_c16(C, acc);
+  } else {
+    PE::load_c16(C, acc);
+    PE p;
+    p.run16(s, A, B, depth, acc);
+    PE::store_c16(C, acc);
+  }
+}
+
+}  // namespace fwx

@@ -98,6 +98,21 @@ int fwx_tuned_config(int32_t n, int32_t* tile, int32_t* kappa);
 #define FWX_FORM_PK64 1
 int fwx_relaxation_form(const int32_t* d_D, int32_t n, int32_t* form, void* stream);
 
+/*
+ * What the calling thread's last plain blocked solve (tiling_gpu /
+ * multi_layer_tiling_gpu) ran in.  *form: FWX_FORM_MIN3, FWX_FORM_PK64, or
+ * FWX_FORM_PK16 -- pk64 lead rounds with the 128-wide rest passes in the
+ * packed 16-bit form (2 VALU instructions per 4 relaxations on values
+ * clamped at FWX_PK16_CAP; exact for every cell below the cap) -- or -1 for
+ * the fused small-n rounds (each block picks pk64 or min3 itself).
+ * *exact_finish = 1 when cells at or above FWX_PK16_CAP remained after the
+ * pk16 run and the solve finished with an exact pk64 run on its result.
+ * -1 / 0 before any solve.  No reference counterpart.
+ */
+#define FWX_FORM_PK16 2
+#define FWX_PK16_CAP 15359
+int fwx_last_solve_form(int32_t* form, int32_t* exact_finish);
+
 /* ---------------- device memory (reference GPUMemory, src/gpu_memory.cu) - */
 
 int fwx_device_alloc(int32_t** d_D, int32_t n);                        /* gpu_memory.cu:6-11 */

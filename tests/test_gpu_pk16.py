@@ -1,0 +1,120 @@
+"""The pk16 rest passes on the GPU (fwx_tile16.hpp, through the C ABI):
+bit-exact parity with the oracle in every regime of the 16-bit cap --
+nothing saturates (the benchmark graphs), unreachable pairs, weights around
+the cap, distances straddling it -- on graph-replayed and direct sizes,
+ragged n, every (tile, kappa) whose rest pass is 128 wide; the form and the
+exact finish reported by fwx_last_solve_form; FWX_PK16=0 identical.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import fw_cuda_amd as fx
+from fw_cuda_amd import _lib
+from oracle import pyoracle as po
+
+INF = fx.INF
+CAP = _lib.PK16_CAP
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def graph(n, p, lo, hi, seed):
+    rng = np.random.default_rng(seed)
+    W = np.where(rng.random((n, n)) < p, rng.integers(lo, hi + 1, (n, n)), INF).astype(np.int32)
+    np.fill_diagonal(W, 0)
+    return W
+
+
+# (name, n, density, wmin, wmax, seed, exact finish expected)
+CASES = [
+    ("dense small weights", 1024, 0.9, 1, 1000, 1, 0),
+    ("ragged dense", 1001, 0.5, 1, 1000, 2, 0),
+    ("sparse, unreachable pairs", 1000, 0.002, 1, 1000, 3, 1),
+    ("weights around the cap", 768, 0.5, CAP - 3, CAP + 3, 4, 1),
+    ("distances straddling the cap", 900, 0.02, 1000, 9000, 5, None),
+]
+
+
+@pytest.fixture(scope="module")
+def cases():
+    out = {}
+    for c in CASES:
+        W = graph(*c[1:6])
+        out[c[0]] = (c, po.fw_rowpar(W), W)
+    return out
+
+
+@pytest.mark.parametrize("tile,kappa", [(64, 2), (64, 4), (128, 1), (128, 2), (32, 4)])
+@pytest.mark.parametrize("name", [c[0] for c in CASES])
+def test_pk16_bit_exact(cases, name, tile, kappa):
+    (_, n, _, _, _, _, finish), want, W = cases[name]
+    say("pk16", name, n, tile, kappa)
+    got = fx.floyd_warshall(W, "multi_layer_tiling_gpu", tile, kappa)
+    form, exact = fx.last_solve_form()
+    assert form == _lib.FORM_PK16
+    if finish is not None:
+        assert exact == finish, name
+    assert (got == want).all(), name
+
+
+@pytest.mark.parametrize("name", ["dense small weights", "sparse, unreachable pairs"])
+def test_pk16_off_is_identical(monkeypatch, cases, name):
+    (_, n, _, _, _, _, _), want, W = cases[name]
+    monkeypatch.setenv("FWX_PK16", "0")
+    got = fx.floyd_warshall(W, "multi_layer_tiling_gpu", 64, 2)
+    assert fx.last_solve_form() == (_lib.FORM_PK64, 0)
+    assert (got == want).all()
+
+
+@pytest.mark.parametrize("kc", ["16", "32"])
+def test_pk16_chunk_depths(monkeypatch, cases, kc):
+    (_, n, _, _, _, _, _), want, W = cases["ragged dense"]
+    monkeypatch.setenv("FWX_P16_KC", kc)
+    got = fx.floyd_warshall(W, "multi_layer_tiling_gpu", 128, 2)
+    assert (got == want).all()
+
+
+def test_pk16_forms_follow_the_range_check():
+    """Negative weights keep min3, operands >= 2^30 keep min3, a 64-wide rest
+    pass keeps pk64; each exact."""
+    rng = np.random.default_rng(9)
+    W = graph(640, 0.3, 1, 1000, 9)
+    W[rng.integers(0, 640, 50), rng.integers(0, 640, 50)] = 1 << 30
+    np.fill_diagonal(W, 0)
+    got = fx.floyd_warshall(W, "multi_layer_tiling_gpu", 64, 2)
+    assert fx.last_solve_form()[0] == _lib.FORM_MIN3
+    assert (got == po.fw_rowpar(W)).all()
+    W = graph(640, 0.3, 1, 1000, 10)
+    got = fx.floyd_warshall(W, "multi_layer_tiling_gpu", 64, 1)
+    assert fx.last_solve_form()[0] == _lib.FORM_PK64
+    assert (got == po.fw_rowpar(W)).all()
+
+
+def test_pk16_direct_launches_large_ragged():
+    """n above the graph-replay limit: direct launches, 16-byte-unaligned rows."""
+    W = graph(4102, 0.02, 1, 1000, 11)
+    say("pk16 direct n=4102")
+    got = fx.floyd_warshall(W, "multi_layer_tiling_gpu", 64, 2)
+    assert fx.last_solve_form()[0] == _lib.FORM_PK16
+    assert (got == po.fw_rowpar(W)).all()
+
+
+def test_pk16_repeated_replays_alternate_finish(cases):
+    """One buffer solved repeatedly with inputs that do and do not saturate
+    (graph replay of both forms, the exact finish in between)."""
+    (_, n, _, _, _, _, _), want_a, Wa = cases["dense small weights"]
+    Wb = graph(1024, 0.001, 1, 1000, 12)
+    want_b = po.fw_rowpar(Wb)
+    with fx.DeviceMatrix(1024) as dm:
+        for i in range(4):
+            W, want = (Wa, want_a) if i % 2 == 0 else (Wb, want_b)
+            dm.upload(W)
+            fx.run_device("multi_layer_tiling_gpu", dm.ptr, 1024, 64, 2)
+            assert fx.last_solve_form() == (_lib.FORM_PK16, i % 2)
+            assert (dm.download() == want).all(), i

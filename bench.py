@@ -31,11 +31,17 @@ CUS, LANES, CLK = 256, 64, 2.4e9
 PEAK_2OP = CUS * LANES * CLK / 2.0
 
 
-def instruction_ceiling(cus, form_pk64):
-    """Relaxations/s the shipped kernels can issue at the published clock:
-    the pk64 form is one VALU lane-op per relaxation (CUs x 64 lanes x
-    2.4 GHz = 3.93e13 at 256 CUs), the min3 form 1.5 (two thirds of that)."""
-    return cus * LANES * CLK * (1.0 if form_pk64 else 2.0 / 3.0)
+def instruction_ceiling(cus, form):
+    """Relaxations/s the dominant kernel's form can issue at the published
+    clock: pk16 is half a VALU lane-op per relaxation (one 64-bit add and one
+    v_pk_minimum3_f16 per 4 relaxations: CUs x 64 lanes x 2.4 GHz x 2 =
+    7.86e13 at 256 CUs), pk64 one lane-op (3.93e13), min3 1.5 (2.62e13)."""
+    return cus * LANES * CLK * {2: 2.0, 1: 1.0}.get(form, 2.0 / 3.0)
+
+
+FORM_NAMES = {0: "min3", 1: "pk64", 2: "pk16"}
+FORM_BASIS = {0: "1.5 VALU lane-ops per relaxation (x 2/3)", 1: "one VALU lane-op per relaxation",
+              2: "half a VALU lane-op per relaxation (x 2)"}
 
 
 def parse():
@@ -364,27 +370,31 @@ def cpu_baseline(n, n_warm):
                         "value": 1024.0 ** 3 / dt1}}
 
 
-def traffic_from_profiles(n, tile, kappa):
+def traffic_from_profiles(n, tile, kappa, kernel):
     """HBM bytes per rest-kernel launch from the committed rocprofv3 FETCH_SIZE
     / WRITE_SIZE runs (calibrated against a copy kernel), if they were taken
-    on this configuration; else None."""
+    on this configuration and kernel; else None."""
     p = os.path.join(ROOT, "profiles", "rest_kernel_traffic.json")
     if not os.path.exists(p):
         return None
     with open(p) as f:
         t = json.load(f)
-    if (t.get("n"), t.get("tile"), t.get("kappa")) != (n, tile, kappa):
+    if (t.get("n"), t.get("tile"), t.get("kappa")) != (n, tile, kappa) or \
+            not t.get("kernel", "").startswith(kernel.split("<")[0] + "<"):
         return None
     return t.get("bytes_per_launch")
 
 
-def rest_kernel_name(tile, kappa):
+def rest_kernel_name(tile, kappa, form):
     """The rest-pass kernel the library launches for this configuration: block
     R = 128 if the stage width kappa*tile is a multiple of 128, else 64, else
-    32; R = 128 runs the 512-thread kernel."""
+    32; R = 128 runs the pk16 kernel in the pk16 form, else the 512-thread
+    kernel."""
     span = tile * kappa
     R = 128 if span % 128 == 0 else 64 if span % 64 == 0 else 32
-    return "k_rest512<128>" if R == 128 else f"k_rest<{R}>"
+    if R == 128:
+        return "k_rest16<128>" if form == 2 else "k_rest512<128>"
+    return f"k_rest<{R}>"
 
 
 def main():
@@ -414,16 +424,13 @@ def main():
         work.copy_(src)
         fx.run_device(a.algo, _lib.ptr(work), n, tile, kappa, stream)
 
-    # the relaxation form the solves run in (the same range check every
-    # solve starts with) fixes the VALU instruction ceiling
-    form = ctypes.c_int32()
-    _lib.check(L.fwx_relaxation_form(_lib.ptr(src), n, ctypes.byref(form), stream))
-    pk64 = form.value == _lib.FORM_PK64
-    cus = torch.cuda.get_device_properties(dev).multi_processor_count
-    ceiling = instruction_ceiling(cus, pk64)
-
     for _ in range(a.warmup):
         step()
+    # the relaxation form the solves ran in (fwx_last_solve_form) fixes the
+    # dominant kernel's VALU instruction ceiling
+    form, exact_finish = fx.last_solve_form() if a.warmup else (_lib.FORM_PK64, 0)
+    cus = torch.cuda.get_device_properties(dev).multi_processor_count
+    ceiling = instruction_ceiling(cus, form)
     # timed region: the steps alone, no profiler markers
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -527,14 +534,14 @@ def main():
                    "n": n, "tile": tile, "kappa": kappa, "algorithm": a.algo},
         "pct_valu_minplus_peak": 100.0 * value / PEAK_2OP,
         "frac_of_instruction_ceiling": value / ceiling,
-        "roofline": {"bound": "valu", "kernel": rest_kernel_name(tile, kappa), "achieved": achieved,
+        "roofline": {"bound": "valu", "kernel": rest_kernel_name(tile, kappa, form), "achieved": achieved,
                      "peak": ceiling, "unit": "relaxations/s", "frac": achieved / ceiling,
-                     "peak_basis": f"{'pk64' if pk64 else 'min3'} form: {cus} CUs x 64 lanes x 2.4 GHz"
-                                   f"{'' if pk64 else ' x 2/3'} (one VALU lane-op per relaxation"
-                                   f"{'' if pk64 else ' x 1.5'}); the metric's 2-op peak is "
+                     "form": FORM_NAMES.get(form, str(form)), "exact_finish": exact_finish,
+                     "peak_basis": f"{FORM_NAMES.get(form, form)} form: {cus} CUs x 64 lanes x 2.4 GHz, "
+                                   f"{FORM_BASIS.get(form, '')}; the metric's 2-op peak is "
                                    f"{PEAK_2OP:.4g}",
                      "x_2op_peak": achieved / PEAK_2OP,
-                     "traffic": traffic_from_profiles(n, tile, kappa),
+                     "traffic": traffic_from_profiles(n, tile, kappa, rest_kernel_name(tile, kappa, form)),
                      "measured_over": f"{a.steps} profiled steps after the timed region "
                                       f"({ms_per_step_prof:.3f} ms per step with the markers)",
                      "launches": int(rest_launches), "avg_launch_ms": rest_ms / max(1, rest_launches),

@@ -3,11 +3,13 @@
 Plain solves: for every size class it times fwx_multi_layer_tiling_gpu on a
 representative graph (density 0.9, weights U[1,1000]) over the candidate
 (tile, kappa) pairs and, where the stage width kappa*tile makes the rest pass
-128 wide, over the three shapes of that rest kernel -- the register
-micro-tile: 512 threads of 8x4 cells, 256 threads of 8x8 cells, or 256
-threads of 8x8 cells on 16-deep chunks (FWX_REST_VARIANT 2, 0, 1) -- and
-keeps the fastest.  (64- and 32-wide rest blocks have one shape each: 4x4
-and 2x2 cells on 256 threads.)
+128 wide (those passes run in the pk16 form on such graphs), over the chunk
+depth of the pk16 rest kernel (16 or 32, FWX_P16_KC), and keeps the
+fastest.  The shape of the pk64 128-wide rest kernel, which runs when the
+pk16 form does not apply (FWX_REST_VARIANT: 512 threads of 8x4 cells, 256 of
+8x8, 256 of 8x8 on 16-deep chunks), stays at the 512-thread shape that won
+every class when it was searched, before the pk16 form existed.  (64- and
+32-wide rest blocks have one shape each: 4x4 and 2x2 cells on 256 threads.)
 
 Batched plans: per size class of a graph inside a batch, (tile, kappa) and
 the threads of the 128-wide products (512: 8x4 cells, 256: 8x8), chosen by
@@ -17,7 +19,7 @@ the classes from the current table, two passes, each candidate table set
 through FWX_BATCH_TABLE.
 
 Writes (into --out-dir, default the source tree)
-  fwx_tune_table.inc    -- {n_max, tile, kappa, rest shape} compiled into libfwx
+  fwx_tune_table.inc    -- {n_max, tile, kappa, rest shape, pk16 chunk} compiled into libfwx
   fwx_batch_table.inc   -- {n_max, tile, kappa, product threads}
   autotune_mi355x.json  -- every timing behind both
 
@@ -44,7 +46,8 @@ CLASSES = [64, 128, 256, 512, 1024, 1536, 2048, 4096, 8192, 16384]
 MAX_N = 32769
 TILES = (32, 64, 128)
 KAPPAS = (1, 2, 3, 4, 6, 8, 12, 16)
-REST_SHAPES = (2, 0, 1)  # FWX_REST_VARIANT of a 128-wide rest pass
+REST_SHAPE = 2  # FWX_REST_VARIANT of a pk64 128-wide rest pass (512 threads, 8x4 cells)
+P16_CHUNKS = (16, 32)  # FWX_P16_KC of a pk16 rest pass
 
 # batched size classes (n_max) and their candidates
 BATCH_CLASSES = [64, 127, 255, 511, 1023, MAX_N]
@@ -67,8 +70,8 @@ def candidates(n):
                 continue  # (32, 1) runs the fused small-n rounds, up to n = 4096
             if t == 32 and k * t < 64 and k > 1 and n > 256:
                 continue  # 32-wide rest blocks only pay off on tiny graphs
-            for v in (REST_SHAPES if rest_block(t, k) == 128 and N > k else (2,)):
-                out.append((t, k, v))
+            for kc in (P16_CHUNKS if rest_block(t, k) == 128 and N > k else (0,)):
+                out.append((t, k, kc))
     return out
 
 
@@ -98,18 +101,21 @@ def tune_class(n):
     work = torch.empty_like(src)
     reps = 5 if n <= 2048 else 3
     res = {}
-    saved = os.environ.get("FWX_REST_VARIANT")
+    saved = {v: os.environ.get(v) for v in ("FWX_REST_VARIANT", "FWX_P16_KC")}
     try:
-        for t, k, v in candidates(n):
-            os.environ["FWX_REST_VARIANT"] = str(v)
-            key = f"{t}x{k}/r{v}"
+        os.environ["FWX_REST_VARIANT"] = str(REST_SHAPE)
+        for t, k, kc in candidates(n):
+            if kc:
+                os.environ["FWX_P16_KC"] = str(kc)
+            key = f"{t}x{k}/kc{kc}"
             res[key] = time_config(src, work, n, t, k, reps)
-            print(f"n={n} tile={t} kappa={k} rest={v}: {res[key] * 1e3:.3f} ms", flush=True)
+            print(f"n={n} tile={t} kappa={k} pk16 chunk={kc}: {res[key] * 1e3:.3f} ms", flush=True)
     finally:
-        if saved is None:
-            os.environ.pop("FWX_REST_VARIANT", None)
-        else:
-            os.environ["FWX_REST_VARIANT"] = saved
+        for v, val in saved.items():
+            if val is None:
+                os.environ.pop(v, None)
+            else:
+                os.environ[v] = val
     return res
 
 
@@ -132,11 +138,11 @@ def tune_plain(max_n):
             with open(tf.name) as f:
                 res = json.load(f)
         best = min(res, key=res.get)
-        tk, v = best.split("/r")
+        tk, kc = best.split("/kc")
         t, k = (int(x) for x in tk.split("x"))
-        table.append((n, t, k, int(v)))
+        table.append((n, t, k, REST_SHAPE, int(kc)))
         record[str(n)] = {"best": best, "seconds": res, "n3_per_s": n ** 3 / res[best]}
-        print(f"n<={n}: tile={t} kappa={k} rest={v} ({n ** 3 / res[best]:.3e} n^3/s)", flush=True)
+        print(f"n<={n}: tile={t} kappa={k} pk16 chunk={kc} ({n ** 3 / res[best]:.3e} n^3/s)", flush=True)
     if table and table[-1][0] < MAX_N:
         table.append((MAX_N,) + table[-1][1:])
     return table, record
@@ -252,14 +258,16 @@ def main():
     if "plain" in what:
         table, rec = tune_plain(a.max_n)
         # classes above --max-n keep their current entries and timings
-        kept = [e for e in read_table(os.path.join(src_dir, "fwx_tune_table.inc")) if e[0] > a.max_n]
+        kept = [tuple(e) + (16,) * (5 - len(e)) for e in read_table(os.path.join(src_dir, "fwx_tune_table.inc"))
+                if e[0] > a.max_n]
         if kept:
             table = [e for e in table if e[0] <= a.max_n] + kept
         record.setdefault("plain", {}).update(rec)
         with open(os.path.join(out_dir, "fwx_tune_table.inc"), "w") as f:
-            f.write("// Generated by fw_cuda_amd/autotune.py on the target GPU: {n_max, tile, kappa, rest}.\n")
+            f.write("// Generated by fw_cuda_amd/autotune.py on the target GPU: "
+                    "{n_max, tile, kappa, rest shape, pk16 chunk}.\n")
             for e in table:
-                f.write("{%d, %d, %d, %d},\n" % e)
+                f.write("{%d, %d, %d, %d, %d},\n" % tuple(e))
     if "batched" in what:
         start = read_table(os.path.join(src_dir, "fwx_batch_table.inc"))
         start = [next((e for e in start if e[0] >= c), start[-1]) for c in BATCH_CLASSES]

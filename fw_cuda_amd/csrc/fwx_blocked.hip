@@ -355,10 +355,13 @@ static int band_pass(int32_t* d, int32_t n, int32_t s0, int32_t S, int32_t en, i
   return FWX_OK;
 }
 
-// Chunk depth of the pk16 rest kernel: 32 (default) or FWX_P16_KC=16.
-static int p16_chunk() {
+// Chunk depth of the pk16 rest kernel: the tuned one for size n, or
+// FWX_P16_KC=16 / 32.  16-deep chunks hold it to 116 VGPRs and 24 KB of LDS
+// (four blocks per CU), 32-deep ones take 157 VGPRs and 52 KB (three).
+static int p16_chunk(int32_t n) {
   const char* e = std::getenv("FWX_P16_KC");
-  return (e && std::atoi(e) == 16) ? 16 : 32;
+  const int v = e ? std::atoi(e) : 0;
+  return (v == 16 || v == 32) ? v : tuned_p16_chunk(n);
 }
 
 template <int R, bool VEC>
@@ -370,7 +373,7 @@ static int rest_pass16(int32_t* d, int32_t n, int32_t s0, int32_t S, int32_t exc
   if (m <= 0) return FWX_OK;
   const int64_t total = (int64_t)m * m;
   const int per = (int)((total + 7) / 8);
-  if (p16_chunk() == 16)
+  if (p16_chunk(n) == 16)
     k_rest16<R, VEC, 16><<<8 * per, 256, 0, st>>>(d, n, s0, S, sr0, srn, m, per);
   else
     k_rest16<R, VEC, 32><<<8 * per, 256, 0, st>>>(d, n, s0, S, sr0, srn, m, per);
@@ -820,7 +823,7 @@ static int run_form(int32_t* d, int32_t n, int32_t tile, int32_t kappa, bool vec
   int dev = 0;
   FWX_HIP(hipGetDevice(&dev));
   const int knobs = (fused_dpanel() ? 1 : 0) | (rest_variant(n) << 1) | (band_narrow() ? 8 : 0) | (diag_threads() << 4) |
-                    (lead_priority_enabled() ? 1 << 20 : 0) | (p16_chunk() == 16 ? 1 << 21 : 0);
+                    (lead_priority_enabled() ? 1 << 20 : 0) | (p16_chunk(n) == 16 ? 1 << 21 : 0);
   const GraphKey key{d, n, tile, kappa, vec, form, small, small ? small_region_knob(n) : 0, lookahead_mode(), knobs, st,
                      dev};
   std::lock_guard<std::mutex> lk(g_graph_mu);

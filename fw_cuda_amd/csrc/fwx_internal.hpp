@@ -84,9 +84,11 @@ int side_stream(hipStream_t* out);
 int device_negative_cycle(const int32_t* d, int32_t n, hipStream_t st, bool* neg);
 
 // Tuned (tile, kappa) for a plain run of size n, and the shape of its
-// 128-wide rest kernel (0, 1 or 2, see fwx_blocked.hip).
+// 128-wide rest kernel (0, 1 or 2, see fwx_blocked.hip; pk64 and min3 runs).
 void tuned_config(int32_t n, int32_t* tile, int32_t* kappa);
 int tuned_rest_variant(int32_t n);
+// Chunk depth (16 or 32) of a pk16 128-wide rest pass of size n.
+int tuned_p16_chunk(int32_t n);
 // Tuned (tile, kappa) of a graph of size n inside a batch, and the threads of
 // its 128-wide products (512: 8x4 cells each; 256: 8x8).
 void batch_tuned_config(int32_t n, int32_t* tile, int32_t* kappa, int32_t* rest_threads);

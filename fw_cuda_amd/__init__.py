@@ -218,6 +218,13 @@ class BatchPlan:
         check(lib().fwx_batch_plan_stats(self.plan, ctypes.byref(st), ctypes.byref(la), ctypes.byref(it)))
         return {"steps": st.value, "launches": la.value, "items": it.value}
 
+    def last_form(self):
+        """(form, exact_graphs) of the last run: FORM_MIN3 / FORM_PK64 /
+        FORM_PK16, and the graphs its pk16 check finished in pk64."""
+        f, e = ctypes.c_int32(), ctypes.c_int32()
+        check(lib().fwx_batch_plan_last_form(self.plan, ctypes.byref(f), ctypes.byref(e)))
+        return f.value, e.value
+
     def destroy(self):
         if self.plan is not None and self.plan.value:
             check(lib().fwx_batch_plan_destroy(self.plan))

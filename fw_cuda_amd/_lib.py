@@ -77,6 +77,7 @@ SIGNATURES = {
     "fwx_batch_plan_create": (ctypes.c_int, [ctypes.POINTER(_vp), _vp, _vp, _i32, _vp, _vp]),
     "fwx_batch_plan_run": (ctypes.c_int, [_vp, _i32]),
     "fwx_batch_plan_stats": (ctypes.c_int, [_vp, _i32p, _i32p, ctypes.POINTER(_i64)]),
+    "fwx_batch_plan_last_form": (ctypes.c_int, [_vp, _i32p, _i32p]),
     "fwx_batch_plan_persistent_info": (ctypes.c_int, [_vp, _i32p, _i32p, ctypes.POINTER(_i64)]),
     "fwx_batch_plan_destroy": (ctypes.c_int, [_vp]),
     "fwx_batch_loop": (ctypes.c_int, [_vp, _vp, _i32, _vp, _vp]),

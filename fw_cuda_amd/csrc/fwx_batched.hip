@@ -37,6 +37,7 @@
 #include "fwx_device.hpp"
 #include "fwx_internal.hpp"
 #include "fwx_tile.hpp"
+#include "fwx_tile16.hpp"
 
 namespace fwx {
 
@@ -107,6 +108,52 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
     tile_update<128, 128, 32, true, 32, PK>(s, A, B, C, it.depth);
   else
     tile_update<128, 128, 32, false, 32, PK>(s, A, B, C, it.depth);
+}
+
+// 128 x 128 products in the pk16 form (fwx_tile16.hpp): 256 threads of 8 x 8
+// cells, 16-deep chunks; a cell whose result reaches the cap keeps its value
+// (the run's check afterwards finds it).
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_batch_prod16(
+    int32_t* __restrict__ data, const ProdItem* __restrict__ items) {
+  __shared__ TileSmem16<128, 128, 16> s;
+  const ProdItem it = items[blockIdx.x];
+  int32_t* g = data + it.base;
+  const View A = make_view(g, it.n, it.ar, it.ac);
+  const View B = make_view(g, it.n, it.br, it.bc);
+  const View C = make_view(g, it.n, it.cr, it.cc);
+  if (it.vec)
+    tile_update16<128, 128, 16, true>(s, A, B, C, it.depth);
+  else
+    tile_update16<128, 128, 16, false>(s, A, B, C, it.depth);
+}
+
+// After a pk16 run: flags[g] = 1 if graph g holds a cell >= cap.  Block
+// (x, g) scans a 1/gridDim.x share of graph g's cells (16-byte loads; every
+// graph starts 16-byte aligned).
+struct GraphSpan {
+  int64_t base;
+  int64_t cells;
+};
+
+__global__ void k_zero_flags(int32_t* __restrict__ flags, int32_t count) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < count) flags[i] = 0;
+}
+
+__global__ __launch_bounds__(256) void k_batch_at_least(const int32_t* __restrict__ data,
+                                                        const GraphSpan* __restrict__ spans, int32_t cap,
+                                                        int32_t* __restrict__ flags) {
+  const GraphSpan sp = spans[blockIdx.y];
+  const int32_t* g = data + sp.base;
+  const int64_t n4 = sp.cells / 4;
+  const int64_t tid = (int64_t)blockIdx.x * 256 + threadIdx.x, stride = (int64_t)gridDim.x * 256;
+  int32_t mx = INT32_MIN;
+  for (int64_t i = tid; i < n4; i += stride) {
+    const int4 v = reinterpret_cast<const int4*>(g)[i];
+    mx = max(mx, max(max(v.x, v.y), max(v.z, v.w)));
+  }
+  for (int64_t i = n4 * 4 + tid; i < sp.cells; i += stride) mx = max(mx, g[i]);
+  if (__ballot(mx >= cap) != 0ull && (threadIdx.x & 63) == 0) atomicOr(&flags[blockIdx.y], 1);
 }
 
 // ---------------------------------------------------------------------------
@@ -419,8 +466,17 @@ struct fwx_batch_plan {
   std::vector<int32_t> step_of;       // step index of each launch
   fwx::ProdItem* d_prod[fwx::kKinds] = {};
   fwx::DiagItem* d_diag[fwx::kKinds] = {};
-  hipGraph_t graph[2] = {};
-  hipGraphExec_t exec[2] = {};  // [0] min3 form, [1] pk64 form
+  hipGraph_t graph[3] = {};
+  hipGraphExec_t exec[3] = {};  // [0] min3 form, [1] pk64 form, [2] pk64 with pk16 128-wide products
+  // the pk16 check: the graphs with 128-wide products (the only ones a pk16
+  // run can leave at the cap), their cell spans, a flag each, and their
+  // sizes and offsets for the exact finish of the ones it flags
+  fwx::GraphSpan* d_spans = nullptr;
+  int32_t* d_flags = nullptr;
+  std::vector<int32_t> sizes;
+  std::vector<int64_t> offsets;
+  int32_t last_form = -1;     // form of the last run (FWX_FORM_*)
+  int32_t last_finished = 0;  // graphs the last pk16 run finished exactly
   int64_t items = 0;
   int32_t steps = 0;
   int64_t span = 0;  // elements from d_data covering every graph
@@ -442,7 +498,8 @@ namespace fwx {
 
 // Step by step on p->stream; a step's side launches run on p->side, forked
 // from p->stream at the step's start and joined back before the next step.
-template <bool PK>
+// P16: the 128-wide products run in the pk16 form.
+template <bool PK, bool P16>
 static int enqueue_plan(fwx_batch_plan* p) {
   int cur = -1;
   bool open = false;
@@ -476,8 +533,18 @@ static int enqueue_plan(fwx_batch_plan* p) {
       case kDiag128: k_batch_diag<128, 512><<<L.count, 512, 0, st>>>(p->data, p->d_diag[kDiag128] + L.first); break;
       case kProd32: k_batch_prod<32, PK><<<L.count, 256, 0, st>>>(p->data, p->d_prod[kProd32] + L.first); break;
       case kProd64: k_batch_prod<64, PK><<<L.count, 256, 0, st>>>(p->data, p->d_prod[kProd64] + L.first); break;
-      case kProd128: k_batch_prod512<PK><<<L.count, 512, 0, st>>>(p->data, p->d_prod[kProd128] + L.first); break;
-      case kProd128w: k_batch_prod<128, PK><<<L.count, 256, 0, st>>>(p->data, p->d_prod[kProd128w] + L.first); break;
+      case kProd128:
+        if (P16)
+          k_batch_prod16<<<L.count, 256, 0, st>>>(p->data, p->d_prod[kProd128] + L.first);
+        else
+          k_batch_prod512<PK><<<L.count, 512, 0, st>>>(p->data, p->d_prod[kProd128] + L.first);
+        break;
+      case kProd128w:
+        if (P16)
+          k_batch_prod16<<<L.count, 256, 0, st>>>(p->data, p->d_prod[kProd128w] + L.first);
+        else
+          k_batch_prod<128, PK><<<L.count, 256, 0, st>>>(p->data, p->d_prod[kProd128w] + L.first);
+        break;
       default: return fail(FWX_ERR_INVALID_ARG, "corrupt batch plan");
     }
     FWX_LAUNCHED("k_batch");
@@ -489,7 +556,45 @@ static int enqueue_plan(fwx_batch_plan* p) {
   return FWX_OK;
 }
 
-static int enqueue_mode(fwx_batch_plan* p, bool pk) { return pk ? enqueue_plan<true>(p) : enqueue_plan<false>(p); }
+// form: 0 min3, 1 pk64, 2 pk64 with pk16 128-wide products
+static int enqueue_mode(fwx_batch_plan* p, int form) {
+  return form == 2 ? enqueue_plan<true, true>(p) : form ? enqueue_plan<true, false>(p) : enqueue_plan<false, false>(p);
+}
+
+// The check after a pk16 run: graphs holding a cell at or above the cap are
+// solved once more, each in the exact pk64 form on its own matrix (walk
+// lengths of its input graph, each at most its input value, so the closure
+// is the input's).  Synchronises the plan's stream.
+static int finish_saturated(fwx_batch_plan* p) {
+  const int32_t count = (int32_t)p->sizes.size();
+  p->last_finished = 0;
+  if (count == 0) {
+    FWX_HIP(hipStreamSynchronize(p->stream));
+    return FWX_OK;
+  }
+  k_zero_flags<<<(count + 255) / 256, 256, 0, p->stream>>>(p->d_flags, count);
+  FWX_LAUNCHED("k_zero_flags");
+  k_batch_at_least<<<dim3(16, (unsigned)count), 256, 0, p->stream>>>(p->data, p->d_spans, (int32_t)kP16Cap,
+                                                                     p->d_flags);
+  FWX_LAUNCHED("k_batch_at_least");
+  std::vector<int32_t> flags(count, 0);
+  FWX_HIP(hipMemcpyAsync(flags.data(), p->d_flags, count * sizeof(int32_t), hipMemcpyDeviceToHost, p->stream));
+  FWX_HIP(hipStreamSynchronize(p->stream));
+  p->last_finished = 0;
+  for (int32_t g = 0; g < count; ++g) {
+    if (!flags[g]) continue;
+    ++p->last_finished;
+    if (int rc = enqueue_blocked_exact(p->data + p->offsets[g], p->sizes[g], p->stream)) return rc;
+  }
+  if (p->last_finished) FWX_HIP(hipStreamSynchronize(p->stream));
+  return FWX_OK;
+}
+
+// pk16 128-wide batch products; FWX_PK16=0 keeps pk64 (read at every run).
+static bool batch_pk16_enabled() {
+  const char* e = std::getenv("FWX_PK16");
+  return !(e && e[0] == '0');
+}
 
 template <bool PK>
 static int enqueue_persistent(fwx_batch_plan* p) {
@@ -626,10 +731,12 @@ static void destroy_plan(fwx_batch_plan* p) {
   if (p->fork) (void)hipEventDestroy(p->fork);
   if (p->join) (void)hipEventDestroy(p->join);
   if (p->side) (void)hipStreamDestroy(p->side);
-  for (int m = 0; m < 2; ++m) {
+  for (int m = 0; m < 3; ++m) {
     if (p->exec[m]) (void)hipGraphExecDestroy(p->exec[m]);
     if (p->graph[m]) (void)hipGraphDestroy(p->graph[m]);
   }
+  if (p->d_spans) (void)hipFree(p->d_spans);
+  if (p->d_flags) (void)hipFree(p->d_flags);
   for (int k = 0; k < kKinds; ++k) {
     if (p->d_prod[k]) (void)hipFree(p->d_prod[k]);
     if (p->d_diag[k]) (void)hipFree(p->d_diag[k]);
@@ -762,6 +869,29 @@ int fwx_batch_plan_create(fwx_batch_plan** out, const int32_t* sizes, const int6
     destroy_plan(p);
     return rc;
   }
+  {
+    std::vector<GraphSpan> spans;
+    for (int32_t q = 0; q < count; ++q) {
+      bool wide = false;
+      for (const auto* steps : {&P[q], &SP[q]})
+        for (const auto& st : *steps)
+          for (const auto& kv : st) wide = wide || kv.first == kProd128 || kv.first == kProd128w;
+      if (!wide) continue;
+      const int32_t g = order[q];
+      p->sizes.push_back(sizes[g]);
+      p->offsets.push_back(offsets[g]);
+      spans.push_back(GraphSpan{offsets[g], (int64_t)sizes[g] * sizes[g]});
+    }
+    int rc = upload(&p->d_spans, spans.data(), spans.size());
+    if (!rc) {
+      const hipError_t e = hipMalloc(reinterpret_cast<void**>(&p->d_flags), std::max<size_t>(1, spans.size()) * sizeof(int32_t));
+      if (e != hipSuccess) rc = hip_fail(e, "batch plan flags", __FILE__, __LINE__);
+    }
+    if (rc) {
+      destroy_plan(p);
+      return rc;
+    }
+  }
   *out = p;
   return FWX_OK;
 }
@@ -773,14 +903,18 @@ int fwx_batch_plan_run(fwx_batch_plan* p, int32_t mode) {
   // one range check over the buffer span selects the relaxation form
   bool pk = false;
   if (int rc = values_in_pk64_range(p->data, p->span, p->stream, &pk)) return rc;
-  const int m = pk ? 1 : 0;
   const bool persistent = mode == FWX_BATCH_PERSISTENT || mode == FWX_BATCH_PERSISTENT_GRAPH;
+  // the step launches take the pk16 form for their 128-wide products where
+  // pk64 is exact; the persistent kernel keeps pk64
+  const int m = !pk ? 0 : (!persistent && batch_pk16_enabled()) ? 2 : 1;
+  p->last_form = m;
+  p->last_finished = 0;
   const bool use_graph = mode == FWX_BATCH_GRAPH || mode == FWX_BATCH_PERSISTENT_GRAPH;
   if (persistent && p->cgrid <= 0)
     return fail(FWX_ERR_UNSUPPORTED, "persistent batch mode needs a cooperative launch, which this device lacks");
   hipGraph_t* graph = persistent ? &p->pgraph[m] : &p->graph[m];
   hipGraphExec_t* exec = persistent ? &p->pexec[m] : &p->exec[m];
-  auto enqueue = [&](fwx_batch_plan* pl) { return persistent ? enqueue_persistent_mode(pl, pk) : enqueue_mode(pl, pk); };
+  auto enqueue = [&](fwx_batch_plan* pl) { return persistent ? enqueue_persistent_mode(pl, pk) : enqueue_mode(pl, m); };
   if (use_graph) {
     if (!*exec) {
       // a private capture stream: the caller's stream may be the legacy
@@ -825,7 +959,16 @@ int fwx_batch_plan_run(fwx_batch_plan* p, int32_t mode) {
       return fail(FWX_ERR_TIMEOUT, "a grid-barrier wait of the persistent kernel gave up; the result is incomplete");
     return FWX_OK;
   }
+  if (m == 2) return finish_saturated(p);
   FWX_HIP(hipStreamSynchronize(p->stream));
+  return FWX_OK;
+}
+
+int fwx_batch_plan_last_form(const fwx_batch_plan* p, int32_t* form, int32_t* exact_graphs) {
+  clear_error();
+  if (!p || !form || !exact_graphs) return fail(FWX_ERR_INVALID_ARG, "null argument");
+  *form = p->last_form;
+  *exact_graphs = p->last_finished;
   return FWX_OK;
 }
 

@@ -875,6 +875,14 @@ static int run_form(int32_t* d, int32_t n, int32_t tile, int32_t kappa, bool vec
   return replay(g_graphs.back());
 }
 
+int enqueue_blocked_exact(int32_t* d, int32_t n, hipStream_t st) {
+  int32_t tile = 0, kappa = 0;
+  tuned_config(n, &tile, &kappa);
+  const bool vec = (n % 4 == 0) && ((reinterpret_cast<uintptr_t>(d) & 15u) == 0);
+  if (small_path(n, tile, kappa)) return enqueue_small(d, n, st);
+  return enqueue_direct(d, n, tile, kappa, vec, 1, st);
+}
+
 static thread_local int t_last_form = -1;
 static thread_local int t_last_exact_finish = 0;
 

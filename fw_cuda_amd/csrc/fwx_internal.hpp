@@ -35,6 +35,9 @@ inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s);
 // Plain-FW drivers (enqueue only; the C entry points add the final sync).
 int enqueue_baseline(int32_t* d, int32_t n, hipStream_t st);
 int enqueue_blocked(int32_t* d, int32_t n, int32_t tile, int32_t kappa, hipStream_t st);
+// A plain solve of a matrix whose cells are all in [0, 2^30), with the tuned
+// configuration and no pk16 passes (the exact finish of a batched graph).
+int enqueue_blocked_exact(int32_t* d, int32_t n, hipStream_t st);
 // The form of this thread's last enqueue_blocked (fwx_last_solve_form).
 void last_solve_form(int32_t* form, int32_t* exact_finish);
 int enqueue_refshape(int32_t* d, int32_t n, hipStream_t st);

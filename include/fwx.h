@@ -216,9 +216,9 @@ int fwx_certify_rows(const int32_t* d_W, int32_t n, const int32_t* d_src, int32_
  * Many independent graphs solved together.  Graph g is an n_g x n_g row-major
  * int32 matrix at d_data + offsets[g] (element offsets; `sizes` and
  * `offsets` are host arrays of `count` entries; d_data 16-byte aligned).
- * The plan builds every graph's multi-layer schedule (graphs with n >= 128
- * on tile 128, kappa 3; smaller ones on the tuned table's entry for their
- * size), moves each diagonal closure, with the one product item it needs,
+ * The plan builds every graph's multi-layer schedule ((tile, kappa) per
+ * graph size class from the batched tuned table), moves each diagonal
+ * closure, with the one product item it needs,
  * onto a side branch of the step before it, merges step j of all graphs into
  * one launch per work kind, largest graphs first, and uploads the work lists.
  * Run modes:
@@ -235,6 +235,11 @@ int fwx_certify_rows(const int32_t* d_W, int32_t n, const int32_t* d_src, int32_
  *                          and replayed.
  * The persistent modes return FWX_ERR_UNSUPPORTED on a device without
  * cooperative launches, or if the runtime cannot capture one into a graph.
+ * In the step-launch modes, when every cell of the batch lies in [0, 2^30)
+ * the 128-wide products run in the pk16 form (fwx_last_solve_form); after
+ * such a run one pass flags every graph that still holds a cell at or above
+ * FWX_PK16_CAP, and each flagged graph is finished by an exact pk64 solve of
+ * its own matrix.  The persistent modes run the pk64 form.
  * Results equal a plain fwx_multi_layer_tiling_gpu call per graph in every
  * mode.  The plan owns device memory (its work lists), a side stream and two
  * events until fwx_batch_plan_destroy; side branches fork from the plan's
@@ -255,6 +260,10 @@ int fwx_batch_plan_run(fwx_batch_plan* plan, int32_t mode);
 int fwx_batch_plan_persistent_info(const fwx_batch_plan* plan, int32_t* grid, int32_t* steps, int64_t* tasks);
 /* Dependent steps, kernel launches per run, and work items of a plan. */
 int fwx_batch_plan_stats(const fwx_batch_plan* plan, int32_t* steps, int32_t* launches, int64_t* items);
+/* The form of the plan's last run (FWX_FORM_MIN3 / PK64 / PK16; -1 before
+ * any run) and how many graphs its pk16 check sent to an exact pk64 finish.
+ * No reference counterpart. */
+int fwx_batch_plan_last_form(const fwx_batch_plan* plan, int32_t* form, int32_t* exact_graphs);
 int fwx_batch_plan_destroy(fwx_batch_plan* plan);
 /* The comparator: one plain fwx_multi_layer_tiling_gpu call (with its final
  * stream synchronisation) per graph, in order. */

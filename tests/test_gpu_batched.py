@@ -254,3 +254,46 @@ def test_batch_512_mixed_sizes():
         for g in range(len(graphs)):
             assert (got2[g] == got[g]).all(), (mode, g)
     plan.destroy()
+
+
+def _graph(n, p, lo, hi, seed):
+    rng = np.random.default_rng(seed)
+    W = np.where(rng.random((n, n)) < p, rng.integers(lo, hi + 1, (n, n)), INF).astype(np.int32)
+    np.fill_diagonal(W, 0)
+    return W
+
+
+@pytest.mark.parametrize("pk16", ["1", "0"])
+def test_batch_pk16_products_and_exact_finish(monkeypatch, pk16):
+    """pk16 128-wide products in a batch whose graphs do and do not reach the
+    16-bit cap (dense small weights; unreachable pairs; weights around the
+    cap; a graph too small for 128-wide products): every graph equals the
+    oracle in both step-launch modes, the run reports the pk16 form, and the
+    graphs with 128-wide products that hold cells at or above the cap are the
+    ones finished exactly.  FWX_PK16=0 runs pk64 with the same results."""
+    import torch
+    from fw_cuda_amd import _lib
+    cap = _lib.PK16_CAP
+    monkeypatch.setenv("FWX_PK16", pk16)
+    monkeypatch.setenv("FWX_BATCH_TABLE", "64:32:2:512,127:128:1:512,32769:128:3:512")
+    graphs = [_graph(300, 0.9, 1, 1000, 1), _graph(300, 0.003, 1, 1000, 2), _graph(520, 0.5, 1, 1000, 3),
+              _graph(400, 0.5, cap - 3, cap + 3, 4), _graph(100, 0.01, 1, 1000, 5), _graph(777, 0.02, 1, 1000, 6),
+              _graph(260, 0.05, 2000, 9000, 7)]
+    expect = [po.fw_rowpar(g) for g in graphs]
+    wide = [g.shape[0] > 128 for g in graphs]
+    want_finished = sum(1 for w, e in zip(wide, expect) if w and (e >= cap).any())
+    assert want_finished >= 2  # the batch exercises the exact finish
+    sizes, offs, t, flat0 = upload(graphs)
+    plan = fx.BatchPlan(sizes, offs, _lib.ptr(t))
+    for mode in ("graph", "graph", "direct"):
+        t.copy_(torch.from_numpy(flat0).to("cuda:0"))
+        say("pk16 batch", pk16, mode)
+        plan.run(mode)
+        form, finished = plan.last_form()
+        if pk16 == "1":
+            assert (form, finished) == (_lib.FORM_PK16, want_finished), mode
+        else:
+            assert (form, finished) == (_lib.FORM_PK64, 0), mode
+        got, flat = unpack(t, sizes, offs)
+        check_all(got, expect, flat, flat0, sizes, offs)
+    plan.destroy()

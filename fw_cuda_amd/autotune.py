@@ -19,7 +19,7 @@ the classes from the current table, two passes, each candidate table set
 through FWX_BATCH_TABLE.
 
 Writes (into --out-dir, default the source tree)
-  fwx_tune_table.inc    -- {n_max, tile, kappa, rest shape, pk16 chunk} compiled into libfwx
+  fwx_tune_table.inc    -- {n_max, tile, kappa, rest shape, pk16 chunk, tiling tile} compiled into libfwx
   fwx_batch_table.inc   -- {n_max, tile, kappa, product threads}
   autotune_mi355x.json  -- every timing behind both
 
@@ -140,7 +140,10 @@ def tune_plain(max_n):
         best = min(res, key=res.get)
         tk, kc = best.split("/kc")
         t, k = (int(x) for x in tk.split("x"))
-        table.append((n, t, k, REST_SHAPE, int(kc)))
+        # tiling_gpu (kappa 1) with tile 0: the fastest kappa-1 candidate's tile
+        best1 = min((key for key in res if key.split("/")[0].endswith("x1")), key=res.get)
+        t1 = int(best1.split("x")[0])
+        table.append((n, t, k, REST_SHAPE, int(kc), t1))
         record[str(n)] = {"best": best, "seconds": res, "n3_per_s": n ** 3 / res[best]}
         print(f"n<={n}: tile={t} kappa={k} pk16 chunk={kc} ({n ** 3 / res[best]:.3e} n^3/s)", flush=True)
     if table and table[-1][0] < MAX_N:
@@ -258,16 +261,16 @@ def main():
     if "plain" in what:
         table, rec = tune_plain(a.max_n)
         # classes above --max-n keep their current entries and timings
-        kept = [tuple(e) + (16,) * (5 - len(e)) for e in read_table(os.path.join(src_dir, "fwx_tune_table.inc"))
-                if e[0] > a.max_n]
+        kept = [tuple(e[:5]) + (16,) * (5 - len(e)) + ((e[5] if len(e) > 5 else e[1]),)
+                for e in read_table(os.path.join(src_dir, "fwx_tune_table.inc")) if e[0] > a.max_n]
         if kept:
             table = [e for e in table if e[0] <= a.max_n] + kept
         record.setdefault("plain", {}).update(rec)
         with open(os.path.join(out_dir, "fwx_tune_table.inc"), "w") as f:
             f.write("// Generated by fw_cuda_amd/autotune.py on the target GPU: "
-                    "{n_max, tile, kappa, rest shape, pk16 chunk}.\n")
+                    "{n_max, tile, kappa, rest shape, pk16 chunk, tiling tile}.\n")
             for e in table:
-                f.write("{%d, %d, %d, %d, %d},\n" % tuple(e))
+                f.write("{%d, %d, %d, %d, %d, %d},\n" % tuple(e))
     if "batched" in what:
         start = read_table(os.path.join(src_dir, "fwx_batch_table.inc"))
         start = [next((e for e in start if e[0] >= c), start[-1]) for c in BATCH_CLASSES]

@@ -39,10 +39,12 @@ static int check_matrix(const void* p, int32_t n) {
   return FWX_OK;
 }
 
+// tile 0 / kappa <= 0 select the tuned values; kappa 1 passed explicitly
+// (tiling_gpu) takes the tile tuned for kappa 1.
 static int resolve(int32_t n, int32_t& tile, int32_t& kappa) {
   int32_t t0, k0;
   tuned_config(n, &t0, &k0);
-  if (tile == 0) tile = t0;
+  if (tile == 0) tile = kappa == 1 ? tuned_tiling_tile(n) : t0;
   if (tile != 32 && tile != 64 && tile != 128)
     return fail(FWX_ERR_INVALID_ARG, "tile must be 0 (tuned), 32, 64 or 128");
   if (kappa <= 0) kappa = k0;

@@ -63,6 +63,7 @@
 #include "fwx_device.hpp"
 #include "fwx_internal.hpp"
 #include "fwx_tile.hpp"
+#include "fwx_tile16.hpp"
 
 namespace fwx {
 
@@ -190,6 +191,111 @@ __device__ __forceinline__ void prod_lds(const int32_t* A, const int2* B, Cells<
   }
 }
 
+// ---------------------------------------------------------------------------
+// The pk16 form (fwx_tile16.hpp) for blocks whose loaded operands are all
+// below kSmall16: every sum of two values the block derives is then below
+// 2 kSmall16 <= the pk16 cap (each derived value is a min over loaded values
+// and such sums, and never exceeds the value it replaces), so nothing
+// saturates and no check is needed afterwards.  Such a block holds no cell
+// outside the matrix (those load as INF), so its staging needs no mask.
+// A operand: (v + BIAS) in both 16-bit lanes, the int32 A layout.  B operand:
+// per k pair, column pairs (j, j + 16) with j = (cp % 16) + 32 (cp / 16), one
+// 8-byte slot each (low word row 2p, high word row 2p + 1) -- the pairs a
+// thread's cells form (columns tx + 16 q and tx + 16 (q + 1), q even).
+constexpr uint32_t kSmall16 = 0x1000u;
+
+template <int NT, int BM>
+__device__ __forceinline__ void to_a16(uint32_t* A, const Cells<NT, BM, kT>& t, int tx, int ty) {
+  constexpr int RS = NT / 16;
+#pragma unroll
+  for (int r = 0; r < BM / RS; ++r)
+#pragma unroll
+    for (int q = 0; q < 2; ++q) A[(ty + RS * r) * kLda + tx + 16 * q] = ((uint32_t)t[r][q] + kP16Bias) * 0x10001u;
+}
+
+// B slot (p, cp) low or high word: 16 lanes of consecutive tx write
+// consecutive slots, the next 16 (row + 1) the other words of the same slots.
+template <int NT, int BN>
+__device__ __forceinline__ void to_b16(uint64_t* B, const Cells<NT, kT, BN>& t, int tx, int ty) {
+  constexpr int RS = NT / 16;
+  uint32_t* b = reinterpret_cast<uint32_t*>(B);
+#pragma unroll
+  for (int r = 0; r < kT / RS; ++r)
+#pragma unroll
+    for (int qp = 0; qp < BN / 32; ++qp) {
+      const int row = ty + RS * r;
+      b[((row >> 1) * (BN / 2) + tx + 16 * qp) * 2 + (row & 1)] =
+          (uint32_t)t[r][2 * qp] | ((uint32_t)t[r][2 * qp + 1] << 16);
+    }
+}
+
+template <int NT, int BM, int BN>
+using Pairs = uint32_t[BM / (NT / 16)][BN / 32];
+
+template <int NT, int BM, int BN>
+__device__ __forceinline__ void enc16(const Cells<NT, BM, BN>& t, Pairs<NT, BM, BN>& c) {
+#pragma unroll
+  for (int r = 0; r < BM / (NT / 16); ++r)
+#pragma unroll
+    for (int qp = 0; qp < BN / 32; ++qp)
+      c[r][qp] = ((uint32_t)t[r][2 * qp] | ((uint32_t)t[r][2 * qp + 1] << 16)) + kP16Bias2;
+}
+
+template <int NT, int BM, int BN>
+__device__ __forceinline__ void dec16(const Pairs<NT, BM, BN>& c, Cells<NT, BM, BN>& t) {
+#pragma unroll
+  for (int r = 0; r < BM / (NT / 16); ++r)
+#pragma unroll
+    for (int qp = 0; qp < BN / 32; ++qp) {
+      const uint32_t v = c[r][qp] - kP16Bias2;
+      t[r][2 * qp] = (int32_t)(v & 0xFFFFu);
+      t[r][2 * qp + 1] = (int32_t)(v >> 16);
+    }
+}
+
+// acc <- min(acc, A (x) B) over the round's 32 k in the pk16 form; the next
+// group of 4 k is read while the current one is computed.
+template <int NT, int BM, int BN>
+__device__ __forceinline__ void prod_lds16(const uint32_t* A, const uint64_t* B, Pairs<NT, BM, BN>& acc, int tx,
+                                           int ty) {
+  constexpr int RS = NT / 16, TR = BM / RS, TP = BN / 32;
+  uint4 a[2][TR];
+  uint64_t b[2][2][TP];
+  auto load = [&](int buf, int g) {
+#pragma unroll
+    for (int r = 0; r < TR; ++r) a[buf][r] = *reinterpret_cast<const uint4*>(&A[(ty + RS * r) * kLda + 4 * g]);
+#pragma unroll
+    for (int qp = 0; qp < TP; ++qp) {
+      b[buf][0][qp] = B[(2 * g) * (BN / 2) + tx + 16 * qp];
+      b[buf][1][qp] = B[(2 * g + 1) * (BN / 2) + tx + 16 * qp];
+    }
+  };
+  load(0, 0);
+#pragma unroll 2
+  for (int g = 0; g < kT / 4; ++g) {
+    const int cur = g & 1;
+    if (g + 1 < kT / 4) load(cur ^ 1, g + 1);
+#pragma unroll
+    for (int r = 0; r < TR; ++r) {
+      const uint64_t a01 = (uint64_t)a[cur][r].x | ((uint64_t)a[cur][r].y << 32);
+      const uint64_t a23 = (uint64_t)a[cur][r].z | ((uint64_t)a[cur][r].w << 32);
+#pragma unroll
+      for (int qp = 0; qp < TP; ++qp) {
+        acc[r][qp] = relax4_pk16(acc[r][qp], a01, b[cur][0][qp]);
+        acc[r][qp] = relax4_pk16(acc[r][qp], a23, b[cur][1][qp]);
+      }
+    }
+  }
+}
+
+template <int NT, int BM, int BN>
+__device__ __forceinline__ void prod16(const uint32_t* A, const uint64_t* B, Cells<NT, BM, BN>& t, int tx, int ty) {
+  Pairs<NT, BM, BN> acc;
+  enc16<NT, BM, BN>(t, acc);
+  prod_lds16<NT, BM, BN>(A, B, acc, tx, ty);
+  dec16<NT, BM, BN>(acc, t);
+}
+
 // Per round: the two panel products run at once, one per half of the block
 // (NT/2 threads each, wave-uniform), then the region product on all NT.
 template <int R, int NT, bool PK>
@@ -233,8 +339,53 @@ __device__ __forceinline__ void lookahead_products(SmallSmem<R>& s, Cells<NT / 2
   prod_lds<NT, kT, kT, PK>(s.pA, s.pB, y, tx, ty);
 }
 
+// small_products in the pk16 form (operands already staged as pk16).
 template <int R, int NT>
-__device__ __forceinline__ void lookahead_block(SmallSmem<R>& s, int32_t* D, int32_t n, int32_t c, bool closed) {
+__device__ __forceinline__ void small_products16(SmallSmem<R>& s, Cells<NT / 2, R, kT>& pa, Cells<NT / 2, kT, R>& pb,
+                                                 Cells<NT, R, R>& xc, bool upper) {
+  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4, th = (threadIdx.x & (NT / 2 - 1)) >> 4;
+  uint32_t* dA = reinterpret_cast<uint32_t*>(s.dA);
+  uint64_t* dB = reinterpret_cast<uint64_t*>(s.dB);
+  uint32_t* pA = reinterpret_cast<uint32_t*>(s.pA);
+  uint64_t* pB = reinterpret_cast<uint64_t*>(s.pB);
+  if (!upper)
+    prod16<NT / 2, R, kT>(pA, dB, pa, tx, th);  // P_A = min(X(I,c), X(I,c) (x) D*)
+  else
+    prod16<NT / 2, kT, R>(dA, pB, pb, tx, th);  // P_B = min(X(c,J), D* (x) X(c,J))
+  __syncthreads();
+  if (!upper)
+    to_a16<NT / 2, R>(pA, pa, tx, th);
+  else
+    to_b16<NT / 2, R>(pB, pb, tx, th);
+  __syncthreads();
+  prod16<NT, R, R>(pA, pB, xc, tx, ty);
+}
+
+// lookahead_products in the pk16 form.
+template <int R, int NT>
+__device__ __forceinline__ void lookahead_products16(SmallSmem<R>& s, Cells<NT / 2, kT, kT>& p,
+                                                     Cells<NT, kT, kT>& y, bool upper) {
+  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4, th = (threadIdx.x & (NT / 2 - 1)) >> 4;
+  uint32_t* dA = reinterpret_cast<uint32_t*>(s.dA);
+  uint64_t* dB = reinterpret_cast<uint64_t*>(s.dB);
+  uint32_t* pA = reinterpret_cast<uint32_t*>(s.pA);
+  uint64_t* pB = reinterpret_cast<uint64_t*>(s.pB);
+  if (!upper)
+    prod16<NT / 2, kT, kT>(pA, dB, p, tx, th);  // P(c+1,c)
+  else
+    prod16<NT / 2, kT, kT>(dA, pB, p, tx, th);  // P(c,c+1)
+  __syncthreads();
+  if (!upper)
+    to_a16<NT / 2, kT>(pA, p, tx, th);
+  else
+    to_b16<NT / 2, kT>(pB, p, tx, th);
+  __syncthreads();
+  prod16<NT, kT, kT>(pA, pB, y, tx, ty);
+}
+
+template <int R, int NT>
+__device__ __forceinline__ void lookahead_block(SmallSmem<R>& s, int32_t* D, int32_t n, int32_t c, bool closed,
+                                                bool small16_enabled) {
   const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4, th = (threadIdx.x & (NT / 2 - 1)) >> 4;
   const bool upper = threadIdx.x >= NT / 2;
   const int32_t c0 = c * kT, e0 = c0 + kT;
@@ -249,6 +400,20 @@ __device__ __forceinline__ void lookahead_block(SmallSmem<R>& s, int32_t* D, int
   if (!closed) close_regs<kT, NT>(d, s.lines, min(kT, n - c0));
   const uint32_t o = or_cells<NT, kT, kT>(d) | or_cells<NT / 2, kT, kT>(p) | or_cells<NT, kT, kT>(y);
   const bool pk = !__syncthreads_or((o & kPk64Mask) != 0u);
+  const bool p16 = small16_enabled && !__syncthreads_or(o >= kSmall16);
+  if (p16) {
+    to_a16<NT, kT>(reinterpret_cast<uint32_t*>(s.dA), d, tx, ty);
+    to_b16<NT, kT>(reinterpret_cast<uint64_t*>(s.dB), d, tx, ty);
+    if (!upper)
+      to_a16<NT / 2, kT>(reinterpret_cast<uint32_t*>(s.pA), p, tx, th);
+    else
+      to_b16<NT / 2, kT>(reinterpret_cast<uint64_t*>(s.pB), p, tx, th);
+    __syncthreads();
+    lookahead_products16<R, NT>(s, p, y, upper);
+    close_regs<kT, NT>(y, s.lines, min(kT, n - e0));
+    store_cells<NT, kT, kT>(D, n, e0, e0, tx, ty, y);
+    return;
+  }
   to_a<NT, kT>(s.dA, d, tx, ty, n - c0, n - c0);
   to_b<NT, kT>(s.dB, d, tx, ty, n - c0, n - c0);
   if (!upper)
@@ -274,12 +439,12 @@ __device__ __forceinline__ void lookahead_block(SmallSmem<R>& s, int32_t* D, int
 // and overlap the closure; the form is decided after it.
 template <int R, int NT>
 __global__ __launch_bounds__(NT) void k_small_round(int32_t* __restrict__ D, int32_t n, int32_t c, int32_t M,
-                                                    int32_t N) {
+                                                    int32_t N, int32_t small16_enabled) {
   __shared__ SmallSmem<R> s;
   const bool closed = c > 0;
   const bool ahead = c + 1 < N;
   if (ahead && blockIdx.x == 0) {
-    lookahead_block<R, NT>(s, D, n, c, closed);
+    lookahead_block<R, NT>(s, D, n, c, closed, small16_enabled != 0);
     return;
   }
   const int b = blockIdx.x - (ahead ? 1 : 0);
@@ -301,6 +466,19 @@ __global__ __launch_bounds__(NT) void k_small_round(int32_t* __restrict__ D, int
   const uint32_t o = or_cells<NT, kT, kT>(d) | (upper ? or_cells<NT / 2, kT, R>(pb) : or_cells<NT / 2, R, kT>(pa)) |
                      or_cells<NT, R, R>(xc);
   const bool pk = !__syncthreads_or((o & kPk64Mask) != 0u);
+  const bool p16 = small16_enabled && !__syncthreads_or(o >= kSmall16);
+  if (p16) {
+    to_a16<NT, kT>(reinterpret_cast<uint32_t*>(s.dA), d, tx, ty);
+    to_b16<NT, kT>(reinterpret_cast<uint64_t*>(s.dB), d, tx, ty);
+    if (!upper)
+      to_a16<NT / 2, R>(reinterpret_cast<uint32_t*>(s.pA), pa, tx, th);
+    else
+      to_b16<NT / 2, R>(reinterpret_cast<uint64_t*>(s.pB), pb, tx, th);
+    __syncthreads();
+    small_products16<R, NT>(s, pa, pb, xc, upper);
+    store_cells<NT, R, R>(D, n, r0, col0, tx, ty, xc, ahead ? c0 + kT : -kT);
+    return;
+  }
   to_a<NT, kT>(s.dA, d, tx, ty, n - c0, n - c0);
   to_b<NT, kT>(s.dB, d, tx, ty, n - c0, n - c0);
   if (!upper)
@@ -331,27 +509,33 @@ int small_threads() {
   return v == 256 ? 256 : 512;
 }
 
+// FWX_SMALL16=0 keeps every block on pk64 / min3 (read at every call).
+int small16() {
+  const char* e = std::getenv("FWX_SMALL16");
+  return (e && e[0] == '0') ? 0 : 1;
+}
+
 }  // namespace
 
 int enqueue_small_rounds(int32_t* d, int32_t n, hipStream_t st) {
   const int N = (n + kT - 1) / kT;
-  const int R = small_region(n), NT = small_threads();
+  const int R = small_region(n), NT = small_threads(), p16 = small16();
   const int M = (n + R - 1) / R;
   for (int c = 0; c < N; ++c) {
     const int blocks = M * M + (c + 1 < N ? 1 : 0);
     if (R == 64 && NT == 256)
-      k_small_round<64, 256><<<blocks, 256, 0, st>>>(d, n, c, M, N);
+      k_small_round<64, 256><<<blocks, 256, 0, st>>>(d, n, c, M, N, p16);
     else if (R == 64)
-      k_small_round<64, 512><<<blocks, 512, 0, st>>>(d, n, c, M, N);
+      k_small_round<64, 512><<<blocks, 512, 0, st>>>(d, n, c, M, N, p16);
     else if (NT == 256)
-      k_small_round<128, 256><<<blocks, 256, 0, st>>>(d, n, c, M, N);
+      k_small_round<128, 256><<<blocks, 256, 0, st>>>(d, n, c, M, N, p16);
     else
-      k_small_round<128, 512><<<blocks, 512, 0, st>>>(d, n, c, M, N);
+      k_small_round<128, 512><<<blocks, 512, 0, st>>>(d, n, c, M, N, p16);
     FWX_LAUNCHED("k_small_round");
   }
   return FWX_OK;
 }
 
-int small_region_knob(int32_t n) { return small_region(n) | (small_threads() << 8); }
+int small_region_knob(int32_t n) { return small_region(n) | (small_threads() << 8) | (small16() << 16); }
 
 }  // namespace fwx

@@ -476,3 +476,27 @@ def test_small_rounds_profiled_and_graph_eviction():
     finally:
         for dm in mats:
             dm.free()
+
+
+@pytest.mark.parametrize("small16", ["1", "0"])
+def test_small_rounds_pk16_blocks(monkeypatch, small16):
+    """The small-n rounds' pk16 blocks (every loaded operand below 4096, so no
+    sum reaches the 16-bit cap): dense small weights (almost every block
+    pk16 after round 0), weights straddling the threshold (blocks of both
+    forms in one round), and distances growing past it during the solve, at
+    ragged and aligned sizes, graph-replayed and direct; FWX_SMALL16=0 keeps
+    pk64/min3 with the same results."""
+    monkeypatch.setenv("FWX_SMALL16", small16)
+    rng = np.random.default_rng(99)
+    for n in (33, 64, 200, 512, 1024, 1537):
+        for tag, lo, hi, p in (("small", 1, 1000, 0.9), ("straddle", 1000, 5000, 0.5),
+                               ("growing", 500, 2000, 0.02)):
+            W = fx.generate_graph(n, p, int(rng.integers(1, 2**31)), lo, hi)
+            exp = po.fw_rowpar(W)
+            with fx.DeviceMatrix(n) as dm:
+                for graph in ("1", "0", "1"):
+                    monkeypatch.setenv("FWX_GRAPH", graph)
+                    say("small pk16", small16, n, tag, graph)
+                    dm.upload(W)
+                    fx.run_device("tiling_gpu", dm.ptr, n, 32, 1)
+                    assert (dm.download() == exp).all(), (small16, n, tag, graph)

@@ -20,9 +20,9 @@
 // more block per launch, the look-ahead block, computes round c's result on
 // tile (c+1, c+1) -- from the two panel tiles P(c+1,c) and P(c,c+1), which it
 // computes itself -- closes it and stores D* of round c+1 there; the region
-// block holding that tile does not store it.  From round 1 on, tile (c,c)
-// therefore holds D* when the launch starts and region blocks only read it;
-// in round 0 every block closes tile (0,0) itself.  For a 32-tile (i,j) of the region
+// block holding that tile does not store it.  Tile (c,c) therefore holds D*
+// when round c's launch starts (tile (0,0) is closed by a one-block launch
+// before round 0), and region blocks only read it.  For a 32-tile (i,j) of the region
 // this is the three-phase round: a cross tile gets min(X, P(i,c) (x) P(c,j));
 // a panel tile gets min(P(c,j), D* (x) P(c,j)) = P(c,j) (and likewise
 // P(i,c)), since D* (x) D* >= D* (triangle inequality inside the closed
@@ -79,7 +79,14 @@ struct SmallSmem {
   alignas(16) int32_t pA[R * kLda];   // A panel: X(I,c), then P_A
   alignas(16) int2 pB[(kT / 2) * R];  // B panel: X(c,J), then P_B
   int32_t lines[8 * kT];              // closure
+  uint32_t wflags[16];                // form vote, one word per wave
 };
+
+// Block-wide vote on the operands a block loaded, with one barrier: bit 0 =
+// some thread holds a value outside [0, 2^30) (no pk64), bit 1 = some thread
+// holds one >= kSmall16 (no pk16).
+template <int NT>
+__device__ __forceinline__ uint32_t form_vote(uint32_t* wflags, uint32_t o);
 
 // Thread (tx, ty) = (tid % 16, tid / 16) of NT threads holds cells
 // (ty + (NT/16) r, tx + 16 q) of a BM x BN window -- for the 32 x 32 tile
@@ -296,6 +303,17 @@ __device__ __forceinline__ void prod16(const uint32_t* A, const uint64_t* B, Cel
   dec16<NT, BM, BN>(acc, t);
 }
 
+template <int NT>
+__device__ __forceinline__ uint32_t form_vote(uint32_t* wflags, uint32_t o) {
+  const uint32_t f = (__ballot((o & kPk64Mask) != 0u) ? 1u : 0u) | (__ballot(o >= kSmall16) ? 2u : 0u);
+  if ((threadIdx.x & 63) == 0) wflags[threadIdx.x >> 6] = f;
+  __syncthreads();
+  uint32_t all = 0;
+#pragma unroll
+  for (int w = 0; w < NT / 64; ++w) all |= wflags[w];
+  return all;
+}
+
 // Per round: the two panel products run at once, one per half of the block
 // (NT/2 threads each, wave-uniform), then the region product on all NT.
 template <int R, int NT, bool PK>
@@ -384,7 +402,7 @@ __device__ __forceinline__ void lookahead_products16(SmallSmem<R>& s, Cells<NT /
 }
 
 template <int R, int NT>
-__device__ __forceinline__ void lookahead_block(SmallSmem<R>& s, int32_t* D, int32_t n, int32_t c, bool closed,
+__device__ __forceinline__ void lookahead_block(SmallSmem<R>& s, int32_t* D, int32_t n, int32_t c,
                                                 bool small16_enabled) {
   const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4, th = (threadIdx.x & (NT / 2 - 1)) >> 4;
   const bool upper = threadIdx.x >= NT / 2;
@@ -397,10 +415,10 @@ __device__ __forceinline__ void lookahead_block(SmallSmem<R>& s, int32_t* D, int
   else
     load_cells<NT / 2, kT, kT>(D, n, c0, e0, tx, th, p);
   load_cells<NT, kT, kT>(D, n, e0, e0, tx, ty, y);
-  if (!closed) close_regs<kT, NT>(d, s.lines, min(kT, n - c0));
   const uint32_t o = or_cells<NT, kT, kT>(d) | or_cells<NT / 2, kT, kT>(p) | or_cells<NT, kT, kT>(y);
-  const bool pk = !__syncthreads_or((o & kPk64Mask) != 0u);
-  const bool p16 = small16_enabled && !__syncthreads_or(o >= kSmall16);
+  const uint32_t vote = form_vote<NT>(s.wflags, o);
+  const bool pk = !(vote & 1u);
+  const bool p16 = small16_enabled && !(vote & 2u);
   if (p16) {
     to_a16<NT, kT>(reinterpret_cast<uint32_t*>(s.dA), d, tx, ty);
     to_b16<NT, kT>(reinterpret_cast<uint64_t*>(s.dB), d, tx, ty);
@@ -430,21 +448,19 @@ __device__ __forceinline__ void lookahead_block(SmallSmem<R>& s, int32_t* D, int
 }
 
 // Round c: block 0 is the look-ahead block (when c + 1 < N), block b > 0 the
-// region b - 1 of M x M.  From round 1 on, tile (c, c) holds D* already (the
-// previous round's look-ahead block stored it), so the region blocks skip
-// the closure; round 0 closes it in every block.  Thread (tx, ty) holds the
-// region cells and, for a closure, cells of tile (c,c) in the NT-thread
-// layout; the lower half of the block (th = its row within the half) holds
-// the A panel, the upper half the B panel.  Operand loads are issued first
-// and overlap the closure; the form is decided after it.
+// region b - 1 of M x M.  Tile (c, c) holds D* already (the previous
+// round's look-ahead block stored it; k_small_close0 for round 0).  Thread
+// (tx, ty) holds the region cells and the cells of tile (c,c) in the
+// NT-thread layout; the lower half of the block (th = its row within the
+// half) holds the A panel, the upper half the B panel.  The operands are
+// loaded at once, then one block-wide vote picks the form.
 template <int R, int NT>
 __global__ __launch_bounds__(NT) void k_small_round(int32_t* __restrict__ D, int32_t n, int32_t c, int32_t M,
                                                     int32_t N, int32_t small16_enabled) {
   __shared__ SmallSmem<R> s;
-  const bool closed = c > 0;
   const bool ahead = c + 1 < N;
   if (ahead && blockIdx.x == 0) {
-    lookahead_block<R, NT>(s, D, n, c, closed, small16_enabled != 0);
+    lookahead_block<R, NT>(s, D, n, c, small16_enabled != 0);
     return;
   }
   const int b = blockIdx.x - (ahead ? 1 : 0);
@@ -462,11 +478,11 @@ __global__ __launch_bounds__(NT) void k_small_round(int32_t* __restrict__ D, int
   else
     load_cells<NT / 2, kT, R>(D, n, c0, col0, tx, th, pb);
   load_cells<NT, R, R>(D, n, r0, col0, tx, ty, xc);
-  if (!closed) close_regs<kT, NT>(d, s.lines, min(kT, n - c0));
   const uint32_t o = or_cells<NT, kT, kT>(d) | (upper ? or_cells<NT / 2, kT, R>(pb) : or_cells<NT / 2, R, kT>(pa)) |
                      or_cells<NT, R, R>(xc);
-  const bool pk = !__syncthreads_or((o & kPk64Mask) != 0u);
-  const bool p16 = small16_enabled && !__syncthreads_or(o >= kSmall16);
+  const uint32_t vote = form_vote<NT>(s.wflags, o);
+  const bool pk = !(vote & 1u);
+  const bool p16 = small16_enabled && !(vote & 2u);
   if (p16) {
     to_a16<NT, kT>(reinterpret_cast<uint32_t*>(s.dA), d, tx, ty);
     to_b16<NT, kT>(reinterpret_cast<uint64_t*>(s.dB), d, tx, ty);
@@ -492,6 +508,12 @@ __global__ __launch_bounds__(NT) void k_small_round(int32_t* __restrict__ D, int
     small_products<R, NT, false>(s, pa, pb, xc, upper, n, c0, r0, col0);
   // tile (c+1, c+1) is the look-ahead block's to store
   store_cells<NT, R, R>(D, n, r0, col0, tx, ty, xc, ahead ? c0 + kT : -kT);
+}
+
+// Tile (0, 0) closed once before round 0, so that every round's region
+// blocks only read a closed diagonal tile.
+__global__ __launch_bounds__(256) void k_small_close0(int32_t* __restrict__ D, int32_t n) {
+  close_tile<kT, 256>(make_view(D, n, 0, 0));
 }
 
 // Region width R (FWX_SMALL_R = 64 or 128; default 64: measured faster at
@@ -521,6 +543,8 @@ int enqueue_small_rounds(int32_t* d, int32_t n, hipStream_t st) {
   const int N = (n + kT - 1) / kT;
   const int R = small_region(n), NT = small_threads(), p16 = small16();
   const int M = (n + R - 1) / R;
+  k_small_close0<<<1, 256, 0, st>>>(d, n);
+  FWX_LAUNCHED("k_small_close0");
   for (int c = 0; c < N; ++c) {
     const int blocks = M * M + (c + 1 < N ? 1 : 0);
     if (R == 64 && NT == 256)

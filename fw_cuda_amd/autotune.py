@@ -19,7 +19,7 @@ the classes from the current table, two passes, each candidate table set
 through FWX_BATCH_TABLE.
 
 Writes (into --out-dir, default the source tree)
-  fwx_tune_table.inc    -- {n_max, tile, kappa, rest shape, pk16 chunk, tiling tile} compiled into libfwx
+  fwx_tune_table.inc    -- {n_max, tile, kappa, rest shape, pk16 chunk, tiling tile, schedule} compiled into libfwx
   fwx_batch_table.inc   -- {n_max, tile, kappa, product threads}
   autotune_mi355x.json  -- every timing behind both
 
@@ -48,6 +48,7 @@ TILES = (32, 64, 128)
 KAPPAS = (1, 2, 3, 4, 6, 8, 12, 16)
 REST_SHAPE = 2  # FWX_REST_VARIANT of a pk64 128-wide rest pass (512 threads, 8x4 cells)
 P16_CHUNKS = (16, 32)  # FWX_P16_KC of a pk16 rest pass
+SCHEDULES = (2, 3)  # FWX_LOOKAHEAD: side-band, side-band a stage ahead
 
 # batched size classes (n_max) and their candidates
 BATCH_CLASSES = [64, 127, 255, 511, 1023, MAX_N]
@@ -71,7 +72,8 @@ def candidates(n):
             if t == 32 and k * t < 64 and k > 1 and n > 256:
                 continue  # 32-wide rest blocks only pay off on tiny graphs
             for kc in (P16_CHUNKS if rest_block(t, k) == 128 and N > k else (0,)):
-                out.append((t, k, kc))
+                for la in (SCHEDULES if N > k and not (t == 32 and k == 1) else (2,)):
+                    out.append((t, k, kc, la))
     return out
 
 
@@ -101,15 +103,16 @@ def tune_class(n):
     work = torch.empty_like(src)
     reps = 5 if n <= 2048 else 3
     res = {}
-    saved = {v: os.environ.get(v) for v in ("FWX_REST_VARIANT", "FWX_P16_KC")}
+    saved = {v: os.environ.get(v) for v in ("FWX_REST_VARIANT", "FWX_P16_KC", "FWX_LOOKAHEAD")}
     try:
         os.environ["FWX_REST_VARIANT"] = str(REST_SHAPE)
-        for t, k, kc in candidates(n):
+        for t, k, kc, la in candidates(n):
             if kc:
                 os.environ["FWX_P16_KC"] = str(kc)
-            key = f"{t}x{k}/kc{kc}"
+            os.environ["FWX_LOOKAHEAD"] = str(la)
+            key = f"{t}x{k}/kc{kc}/la{la}"
             res[key] = time_config(src, work, n, t, k, reps)
-            print(f"n={n} tile={t} kappa={k} pk16 chunk={kc}: {res[key] * 1e3:.3f} ms", flush=True)
+            print(f"n={n} tile={t} kappa={k} pk16 chunk={kc} schedule={la}: {res[key] * 1e3:.3f} ms", flush=True)
     finally:
         for v, val in saved.items():
             if val is None:
@@ -138,14 +141,16 @@ def tune_plain(max_n):
             with open(tf.name) as f:
                 res = json.load(f)
         best = min(res, key=res.get)
-        tk, kc = best.split("/kc")
+        tk, kc, la = best.split("/")
         t, k = (int(x) for x in tk.split("x"))
+        kc, la = int(kc[2:]), int(la[2:])
         # tiling_gpu (kappa 1) with tile 0: the fastest kappa-1 candidate's tile
         best1 = min((key for key in res if key.split("/")[0].endswith("x1")), key=res.get)
         t1 = int(best1.split("x")[0])
-        table.append((n, t, k, REST_SHAPE, int(kc), t1))
+        table.append((n, t, k, REST_SHAPE, kc, t1, la))
         record[str(n)] = {"best": best, "seconds": res, "n3_per_s": n ** 3 / res[best]}
-        print(f"n<={n}: tile={t} kappa={k} pk16 chunk={kc} ({n ** 3 / res[best]:.3e} n^3/s)", flush=True)
+        print(f"n<={n}: tile={t} kappa={k} pk16 chunk={kc} schedule={la} ({n ** 3 / res[best]:.3e} n^3/s)",
+              flush=True)
     if table and table[-1][0] < MAX_N:
         table.append((MAX_N,) + table[-1][1:])
     return table, record
@@ -261,16 +266,16 @@ def main():
     if "plain" in what:
         table, rec = tune_plain(a.max_n)
         # classes above --max-n keep their current entries and timings
-        kept = [tuple(e[:5]) + (16,) * (5 - len(e)) + ((e[5] if len(e) > 5 else e[1]),)
+        kept = [tuple(e[:5]) + (16,) * (5 - len(e)) + ((e[5] if len(e) > 5 else e[1]), (e[6] if len(e) > 6 else 2))
                 for e in read_table(os.path.join(src_dir, "fwx_tune_table.inc")) if e[0] > a.max_n]
         if kept:
             table = [e for e in table if e[0] <= a.max_n] + kept
         record.setdefault("plain", {}).update(rec)
         with open(os.path.join(out_dir, "fwx_tune_table.inc"), "w") as f:
             f.write("// Generated by fw_cuda_amd/autotune.py on the target GPU: "
-                    "{n_max, tile, kappa, rest shape, pk16 chunk, tiling tile}.\n")
+                    "{n_max, tile, kappa, rest shape, pk16 chunk, tiling tile, schedule}.\n")
             for e in table:
-                f.write("{%d, %d, %d, %d, %d, %d},\n" % tuple(e))
+                f.write("{%d, %d, %d, %d, %d, %d, %d},\n" % tuple(e))
     if "batched" in what:
         start = read_table(os.path.join(src_dir, "fwx_batch_table.inc"))
         start = [next((e for e in start if e[0] >= c), start[-1]) for c in BATCH_CLASSES]

@@ -250,6 +250,32 @@ __global__ __launch_bounds__(256) void k_rest_band(int32_t* __restrict__ D, int3
   tile_update<R, R, kChunk, VEC, 16, PK>(s, A, B, C, S);
 }
 
+// rest_band in the pk16 form (64-wide tiles: 256 threads of 4 x 4 cells,
+// 16-deep chunks); the band is part of stage s's rest pass, so its cells
+// follow the same rule as the pk16 rest passes (a cell at the cap keeps its
+// value; the solve's check afterwards finds it).
+template <int R, bool VEC>
+__global__ __launch_bounds__(256) void k_rest_band16(int32_t* __restrict__ D, int32_t n, int32_t s0, int32_t S,
+                                                     int32_t e0, int32_t en, int32_t f0, int32_t fn, int32_t m1) {
+  __shared__ TileSmem16<R, R, 16> s;
+  int b = blockIdx.x;
+  int I, J;
+  if (b < fn * m1) {
+    I = f0 + b / m1;
+    const int Jr = b % m1;
+    J = Jr + (Jr >= e0 ? en : 0);
+  } else {
+    b -= fn * m1;
+    const int Ir = b / fn;
+    I = Ir + (Ir >= e0 ? en + fn : 0);
+    J = f0 + b % fn;
+  }
+  const View A = make_view(D, n, I * R, s0);
+  const View B = make_view(D, n, s0, J * R);
+  const View C = make_view(D, n, I * R, J * R);
+  tile_update16<R, R, 16, VEC>(s, A, B, C, S);
+}
+
 // ---------------------------------------------------------------------------
 // host driver
 
@@ -339,7 +365,13 @@ static int rest_pass(int32_t* d, int32_t n, int32_t s0, int32_t S, int32_t excl,
   return FWX_OK;
 }
 
-template <int R, bool VEC, bool PK>
+// FWX_BAND16=0 keeps the band pass in pk64 when the rest passes run pk16.
+static bool band16_enabled() {
+  const char* e = std::getenv("FWX_BAND16");
+  return !(e && e[0] == '0');
+}
+
+template <int R, bool VEC, bool PK, bool P16 = false>
 static int band_pass(int32_t* d, int32_t n, int32_t s0, int32_t S, int32_t en, int32_t fn_tiles,
                      hipStream_t st) {
   const int nr = (n + R - 1) / R;
@@ -350,7 +382,10 @@ static int band_pass(int32_t* d, int32_t n, int32_t s0, int32_t S, int32_t en, i
   if (fn <= 0 || m1 <= 0) return FWX_OK;
   const int cnt = fn * m1 + (m1 - fn) * fn;
   if (cnt <= 0) return FWX_OK;
-  k_rest_band<R, VEC, PK><<<cnt, 256, 0, st>>>(d, n, s0, S, e0, en, f0, fn, m1);
+  if (P16 && R == 64 && S % 16 == 0 && band16_enabled())
+    k_rest_band16<64, VEC><<<cnt, 256, 0, st>>>(d, n, s0, S, e0, en, f0, fn, m1);
+  else
+    k_rest_band<R, VEC, PK><<<cnt, 256, 0, st>>>(d, n, s0, S, e0, en, f0, fn, m1);
   FWX_LAUNCHED("k_rest_band");
   return FWX_OK;
 }
@@ -398,11 +433,11 @@ static bool band_narrow() {
   return !(e && e[0] == '1');
 }
 
-template <bool VEC, bool PK>
+template <bool VEC, bool PK, bool P16 = false>
 static int band_dispatch(int R, int32_t* d, int32_t n, int32_t s0, int32_t S, int32_t en, int32_t fn,
                          hipStream_t st) {
   if (R == 128) {
-    if (band_narrow()) return band_pass<64, VEC, PK>(d, n, s0, S, 2 * en, 2 * fn, st);
+    if (band_narrow()) return band_pass<64, VEC, PK, P16>(d, n, s0, S, 2 * en, 2 * fn, st);
     return band_pass<128, VEC, PK>(d, n, s0, S, en, fn, st);
   }
   if (R == 64) return band_pass<64, VEC, PK>(d, n, s0, S, en, fn, st);
@@ -494,7 +529,7 @@ static int run_blocked_lookahead(int32_t* d, int32_t n, int32_t kappa, hipStream
       const int B2 = min(kappa, N - kb2);
       const int fn = (B2 * T + R - 1) / R;
       const double L2 = (double)min(B2 * T, n - kb2 * T);
-      if (int rc = band_dispatch<VEC, PK>(R, d, n, s0, S, en, fn, st)) return rc;
+      if (int rc = band_dispatch<VEC, PK, P16>(R, d, n, s0, S, en, fn, st)) return rc;
       if (int rc = prof_chain(st, &ev, FWX_PROF_BAND, ((nn - L) * (nn - L) - (nn - L - L2) * (nn - L - L2)) * L))
         return rc;
       if (!ev) FWX_HIP(hipEventRecord(evA, st));
@@ -551,7 +586,7 @@ static int run_blocked_sideband(int32_t* d, int32_t n, int32_t kappa, hipStream_
       const double L2 = (double)min(B2 * T, n - kb2 * T);
       hipEvent_t es;
       if (int rc = prof_open(side, &es)) return rc;
-      if (int rc = band_dispatch<VEC, PK>(R, d, n, s0, S, en, fn, side)) return rc;
+      if (int rc = band_dispatch<VEC, PK, P16>(R, d, n, s0, S, en, fn, side)) return rc;
       if (int rc = prof_chain(side, &es, FWX_PROF_BAND, ((nn - L) * (nn - L) - (nn - L - L2) * (nn - L - L2)) * L))
         return rc;
       if (int rc = lead_rounds<T, VEC, PK>(d, n, kb2, B2, N, prio, side)) return rc;
@@ -616,7 +651,7 @@ static int run_blocked_ahead(int32_t* d, int32_t n, int32_t kappa, hipStream_t s
       const double L2 = (double)min(B2 * T, n - kb2 * T);
       hipEvent_t es;
       if (int rc = prof_open(side, &es)) return rc;
-      if (int rc = band_dispatch<VEC, PK>(R, d, n, s0, S, en, fn, side)) return rc;
+      if (int rc = band_dispatch<VEC, PK, P16>(R, d, n, s0, S, en, fn, side)) return rc;
       if (int rc = prof_chain(side, &es, FWX_PROF_BAND, ((nn - L) * (nn - L) - (nn - L - L2) * (nn - L - L2)) * L))
         return rc;
       if (int rc = lead_rounds<T, VEC, PK>(d, n, kb2, B2, N, prio, side)) return rc;
@@ -628,7 +663,7 @@ static int run_blocked_ahead(int32_t* d, int32_t n, int32_t kappa, hipStream_t s
         const int gn = (B3 * T + R - 1) / R;
         const double L3 = (double)min(B3 * T, n - kb3 * T);
         const double outside2 = nn - L - L2;  // rows outside bands s and s+1
-        if (int rc = band_dispatch<VEC, PK>(R, d, n, s0, S, en + fn, gn, st)) return rc;
+        if (int rc = band_dispatch<VEC, PK, P16>(R, d, n, s0, S, en + fn, gn, st)) return rc;
         if (int rc = prof_chain(st, &ev, FWX_PROF_BAND, (outside2 * outside2 - (outside2 - L3) * (outside2 - L3)) * L))
           return rc;
         FWX_HIP(hipEventRecord(evX, st));
@@ -671,18 +706,19 @@ int side_stream(hipStream_t* out) {
   return FWX_OK;
 }
 
-// FWX_LOOKAHEAD selects the schedule (read on every call): 0 = single
-// stream, 1 = look-ahead with the band on the caller's stream, 2 =
-// side-band (the default), 3 = side-band with the side stream a stage ahead.
-static int lookahead_mode() {
+// The schedule: the tuned one of the size class (2 or 3), or FWX_LOOKAHEAD
+// (read on every call): 0 = single stream, 1 = look-ahead with the band on
+// the caller's stream, 2 = side-band, 3 = side-band with the side stream a
+// stage ahead.
+static int lookahead_mode(int32_t n) {
   const char* e = std::getenv("FWX_LOOKAHEAD");
-  return (e && e[0] >= '0' && e[0] <= '3') ? e[0] - '0' : 2;
+  return (e && e[0] >= '0' && e[0] <= '3') ? e[0] - '0' : tuned_lookahead(n);
 }
 
 template <int T, bool VEC, bool PK, bool P16>
 static int run_any(int32_t* d, int32_t n, int32_t kappa, hipStream_t st) {
   const int N = (n + T - 1) / T;
-  const int mode = lookahead_mode();
+  const int mode = lookahead_mode(n);
   if (mode == 0 || N <= kappa) return run_blocked<T, VEC, PK, P16>(d, n, kappa, st);
   hipStream_t side;
   if (int rc = side_stream(&side)) return rc;
@@ -824,7 +860,7 @@ static int run_form(int32_t* d, int32_t n, int32_t tile, int32_t kappa, bool vec
   FWX_HIP(hipGetDevice(&dev));
   const int knobs = (fused_dpanel() ? 1 : 0) | (rest_variant(n) << 1) | (band_narrow() ? 8 : 0) | (diag_threads() << 4) |
                     (lead_priority_enabled() ? 1 << 20 : 0) | (p16_chunk(n) == 16 ? 1 << 21 : 0);
-  const GraphKey key{d, n, tile, kappa, vec, form, small, small ? small_region_knob(n) : 0, lookahead_mode(), knobs, st,
+  const GraphKey key{d, n, tile, kappa, vec, form, small, small ? small_region_knob(n) : 0, lookahead_mode(n), knobs, st,
                      dev};
   std::lock_guard<std::mutex> lk(g_graph_mu);
   auto replay = [&](GraphEntry& e) -> int {

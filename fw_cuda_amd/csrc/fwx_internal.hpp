@@ -94,6 +94,8 @@ int tuned_rest_variant(int32_t n);
 int tuned_p16_chunk(int32_t n);
 // Tuned tile of tiling_gpu (kappa 1) for size n.
 int tuned_tiling_tile(int32_t n);
+// Tuned two-stream schedule (FWX_LOOKAHEAD values 1..3) for size n.
+int tuned_lookahead(int32_t n);
 // Tuned (tile, kappa) of a graph of size n inside a batch, and the threads of
 // its 128-wide products (512: 8x4 cells each; 256: 8x8).
 void batch_tuned_config(int32_t n, int32_t* tile, int32_t* kappa, int32_t* rest_threads);

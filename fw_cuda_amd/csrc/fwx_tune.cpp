@@ -16,18 +16,19 @@ namespace fwx {
 
 namespace {
 struct TuneEntry {
-  int32_t n_max, tile, kappa, rest, p16kc, tile1;
+  int32_t n_max, tile, kappa, rest, p16kc, tile1, la;
 };
 // {n_max, tile, kappa, rest, p16kc}: rest = the shape of a pk64 128-wide
 // rest pass (2: 512 threads of 8x4 cells; 0: 256 threads of 8x8 cells; 1:
 // 256 threads of 8x8 cells on 16-deep chunks); p16kc = the chunk depth of a
 // pk16 128-wide rest pass (16 or 32; 0 when the rest pass is narrower);
-// tile1 = the tile of tiling_gpu (kappa 1) when the caller passes tile 0.
+// tile1 = the tile of tiling_gpu (kappa 1) when the caller passes tile 0;
+// la = the two-stream schedule (2 side-band, 3 side-band a stage ahead).
 const TuneEntry kTable[] = {
 #include "fwx_tune_table.inc"
 };
 // {n_max, tile, kappa, rest}: rest = threads of the 128-wide products of a
-// batched graph (512: 8x4 cells each; 256: 8x8); p16kc and tile1 unused.
+// batched graph (512: 8x4 cells each; 256: 8x8); the other fields unused.
 const TuneEntry kBatchTable[] = {
 #include "fwx_batch_table.inc"
 };
@@ -49,6 +50,11 @@ void tuned_config(int32_t n, int32_t* tile, int32_t* kappa) {
 int tuned_rest_variant(int32_t n) { return lookup(kTable, n).rest; }
 
 int tuned_p16_chunk(int32_t n) { return lookup(kTable, n).p16kc == 32 ? 32 : 16; }
+
+int tuned_lookahead(int32_t n) {
+  const int32_t v = lookup(kTable, n).la;
+  return (v == 1 || v == 3) ? v : 2;
+}
 
 int tuned_tiling_tile(int32_t n) {
   const int32_t t = lookup(kTable, n).tile1;

@@ -110,21 +110,22 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
     tile_update<128, 128, 32, false, 32, PK>(s, A, B, C, it.depth);
 }
 
-// 128 x 128 products in the pk16 form (fwx_tile16.hpp): 256 threads of 8 x 8
-// cells, 16-deep chunks; a cell whose result reaches the cap keeps its value
-// (the run's check afterwards finds it).
+// R x R products in the pk16 form (fwx_tile16.hpp): 256 threads of 8 x 8
+// (R = 128) or 4 x 4 (R = 64) cells, 16-deep chunks; a cell whose result
+// reaches the cap keeps its value (the run's check afterwards finds it).
+template <int R>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_batch_prod16(
     int32_t* __restrict__ data, const ProdItem* __restrict__ items) {
-  __shared__ TileSmem16<128, 128, 16> s;
+  __shared__ TileSmem16<R, R, 16> s;
   const ProdItem it = items[blockIdx.x];
   int32_t* g = data + it.base;
   const View A = make_view(g, it.n, it.ar, it.ac);
   const View B = make_view(g, it.n, it.br, it.bc);
   const View C = make_view(g, it.n, it.cr, it.cc);
   if (it.vec)
-    tile_update16<128, 128, 16, true>(s, A, B, C, it.depth);
+    tile_update16<R, R, 16, true>(s, A, B, C, it.depth);
   else
-    tile_update16<128, 128, 16, false>(s, A, B, C, it.depth);
+    tile_update16<R, R, 16, false>(s, A, B, C, it.depth);
 }
 
 // After a pk16 run: flags[g] = 1 if graph g holds a cell >= cap.  Block
@@ -467,10 +468,10 @@ struct fwx_batch_plan {
   fwx::ProdItem* d_prod[fwx::kKinds] = {};
   fwx::DiagItem* d_diag[fwx::kKinds] = {};
   hipGraph_t graph[3] = {};
-  hipGraphExec_t exec[3] = {};  // [0] min3 form, [1] pk64 form, [2] pk64 with pk16 128-wide products
-  // the pk16 check: the graphs with 128-wide products (the only ones a pk16
-  // run can leave at the cap), their cell spans, a flag each, and their
-  // sizes and offsets for the exact finish of the ones it flags
+  hipGraphExec_t exec[3] = {};  // [0] min3 form, [1] pk64 form, [2] pk64 with pk16 64- and 128-wide products
+  // the pk16 check: the graphs with 64- or 128-wide products (the only ones
+  // a pk16 run can leave at the cap), their cell spans, a flag each, and
+  // their sizes and offsets for the exact finish of the ones it flags
   fwx::GraphSpan* d_spans = nullptr;
   int32_t* d_flags = nullptr;
   std::vector<int32_t> sizes;
@@ -498,7 +499,7 @@ namespace fwx {
 
 // Step by step on p->stream; a step's side launches run on p->side, forked
 // from p->stream at the step's start and joined back before the next step.
-// P16: the 128-wide products run in the pk16 form.
+// P16: the 64- and 128-wide products run in the pk16 form.
 template <bool PK, bool P16>
 static int enqueue_plan(fwx_batch_plan* p) {
   int cur = -1;
@@ -532,16 +533,21 @@ static int enqueue_plan(fwx_batch_plan* p) {
         break;
       case kDiag128: k_batch_diag<128, 512><<<L.count, 512, 0, st>>>(p->data, p->d_diag[kDiag128] + L.first); break;
       case kProd32: k_batch_prod<32, PK><<<L.count, 256, 0, st>>>(p->data, p->d_prod[kProd32] + L.first); break;
-      case kProd64: k_batch_prod<64, PK><<<L.count, 256, 0, st>>>(p->data, p->d_prod[kProd64] + L.first); break;
+      case kProd64:
+        if (P16)
+          k_batch_prod16<64><<<L.count, 256, 0, st>>>(p->data, p->d_prod[kProd64] + L.first);
+        else
+          k_batch_prod<64, PK><<<L.count, 256, 0, st>>>(p->data, p->d_prod[kProd64] + L.first);
+        break;
       case kProd128:
         if (P16)
-          k_batch_prod16<<<L.count, 256, 0, st>>>(p->data, p->d_prod[kProd128] + L.first);
+          k_batch_prod16<128><<<L.count, 256, 0, st>>>(p->data, p->d_prod[kProd128] + L.first);
         else
           k_batch_prod512<PK><<<L.count, 512, 0, st>>>(p->data, p->d_prod[kProd128] + L.first);
         break;
       case kProd128w:
         if (P16)
-          k_batch_prod16<<<L.count, 256, 0, st>>>(p->data, p->d_prod[kProd128w] + L.first);
+          k_batch_prod16<128><<<L.count, 256, 0, st>>>(p->data, p->d_prod[kProd128w] + L.first);
         else
           k_batch_prod<128, PK><<<L.count, 256, 0, st>>>(p->data, p->d_prod[kProd128w] + L.first);
         break;
@@ -556,7 +562,7 @@ static int enqueue_plan(fwx_batch_plan* p) {
   return FWX_OK;
 }
 
-// form: 0 min3, 1 pk64, 2 pk64 with pk16 128-wide products
+// form: 0 min3, 1 pk64, 2 pk64 with pk16 64- and 128-wide products
 static int enqueue_mode(fwx_batch_plan* p, int form) {
   return form == 2 ? enqueue_plan<true, true>(p) : form ? enqueue_plan<true, false>(p) : enqueue_plan<false, false>(p);
 }
@@ -875,7 +881,7 @@ int fwx_batch_plan_create(fwx_batch_plan** out, const int32_t* sizes, const int6
       bool wide = false;
       for (const auto* steps : {&P[q], &SP[q]})
         for (const auto& st : *steps)
-          for (const auto& kv : st) wide = wide || kv.first == kProd128 || kv.first == kProd128w;
+          for (const auto& kv : st) wide = wide || kv.first == kProd128 || kv.first == kProd128w || kv.first == kProd64;
       if (!wide) continue;
       const int32_t g = order[q];
       p->sizes.push_back(sizes[g]);
@@ -904,7 +910,7 @@ int fwx_batch_plan_run(fwx_batch_plan* p, int32_t mode) {
   bool pk = false;
   if (int rc = values_in_pk64_range(p->data, p->span, p->stream, &pk)) return rc;
   const bool persistent = mode == FWX_BATCH_PERSISTENT || mode == FWX_BATCH_PERSISTENT_GRAPH;
-  // the step launches take the pk16 form for their 128-wide products where
+  // the step launches take the pk16 form for their 64- and 128-wide products where
   // pk64 is exact; the persistent kernel keeps pk64
   const int m = !pk ? 0 : (!persistent && batch_pk16_enabled()) ? 2 : 1;
   p->last_form = m;

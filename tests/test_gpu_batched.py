@@ -263,23 +263,29 @@ def _graph(n, p, lo, hi, seed):
     return W
 
 
-@pytest.mark.parametrize("pk16", ["1", "0"])
-def test_batch_pk16_products_and_exact_finish(monkeypatch, pk16):
+@pytest.mark.parametrize("pk16,table", [("1", "64:32:2:512,127:128:1:512,32769:128:3:512"),
+                                         ("1", "64:32:2:512,127:128:1:512,32769:64:4:512"),
+                                         ("0", "64:32:2:512,127:128:1:512,32769:128:3:512")])
+def test_batch_pk16_products_and_exact_finish(monkeypatch, pk16, table):
     """pk16 128-wide products in a batch whose graphs do and do not reach the
     16-bit cap (dense small weights; unreachable pairs; weights around the
     cap; a graph too small for 128-wide products): every graph equals the
     oracle in both step-launch modes, the run reports the pk16 form, and the
     graphs with 128-wide products that hold cells at or above the cap are the
-    ones finished exactly.  FWX_PK16=0 runs pk64 with the same results."""
+    ones finished exactly; with 128-wide and with 64-wide products for the
+    graphs above n = 128.  FWX_PK16=0 runs pk64 with the same results."""
     import torch
     from fw_cuda_amd import _lib
     cap = _lib.PK16_CAP
     monkeypatch.setenv("FWX_PK16", pk16)
-    monkeypatch.setenv("FWX_BATCH_TABLE", "64:32:2:512,127:128:1:512,32769:128:3:512")
+    monkeypatch.setenv("FWX_BATCH_TABLE", table)
     graphs = [_graph(300, 0.9, 1, 1000, 1), _graph(300, 0.003, 1, 1000, 2), _graph(520, 0.5, 1, 1000, 3),
               _graph(400, 0.5, cap - 3, cap + 3, 4), _graph(100, 0.01, 1, 1000, 5), _graph(777, 0.02, 1, 1000, 6),
               _graph(260, 0.05, 2000, 9000, 7)]
     expect = [po.fw_rowpar(g) for g in graphs]
+    # graphs with 64- or 128-wide products under the forced tables: n > 128
+    # (a single 128-wide tile has no products; every graph above 128 has
+    # 64-wide ones under the second table)
     wide = [g.shape[0] > 128 for g in graphs]
     want_finished = sum(1 for w, e in zip(wide, expect) if w and (e >= cap).any())
     assert want_finished >= 2  # the batch exercises the exact finish

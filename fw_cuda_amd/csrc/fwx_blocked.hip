@@ -86,6 +86,31 @@ __global__ __launch_bounds__(256) void k_panel(int32_t* __restrict__ D, int32_t 
   tile_update<T, T, kChunk, VEC, 16, PK>(s, A, B, C, T);
 }
 
+// panel in the pk16 form (64-wide tiles, 256 threads of 4 x 4 cells).
+template <int T, bool VEC>
+__global__ __launch_bounds__(256) void k_panel16(int32_t* __restrict__ D, int32_t n, int32_t c, int32_t N,
+                                                 int32_t prio) {
+  if (prio) lead_wave_priority();
+  __shared__ TileSmem16<T, T, 16> s;
+  const int b = blockIdx.x;
+  const int32_t base = c * T;
+  const View diag = make_view(D, n, base, base);
+  View A, B, C;
+  if (b < N - 1) {
+    const int j = b + (b >= c);
+    C = make_view(D, n, base, j * T);
+    A = diag;
+    B = C;
+  } else {
+    int i = b - (N - 1);
+    i += (i >= c);
+    C = make_view(D, n, i * T, base);
+    A = C;
+    B = diag;
+  }
+  tile_update16<T, T, 16, VEC>(s, A, B, C, T);
+}
+
 // ---------------------------------------------------------------------------
 // dpanel: diag and panel fused into one launch.  Every block first closes the
 // diagonal tile itself and stores it (all blocks store the same bits), then
@@ -153,6 +178,36 @@ __global__ __launch_bounds__(256) void k_cross(int32_t* __restrict__ D, int32_t 
   const View B = make_view(D, n, c * T, j * T);
   const View C = make_view(D, n, i * T, j * T);
   tile_update<T, T, kChunk, VEC, 16, PK>(s, A, B, C, T);
+}
+
+// cross in the pk16 form (64-wide tiles).
+template <int T, bool VEC>
+__global__ __launch_bounds__(256) void k_cross16(int32_t* __restrict__ D, int32_t n, int32_t c, int32_t kb,
+                                                 int32_t Bst, int32_t N, int32_t prio) {
+  if (prio) lead_wave_priority();
+  __shared__ TileSmem16<T, T, 16> s;
+  int b = blockIdx.x;
+  const int l = c - kb;
+  const int r1 = (Bst - 1) * (N - 1);
+  int i, j;
+  if (b < r1) {
+    int m = b / (N - 1);
+    const int jj = b % (N - 1);
+    m += (m >= l);
+    i = kb + m;
+    j = jj + (jj >= c);
+  } else {
+    b -= r1;
+    const int ii = b / (Bst - 1);
+    int m = b % (Bst - 1);
+    m += (m >= l);
+    j = kb + m;
+    i = ii + (ii >= kb ? Bst : 0);
+  }
+  const View A = make_view(D, n, i * T, c * T);
+  const View B = make_view(D, n, c * T, j * T);
+  const View C = make_view(D, n, i * T, j * T);
+  tile_update16<T, T, 16, VEC>(s, A, B, C, T);
 }
 
 // ---------------------------------------------------------------------------
@@ -310,9 +365,31 @@ static int diag_launch(int32_t* d, int32_t n, int32_t c, int prio, hipStream_t s
   return FWX_OK;
 }
 
-template <int T, bool VEC, bool PK>
+// FWX_LEAD16=0 keeps the 64-wide lead products in pk64 when the solve runs
+// pk16 rest passes.
+static bool lead16_enabled() {
+  const char* e = std::getenv("FWX_LEAD16");
+  return !(e && e[0] == '0');
+}
+
+template <int T, bool VEC, bool PK, bool P16 = false>
 static int lead_round(int32_t* d, int32_t n, int32_t c, int32_t kb, int32_t B, int32_t N, int prio,
                       hipStream_t st) {
+  if (P16 && T == 64 && lead16_enabled()) {
+    if (int rc = diag_launch<T>(d, n, c, prio, st)) return rc;
+    if (N > 1) {
+      k_panel16<64, VEC><<<2 * (N - 1), 256, 0, st>>>(d, n, c, N, prio);
+      FWX_LAUNCHED("k_panel16");
+    }
+    if (B > 1) {
+      const int cnt = (B - 1) * (2 * N - B - 1);
+      if (cnt > 0) {
+        k_cross16<64, VEC><<<cnt, 256, 0, st>>>(d, n, c, kb, B, N, prio);
+        FWX_LAUNCHED("k_cross16");
+      }
+    }
+    return FWX_OK;
+  }
   if (N > 1 && fused_dpanel()) {
     k_dpanel<T, VEC, PK><<<2 * (N - 1), 256, 0, st>>>(d, n, c, N, prio);
     FWX_LAUNCHED("k_dpanel");
@@ -445,10 +522,10 @@ static int band_dispatch(int R, int32_t* d, int32_t n, int32_t s0, int32_t S, in
 }
 
 // The lead rounds of one stage.
-template <int T, bool VEC, bool PK>
+template <int T, bool VEC, bool PK, bool P16 = false>
 static int lead_rounds(int32_t* d, int32_t n, int32_t kb, int32_t B, int32_t N, int prio, hipStream_t st) {
   for (int l = 0; l < B; ++l)
-    if (int rc = lead_round<T, VEC, PK>(d, n, kb + l, kb, B, N, prio, st)) return rc;
+    if (int rc = lead_round<T, VEC, PK, P16>(d, n, kb + l, kb, B, N, prio, st)) return rc;
   return FWX_OK;
 }
 
@@ -458,11 +535,11 @@ static double lead_work(int32_t n, int32_t T, int32_t kb, int32_t B) {
 }
 
 // The lead rounds of one stage, bracketed for the profiler.
-template <int T, bool VEC, bool PK>
+template <int T, bool VEC, bool PK, bool P16 = false>
 static int lead_stage(int32_t* d, int32_t n, int32_t kb, int32_t B, int32_t N, int prio, hipStream_t st) {
   hipEvent_t ev;
   if (int rc = prof_open(st, &ev)) return rc;
-  if (int rc = lead_rounds<T, VEC, PK>(d, n, kb, B, N, prio, st)) return rc;
+  if (int rc = lead_rounds<T, VEC, PK, P16>(d, n, kb, B, N, prio, st)) return rc;
   return prof_close(st, ev, FWX_PROF_LEAD, lead_work(n, T, kb, B));
 }
 
@@ -479,7 +556,7 @@ static int run_blocked(int32_t* d, int32_t n, int32_t kappa, hipStream_t st) {
     const int B = min(kappa, N - kb);
     const int s0 = kb * T, S = B * T;
     const double L = (double)min(S, n - s0), nn = (double)n;
-    if (int rc = lead_rounds<T, VEC, PK>(d, n, kb, B, N, 0, st)) return rc;
+    if (int rc = lead_rounds<T, VEC, PK, P16>(d, n, kb, B, N, 0, st)) return rc;
     if (int rc = prof_chain(st, &ev, FWX_PROF_LEAD, lead_work(n, T, kb, B))) return rc;
     if (int rc = rest_dispatch<VEC, PK, P16>(R, d, n, s0, S, (S + R - 1) / R, st)) return rc;
     if (int rc = prof_chain(st, &ev, FWX_PROF_REST, (nn - L) * (nn - L) * L)) return rc;
@@ -517,7 +594,7 @@ static int run_blocked_lookahead(int32_t* d, int32_t n, int32_t kappa, hipStream
   hipEvent_t ev;
   if (int rc = prof_open(st, &ev)) return rc;
   const int B0 = min(kappa, N);
-  if (int rc = lead_rounds<T, VEC, PK>(d, n, 0, B0, N, 0, st)) return rc;
+  if (int rc = lead_rounds<T, VEC, PK, P16>(d, n, 0, B0, N, 0, st)) return rc;
   if (int rc = prof_chain(st, &ev, FWX_PROF_LEAD, lead_work(n, T, 0, B0))) return rc;
   for (int kb = 0; kb < N; kb += kappa) {
     const int B = min(kappa, N - kb);
@@ -534,7 +611,7 @@ static int run_blocked_lookahead(int32_t* d, int32_t n, int32_t kappa, hipStream
         return rc;
       if (!ev) FWX_HIP(hipEventRecord(evA, st));
       FWX_HIP(hipStreamWaitEvent(side, ev ? ev : evA, 0));
-      if (int rc = lead_stage<T, VEC, PK>(d, n, kb2, B2, N, prio, side)) return rc;
+      if (int rc = lead_stage<T, VEC, PK, P16>(d, n, kb2, B2, N, prio, side)) return rc;
       FWX_HIP(hipEventRecord(evB, side));
       if (int rc = rest_dispatch<VEC, PK, P16>(R, d, n, s0, S, en + fn, st)) return rc;
       if (int rc = prof_chain(st, &ev, FWX_PROF_REST, (nn - L - L2) * (nn - L - L2) * L)) return rc;
@@ -567,7 +644,7 @@ static int run_blocked_sideband(int32_t* d, int32_t n, int32_t kappa, hipStream_
   const int prio = lead_priority_enabled() ? 1 : 0;
   FWX_HIP(hipEventRecord(evR, st));  // the side stream starts after the caller's prior work
   FWX_HIP(hipStreamWaitEvent(side, evR, 0));
-  if (int rc = lead_stage<T, VEC, PK>(d, n, 0, min(kappa, N), N, prio, side)) return rc;
+  if (int rc = lead_stage<T, VEC, PK, P16>(d, n, 0, min(kappa, N), N, prio, side)) return rc;
   FWX_HIP(hipEventRecord(evL, side));
   hipEvent_t ev = nullptr;  // profiling markers of the rest passes on the caller's stream
   for (int kb = 0; kb < N; kb += kappa) {
@@ -589,7 +666,7 @@ static int run_blocked_sideband(int32_t* d, int32_t n, int32_t kappa, hipStream_
       if (int rc = band_dispatch<VEC, PK, P16>(R, d, n, s0, S, en, fn, side)) return rc;
       if (int rc = prof_chain(side, &es, FWX_PROF_BAND, ((nn - L) * (nn - L) - (nn - L - L2) * (nn - L - L2)) * L))
         return rc;
-      if (int rc = lead_rounds<T, VEC, PK>(d, n, kb2, B2, N, prio, side)) return rc;
+      if (int rc = lead_rounds<T, VEC, PK, P16>(d, n, kb2, B2, N, prio, side)) return rc;
       if (int rc = prof_close(side, es, FWX_PROF_LEAD, lead_work(n, T, kb2, B2))) return rc;
       FWX_HIP(hipEventRecord(evL, side));
       if (int rc = rest_dispatch<VEC, PK, P16>(R, d, n, s0, S, en + fn, st)) return rc;
@@ -633,7 +710,7 @@ static int run_blocked_ahead(int32_t* d, int32_t n, int32_t kappa, hipStream_t s
   const int prio = lead_priority_enabled() ? 1 : 0;
   FWX_HIP(hipEventRecord(evX, st));  // the side stream starts after the caller's prior work
   FWX_HIP(hipStreamWaitEvent(side, evX, 0));
-  if (int rc = lead_stage<T, VEC, PK>(d, n, 0, min(kappa, N), N, prio, side)) return rc;
+  if (int rc = lead_stage<T, VEC, PK, P16>(d, n, 0, min(kappa, N), N, prio, side)) return rc;
   FWX_HIP(hipEventRecord(evL, side));
   hipEvent_t ev = nullptr;  // profiling markers of the caller's stream
   for (int kb = 0; kb < N; kb += kappa) {
@@ -654,7 +731,7 @@ static int run_blocked_ahead(int32_t* d, int32_t n, int32_t kappa, hipStream_t s
       if (int rc = band_dispatch<VEC, PK, P16>(R, d, n, s0, S, en, fn, side)) return rc;
       if (int rc = prof_chain(side, &es, FWX_PROF_BAND, ((nn - L) * (nn - L) - (nn - L - L2) * (nn - L - L2)) * L))
         return rc;
-      if (int rc = lead_rounds<T, VEC, PK>(d, n, kb2, B2, N, prio, side)) return rc;
+      if (int rc = lead_rounds<T, VEC, PK, P16>(d, n, kb2, B2, N, prio, side)) return rc;
       if (int rc = prof_close(side, es, FWX_PROF_LEAD, lead_work(n, T, kb2, B2))) return rc;
       FWX_HIP(hipEventRecord(evL, side));
       if (kb3 < N) {

@@ -58,6 +58,32 @@ __global__ __launch_bounds__(NTH) void k_diag(int32_t* __restrict__ D, int32_t n
   close_tile<T, NTH>(make_view(D, n, c * T, c * T));
 }
 
+// diag closed by pk16 squaring when every cell of the tile is below
+// kSq16Small and the tile has no padding (close_sq16_regs), else by the T
+// steps; same closure either way.
+template <int T, int NTH>
+__global__ __launch_bounds__(NTH) void k_diag_sq(int32_t* __restrict__ D, int32_t n, int32_t c, int32_t prio) {
+  if (prio) lead_wave_priority();
+  __shared__ SqSmem16<T, NTH> s;
+  constexpr int RS = NTH / 16, TR = T / RS, TQ = T / 16;
+  const View V = make_view(D, n, c * T, c * T);
+  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+  int32_t acc[TR][TQ];
+#pragma unroll
+  for (int r = 0; r < TR; ++r)
+#pragma unroll
+    for (int q = 0; q < TQ; ++q)
+      acc[r][q] = V.in(ty + RS * r, tx + 16 * q) ? *V.at(ty + RS * r, tx + 16 * q) : kInf;
+  const int kmax = min(T, V.rlim);
+  // the T-step closure borrows the A staging buffer for its lines (8 T ints)
+  if (!close_sq16_regs<T, NTH>(acc, s, kmax)) close_regs<T, NTH>(acc, reinterpret_cast<int32_t*>(s.a), kmax);
+#pragma unroll
+  for (int r = 0; r < TR; ++r)
+#pragma unroll
+    for (int q = 0; q < TQ; ++q)
+      if (V.in(ty + RS * r, tx + 16 * q)) *V.at(ty + RS * r, tx + 16 * q) = acc[r][q];
+}
+
 // ---------------------------------------------------------------------------
 // panel: blocks [0, N-1) take row tile (c, j), blocks [N-1, 2N-2) column tile
 // (i, c).  The tile's own pre-update values are the second operand; it is
@@ -352,8 +378,24 @@ static int diag_threads() {
   return (v == 256 || v == 512 || v == 1024) ? v : kDiagThreads;
 }
 
+// FWX_DIAG_SQ=0 closes every diagonal tile by the T Floyd-Warshall steps;
+// else 32- and 64-wide tiles whose cells are all below kSq16Small close by
+// pk16 squaring (k_diag_sq).
+static bool diag_sq_enabled() {
+  const char* e = std::getenv("FWX_DIAG_SQ");
+  return !(e && e[0] == '0');
+}
+
 template <int T>
 static int diag_launch(int32_t* d, int32_t n, int32_t c, int prio, hipStream_t st) {
+  if (T <= 64 && diag_sq_enabled()) {
+    if (T == 32)
+      k_diag_sq<32, 256><<<1, 256, 0, st>>>(d, n, c, prio);
+    else
+      k_diag_sq<64, 512><<<1, 512, 0, st>>>(d, n, c, prio);
+    FWX_LAUNCHED("k_diag_sq");
+    return FWX_OK;
+  }
   const int nth = T == 32 ? 256 : diag_threads();
   if (nth == 1024)
     k_diag<T, T == 32 ? 256 : 1024><<<1, T == 32 ? 256 : 1024, 0, st>>>(d, n, c, prio);
@@ -872,9 +914,12 @@ static int enqueue_small(int32_t* d, int32_t n, hipStream_t st) {
 // measured on MI355X (tuned tile/kappa, best of 4-10 solves) the replay is
 // 10 % faster than direct launches at n=2048 and 2 % at n=4096, but 1.7 %
 // slower at n=8192, where launch gaps are negligible next to ~0.5 ms rest
-// kernels.  Profiled calls launch directly: events recorded by a graph's
-// event nodes cannot be timed with hipEventElapsedTime (the runtime reports
-// an invalid handle).
+// kernels.  The small-n rounds launch directly (FWX_SMALL_GRAPH=1 replays
+// them): one launch per round, and with the look-ahead closure by squaring
+// the direct launches measured 1-20 % faster than the replay at every n
+// from 64 to 1024 and equal at 1536-2048.  Profiled calls launch directly:
+// events recorded by a graph's event nodes cannot be timed with
+// hipEventElapsedTime (the runtime reports an invalid handle).
 
 namespace {
 
@@ -914,6 +959,11 @@ bool graphs_enabled() {
   return !(e && e[0] == '0');
 }
 
+bool small_graphs_enabled() {
+  const char* e = std::getenv("FWX_SMALL_GRAPH");
+  return e && e[0] == '1';
+}
+
 }  // namespace
 
 // The pk16 rest passes (fwx_tile16.hpp) run whenever the pk64 form is exact
@@ -936,7 +986,8 @@ static int run_form(int32_t* d, int32_t n, int32_t tile, int32_t kappa, bool vec
   int dev = 0;
   FWX_HIP(hipGetDevice(&dev));
   const int knobs = (fused_dpanel() ? 1 : 0) | (rest_variant(n) << 1) | (band_narrow() ? 8 : 0) | (diag_threads() << 4) |
-                    (lead_priority_enabled() ? 1 << 20 : 0) | (p16_chunk(n) == 16 ? 1 << 21 : 0);
+                    (lead_priority_enabled() ? 1 << 20 : 0) | (p16_chunk(n) == 16 ? 1 << 21 : 0) |
+                    (diag_sq_enabled() ? 1 << 22 : 0);
   const GraphKey key{d, n, tile, kappa, vec, form, small, small ? small_region_knob(n) : 0, lookahead_mode(n), knobs, st,
                      dev};
   std::lock_guard<std::mutex> lk(g_graph_mu);
@@ -1010,7 +1061,7 @@ int enqueue_blocked(int32_t* d, int32_t n, int32_t tile, int32_t kappa, hipStrea
   const bool direct = n > kGraphMaxN || !graphs_enabled() || profiling();
   t_last_form = -1;
   t_last_exact_finish = 0;
-  if (small) return run_form(d, n, tile, kappa, vec, 0, true, direct, st);
+  if (small) return run_form(d, n, tile, kappa, vec, 0, true, direct || !small_graphs_enabled(), st);
   bool pk = false;
   if (int rc = values_in_pk64_range(d, (int64_t)n * n, st, &pk)) return rc;
   const int form = !pk ? 0 : (pk16_enabled() && rest_width(tile, kappa) == 128) ? 2 : 1;

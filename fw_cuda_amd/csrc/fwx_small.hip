@@ -79,7 +79,7 @@ struct SmallSmem {
   alignas(16) int32_t pA[R * kLda];   // A panel: X(I,c), then P_A
   alignas(16) int2 pB[(kT / 2) * R];  // B panel: X(c,J), then P_B
   int32_t lines[8 * kT];              // closure
-  uint32_t wflags[16];                // form vote, one word per wave
+  uint32_t wflags[16];                // votes: one word per wave, two sets
 };
 
 // Block-wide vote on the operands a block loaded, with one barrier: bit 0 =
@@ -295,6 +295,30 @@ __device__ __forceinline__ void prod_lds16(const uint32_t* A, const uint64_t* B,
   }
 }
 
+// prod_lds16 with every operand of the 32 k read before the first
+// relaxation: one LDS latency per product instead of one per group of 4 k,
+// for the small operand sets of the 32 x 32 closure (TR = TP = 1 at 512
+// threads: 64 VGPRs of operands).
+template <int NT, int BM, int BN>
+__device__ __forceinline__ void prod_lds16_all(const uint32_t* A, const uint64_t* B, Pairs<NT, BM, BN>& acc, int tx,
+                                               int ty) {
+  constexpr int RS = NT / 16, TR = BM / RS, TP = BN / 32;
+  static_assert(TR * TP == 1, "operands of one cell pair per thread");
+  uint4 a[kT / 4];
+  uint64_t b[kT / 2];
+#pragma unroll
+  for (int g = 0; g < kT / 4; ++g) a[g] = *reinterpret_cast<const uint4*>(&A[ty * kLda + 4 * g]);
+#pragma unroll
+  for (int p = 0; p < kT / 2; ++p) b[p] = B[p * (BN / 2) + tx];
+#pragma unroll
+  for (int g = 0; g < kT / 4; ++g) {
+    const uint64_t a01 = (uint64_t)a[g].x | ((uint64_t)a[g].y << 32);
+    const uint64_t a23 = (uint64_t)a[g].z | ((uint64_t)a[g].w << 32);
+    acc[0][0] = relax4_pk16(acc[0][0], a01, b[2 * g]);
+    acc[0][0] = relax4_pk16(acc[0][0], a23, b[2 * g + 1]);
+  }
+}
+
 template <int NT, int BM, int BN>
 __device__ __forceinline__ void prod16(const uint32_t* A, const uint64_t* B, Cells<NT, BM, BN>& t, int tx, int ty) {
   Pairs<NT, BM, BN> acc;
@@ -357,110 +381,198 @@ __device__ __forceinline__ void lookahead_products(SmallSmem<R>& s, Cells<NT / 2
   prod_lds<NT, kT, kT, PK>(s.pA, s.pB, y, tx, ty);
 }
 
-// small_products in the pk16 form (operands already staged as pk16).
+// Closure of the 32 x 32 tile y (all cells below kSmall16) by min-plus
+// squaring in the pk16 form: y <- min(y, y (x) y) until a squaring changes
+// no cell, at most five times.  After s squarings y holds, per cell, the
+// shortest walk of at most 2^s edges over the tile's vertices; a squaring
+// that changes nothing means y obeys every triangle inequality, i.e. it is
+// the closure; after five, walks of 32 edges cover every simple path of the
+// 32 vertices.  So with no negative cycle the result is the closure the
+// 32 k steps of Floyd-Warshall give, in 2-5 dependent products instead of 32
+// dependent steps.  Sums stay below 2 kSmall16 (every value is a min over
+// the loaded ones and such sums), so nothing saturates.
+// One barrier per squaring: each squaring stages its result as the next
+// one's operands (into the other of the D* and panel staging buffers) before
+// the vote, whose barrier then also publishes them.  Buffer b is rewritten
+// two squarings after it was read, with a vote barrier in between; the
+// first squaring's buffers (D*, then the panels) were last read before
+// barriers of the look-ahead products.
 template <int R, int NT>
-__device__ __forceinline__ void small_products16(SmallSmem<R>& s, Cells<NT / 2, R, kT>& pa, Cells<NT / 2, kT, R>& pb,
-                                                 Cells<NT, R, R>& xc, bool upper) {
-  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4, th = (threadIdx.x & (NT / 2 - 1)) >> 4;
-  uint32_t* dA = reinterpret_cast<uint32_t*>(s.dA);
-  uint64_t* dB = reinterpret_cast<uint64_t*>(s.dB);
-  uint32_t* pA = reinterpret_cast<uint32_t*>(s.pA);
-  uint64_t* pB = reinterpret_cast<uint64_t*>(s.pB);
-  if (!upper)
-    prod16<NT / 2, R, kT>(pA, dB, pa, tx, th);  // P_A = min(X(I,c), X(I,c) (x) D*)
-  else
-    prod16<NT / 2, kT, R>(dA, pB, pb, tx, th);  // P_B = min(X(c,J), D* (x) X(c,J))
+__device__ __forceinline__ void close_sq16(SmallSmem<R>& s, Cells<NT, kT, kT>& y) {
+  static_assert(2 * (NT / 64) <= 16, "two sets of vote words");
+  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+  constexpr int TR = kT / (NT / 16), TP = kT / 32;
+  Pairs<NT, kT, kT> acc, was;
+  enc16<NT, kT, kT>(y, acc);
+  to_a16<NT, kT>(reinterpret_cast<uint32_t*>(s.dA), y, tx, ty);
+  to_b16<NT, kT>(reinterpret_cast<uint64_t*>(s.dB), y, tx, ty);
   __syncthreads();
-  if (!upper)
-    to_a16<NT / 2, R>(pA, pa, tx, th);
-  else
-    to_b16<NT / 2, R>(pB, pb, tx, th);
-  __syncthreads();
-  prod16<NT, R, R>(pA, pB, xc, tx, ty);
+  for (int it = 0; it < 5; ++it) {
+    const uint32_t* A = reinterpret_cast<const uint32_t*>((it & 1) ? s.pA : s.dA);
+    const uint64_t* B = reinterpret_cast<const uint64_t*>((it & 1) ? s.pB : s.dB);
+#pragma unroll
+    for (int r = 0; r < TR; ++r)
+#pragma unroll
+      for (int p = 0; p < TP; ++p) was[r][p] = acc[r][p];
+    if constexpr (NT == 512)
+      prod_lds16_all<NT, kT, kT>(A, B, acc, tx, ty);
+    else
+      prod_lds16<NT, kT, kT>(A, B, acc, tx, ty);
+    uint32_t changed = 0;
+#pragma unroll
+    for (int r = 0; r < TR; ++r)
+#pragma unroll
+      for (int p = 0; p < TP; ++p) changed |= acc[r][p] ^ was[r][p];
+    dec16<NT, kT, kT>(acc, y);
+    if (it + 1 < 5) {
+      to_a16<NT, kT>(reinterpret_cast<uint32_t*>((it & 1) ? s.dA : s.pA), y, tx, ty);
+      to_b16<NT, kT>(reinterpret_cast<uint64_t*>((it & 1) ? s.dB : s.pB), y, tx, ty);
+    }
+    // the two halves of the vote words alternate, so a wave's next vote
+    // cannot overwrite a word another wave has still to read
+    if (!form_vote<NT>(s.wflags + (it & 1) * (NT / 64), changed ? kSmall16 : 0u)) break;
+  }
 }
 
-// lookahead_products in the pk16 form.
-template <int R, int NT>
-__device__ __forceinline__ void lookahead_products16(SmallSmem<R>& s, Cells<NT / 2, kT, kT>& p,
-                                                     Cells<NT, kT, kT>& y, bool upper) {
-  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4, th = (threadIdx.x & (NT / 2 - 1)) >> 4;
-  uint32_t* dA = reinterpret_cast<uint32_t*>(s.dA);
-  uint64_t* dB = reinterpret_cast<uint64_t*>(s.dB);
-  uint32_t* pA = reinterpret_cast<uint32_t*>(s.pA);
-  uint64_t* pB = reinterpret_cast<uint64_t*>(s.pB);
-  if (!upper)
-    prod16<NT / 2, kT, kT>(pA, dB, p, tx, th);  // P(c+1,c)
-  else
-    prod16<NT / 2, kT, kT>(dA, pB, p, tx, th);  // P(c,c+1)
-  __syncthreads();
-  if (!upper)
-    to_a16<NT / 2, kT>(pA, p, tx, th);
-  else
-    to_b16<NT / 2, kT>(pB, p, tx, th);
-  __syncthreads();
-  prod16<NT, kT, kT>(pA, pB, y, tx, ty);
+// Read back, in the NT/2-thread layout of one half, a panel staged from the
+// NT-thread layout by to_a (BM x 32, unmasked) or to_b (32 x BN, unmasked).
+template <int NT2, int BM>
+__device__ __forceinline__ void from_a(const int32_t* A, Cells<NT2, BM, kT>& t, int tx, int th) {
+  constexpr int RS = NT2 / 16;
+#pragma unroll
+  for (int r = 0; r < BM / RS; ++r)
+#pragma unroll
+    for (int q = 0; q < 2; ++q) t[r][q] = A[(th + RS * r) * kLda + tx + 16 * q];
 }
 
+template <int NT2, int BN>
+__device__ __forceinline__ void from_b(const int2* B, Cells<NT2, kT, BN>& t, int tx, int th) {
+  constexpr int RS = NT2 / 16;
+  const int32_t* b = reinterpret_cast<const int32_t*>(B);
+#pragma unroll
+  for (int r = 0; r < kT / RS; ++r)
+#pragma unroll
+    for (int q = 0; q < BN / 16; ++q) {
+      const int row = th + RS * r, col = tx + 16 * q;
+      t[r][q] = b[((row >> 1) * BN + col) * 2 + (row & 1)];
+    }
+}
+
+// The round's product on a BM x BN window of rows [r0, r0 + BM) and columns
+// [col0, col0 + BN) by the single-panel identity:
+//   P_A = min(X(I,c), X(I,c) (x) D*)     (all NT threads)
+//   C  <- min(C, P_A (x) X(c,J))
+// which equals min(C, P_A (x) P_B) of the three-phase order when no value
+// is negative: P_A (x) P_B = min(P_A (x) X(c,J), P_A (x) D* (x) X(c,J)) and
+// P_A (x) D* >= X(I,c) (x) D* >= P_A, because D* (x) D* >= D* in a closed
+// tile.  The same bound makes a read of X(c,J) that the block holding it
+// has overwritten with P_B give the same result: P_A (x) P_B <= P_A (x) Y
+// <= P_A (x) X(c,J) for any mix Y.  Staged on entry (and published by a
+// barrier): D* as a B operand in dB, X(I,c) as A in pA, X(c,J) as B in pB
+// -- pk16 (FORM 2) or int32 pk64 (FORM 1); qa holds X(I,c), c the window.
+template <int R, int NT, int BM, int BN, int FORM>
+__device__ __forceinline__ void single_panel(SmallSmem<R>& s, Cells<NT, BM, kT>& qa, Cells<NT, BM, BN>& xc,
+                                             int32_t n, int32_t c0, int32_t r0) {
+  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+  if constexpr (FORM == 2) {
+    uint32_t* pA = reinterpret_cast<uint32_t*>(s.pA);
+    prod16<NT, BM, kT>(pA, reinterpret_cast<const uint64_t*>(s.dB), qa, tx, ty);
+    __syncthreads();
+    to_a16<NT, BM>(pA, qa, tx, ty);
+    __syncthreads();
+    prod16<NT, BM, BN>(pA, reinterpret_cast<const uint64_t*>(s.pB), xc, tx, ty);
+  } else {
+    prod_lds<NT, BM, kT, true>(s.pA, s.dB, qa, tx, ty);
+    __syncthreads();
+    to_a<NT, BM>(s.pA, qa, tx, ty, n - r0, n - c0);
+    __syncthreads();
+    prod_lds<NT, BM, BN, true>(s.pA, s.pB, xc, tx, ty);
+  }
+}
+
+// Stage D*, X(I,c) and X(c,J) for single_panel (FORM 2 or 1) or for the
+// two-panel min3 products (FORM 0, which also stages D* as an A operand);
+// then one barrier.  D* is masked to INF outside the matrix; the loaded
+// panels already are.
+template <int R, int NT, int BM, int BN, int FORM>
+__device__ __forceinline__ void stage_round(SmallSmem<R>& s, const Cells<NT, kT, kT>& d,
+                                            const Cells<NT, BM, kT>& qa, const Cells<NT, kT, BN>& qb, int32_t n,
+                                            int32_t c0) {
+  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+  if constexpr (FORM == 2) {
+    to_b16<NT, kT>(reinterpret_cast<uint64_t*>(s.dB), d, tx, ty);
+    to_a16<NT, BM>(reinterpret_cast<uint32_t*>(s.pA), qa, tx, ty);
+    to_b16<NT, BN>(reinterpret_cast<uint64_t*>(s.pB), qb, tx, ty);
+  } else {
+    if (FORM == 0) to_a<NT, kT>(s.dA, d, tx, ty, n - c0, n - c0);
+    to_b<NT, kT>(s.dB, d, tx, ty, n - c0, n - c0);
+    to_a<NT, BM>(s.pA, qa, tx, ty, BM, kT);
+    to_b<NT, BN>(s.pB, qb, tx, ty, kT, BN);
+  }
+  __syncthreads();
+}
+
+// The look-ahead block of round c (when c + 1 < N): the round's result on
+// tile (e, e) = (c+1, c+1), min(X(e,e), P(e,c) (x) X(c,e)) (single_panel; the
+// min3 form keeps the two panel products, P(e,c) and P(c,e), one per half),
+// closed -- the next round's D* -- and stored.  The region that holds tile
+// (e, e) does not store it this round.
 template <int R, int NT>
 __device__ __forceinline__ void lookahead_block(SmallSmem<R>& s, int32_t* D, int32_t n, int32_t c,
-                                                bool small16_enabled) {
+                                                bool small16_enabled, bool squaring) {
   const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4, th = (threadIdx.x & (NT / 2 - 1)) >> 4;
   const bool upper = threadIdx.x >= NT / 2;
   const int32_t c0 = c * kT, e0 = c0 + kT;
-  Cells<NT, kT, kT> d, y;
-  Cells<NT / 2, kT, kT> p;
+  Cells<NT, kT, kT> d, qa, qb, y;
   load_cells<NT, kT, kT>(D, n, c0, c0, tx, ty, d);
-  if (!upper)
-    load_cells<NT / 2, kT, kT>(D, n, e0, c0, tx, th, p);
-  else
-    load_cells<NT / 2, kT, kT>(D, n, c0, e0, tx, th, p);
+  load_cells<NT, kT, kT>(D, n, e0, c0, tx, ty, qa);
+  load_cells<NT, kT, kT>(D, n, c0, e0, tx, ty, qb);
   load_cells<NT, kT, kT>(D, n, e0, e0, tx, ty, y);
-  const uint32_t o = or_cells<NT, kT, kT>(d) | or_cells<NT / 2, kT, kT>(p) | or_cells<NT, kT, kT>(y);
+  const uint32_t o = or_cells<NT, kT, kT>(d) | or_cells<NT, kT, kT>(qa) | or_cells<NT, kT, kT>(qb) |
+                     or_cells<NT, kT, kT>(y);
   const uint32_t vote = form_vote<NT>(s.wflags, o);
   const bool pk = !(vote & 1u);
   const bool p16 = small16_enabled && !(vote & 2u);
   if (p16) {
-    to_a16<NT, kT>(reinterpret_cast<uint32_t*>(s.dA), d, tx, ty);
-    to_b16<NT, kT>(reinterpret_cast<uint64_t*>(s.dB), d, tx, ty);
-    if (!upper)
-      to_a16<NT / 2, kT>(reinterpret_cast<uint32_t*>(s.pA), p, tx, th);
+    stage_round<R, NT, kT, kT, 2>(s, d, qa, qb, n, c0);
+    single_panel<R, NT, kT, kT, 2>(s, qa, y, n, c0, e0);
+    if (squaring && n - e0 >= kT)
+      close_sq16<R, NT>(s, y);
     else
-      to_b16<NT / 2, kT>(reinterpret_cast<uint64_t*>(s.pB), p, tx, th);
-    __syncthreads();
-    lookahead_products16<R, NT>(s, p, y, upper);
+      close_regs<kT, NT>(y, s.lines, min(kT, n - e0));
+  } else if (pk) {
+    stage_round<R, NT, kT, kT, 1>(s, d, qa, qb, n, c0);
+    single_panel<R, NT, kT, kT, 1>(s, qa, y, n, c0, e0);
     close_regs<kT, NT>(y, s.lines, min(kT, n - e0));
-    store_cells<NT, kT, kT>(D, n, e0, e0, tx, ty, y);
-    return;
-  }
-  to_a<NT, kT>(s.dA, d, tx, ty, n - c0, n - c0);
-  to_b<NT, kT>(s.dB, d, tx, ty, n - c0, n - c0);
-  if (!upper)
-    to_a<NT / 2, kT>(s.pA, p, tx, th, kT, kT);
-  else
-    to_b<NT / 2, kT>(s.pB, p, tx, th, kT, kT);
-  __syncthreads();
-  if (pk)
-    lookahead_products<R, NT, true>(s, p, y, upper, n, c0, e0);
-  else
+  } else {
+    stage_round<R, NT, kT, kT, 0>(s, d, qa, qb, n, c0);
+    Cells<NT / 2, kT, kT> p;
+    if (!upper)
+      from_a<NT / 2, kT>(s.pA, p, tx, th);  // X(e, c)
+    else
+      from_b<NT / 2, kT>(s.pB, p, tx, th);  // X(c, e)
     lookahead_products<R, NT, false>(s, p, y, upper, n, c0, e0);
-  close_regs<kT, NT>(y, s.lines, min(kT, n - e0));
+    close_regs<kT, NT>(y, s.lines, min(kT, n - e0));
+  }
   store_cells<NT, kT, kT>(D, n, e0, e0, tx, ty, y);
 }
 
 // Round c: block 0 is the look-ahead block (when c + 1 < N), block b > 0 the
 // region b - 1 of M x M.  Tile (c, c) holds D* already (the previous
-// round's look-ahead block stored it; k_small_close0 for round 0).  Thread
-// (tx, ty) holds the region cells and the cells of tile (c,c) in the
-// NT-thread layout; the lower half of the block (th = its row within the
-// half) holds the A panel, the upper half the B panel.  The operands are
-// loaded at once, then one block-wide vote picks the form.
+// round's look-ahead block stored it; k_small_close0 for round 0).  Every
+// thread holds, in the NT-thread layout, its cells of tile (c,c), of the A
+// panel X(I,c), of the B panel X(c,J) and of the region; they are loaded at
+// once, then one block-wide vote picks the form.  pk16 and pk64 blocks run
+// single_panel; min3 blocks (negative values) the two panel products of the
+// three-phase order, one per half of the block (operands read back from
+// the staging in the half's layout).
 template <int R, int NT>
 __global__ __launch_bounds__(NT) void k_small_round(int32_t* __restrict__ D, int32_t n, int32_t c, int32_t M,
-                                                    int32_t N, int32_t small16_enabled) {
+                                                    int32_t N, int32_t flags) {
   __shared__ SmallSmem<R> s;
   const bool ahead = c + 1 < N;
   if (ahead && blockIdx.x == 0) {
-    lookahead_block<R, NT>(s, D, n, c, small16_enabled != 0);
+    lookahead_block<R, NT>(s, D, n, c, (flags & 1) != 0, (flags & 2) != 0);
     return;
   }
   const int b = blockIdx.x - (ahead ? 1 : 0);
@@ -469,43 +581,34 @@ __global__ __launch_bounds__(NT) void k_small_round(int32_t* __restrict__ D, int
   const int I = b / M, J = b - (b / M) * M;
   const int32_t c0 = c * kT, r0 = I * R, col0 = J * R;
   Cells<NT, kT, kT> d;
-  Cells<NT / 2, R, kT> pa;
-  Cells<NT / 2, kT, R> pb;
+  Cells<NT, R, kT> qa;
+  Cells<NT, kT, R> qb;
   Cells<NT, R, R> xc;
   load_cells<NT, kT, kT>(D, n, c0, c0, tx, ty, d);
-  if (!upper)
-    load_cells<NT / 2, R, kT>(D, n, r0, c0, tx, th, pa);
-  else
-    load_cells<NT / 2, kT, R>(D, n, c0, col0, tx, th, pb);
+  load_cells<NT, R, kT>(D, n, r0, c0, tx, ty, qa);
+  load_cells<NT, kT, R>(D, n, c0, col0, tx, ty, qb);
   load_cells<NT, R, R>(D, n, r0, col0, tx, ty, xc);
-  const uint32_t o = or_cells<NT, kT, kT>(d) | (upper ? or_cells<NT / 2, kT, R>(pb) : or_cells<NT / 2, R, kT>(pa)) |
+  const uint32_t o = or_cells<NT, kT, kT>(d) | or_cells<NT, R, kT>(qa) | or_cells<NT, kT, R>(qb) |
                      or_cells<NT, R, R>(xc);
   const uint32_t vote = form_vote<NT>(s.wflags, o);
   const bool pk = !(vote & 1u);
-  const bool p16 = small16_enabled && !(vote & 2u);
+  const bool p16 = (flags & 1) && !(vote & 2u);
   if (p16) {
-    to_a16<NT, kT>(reinterpret_cast<uint32_t*>(s.dA), d, tx, ty);
-    to_b16<NT, kT>(reinterpret_cast<uint64_t*>(s.dB), d, tx, ty);
+    stage_round<R, NT, R, R, 2>(s, d, qa, qb, n, c0);
+    single_panel<R, NT, R, R, 2>(s, qa, xc, n, c0, r0);
+  } else if (pk) {
+    stage_round<R, NT, R, R, 1>(s, d, qa, qb, n, c0);
+    single_panel<R, NT, R, R, 1>(s, qa, xc, n, c0, r0);
+  } else {
+    stage_round<R, NT, R, R, 0>(s, d, qa, qb, n, c0);
+    Cells<NT / 2, R, kT> pa;
+    Cells<NT / 2, kT, R> pb;
     if (!upper)
-      to_a16<NT / 2, R>(reinterpret_cast<uint32_t*>(s.pA), pa, tx, th);
+      from_a<NT / 2, R>(s.pA, pa, tx, th);
     else
-      to_b16<NT / 2, R>(reinterpret_cast<uint64_t*>(s.pB), pb, tx, th);
-    __syncthreads();
-    small_products16<R, NT>(s, pa, pb, xc, upper);
-    store_cells<NT, R, R>(D, n, r0, col0, tx, ty, xc, ahead ? c0 + kT : -kT);
-    return;
-  }
-  to_a<NT, kT>(s.dA, d, tx, ty, n - c0, n - c0);
-  to_b<NT, kT>(s.dB, d, tx, ty, n - c0, n - c0);
-  if (!upper)
-    to_a<NT / 2, R>(s.pA, pa, tx, th, R, kT);
-  else
-    to_b<NT / 2, R>(s.pB, pb, tx, th, kT, R);
-  __syncthreads();
-  if (pk)
-    small_products<R, NT, true>(s, pa, pb, xc, upper, n, c0, r0, col0);
-  else
+      from_b<NT / 2, R>(s.pB, pb, tx, th);
     small_products<R, NT, false>(s, pa, pb, xc, upper, n, c0, r0, col0);
+  }
   // tile (c+1, c+1) is the look-ahead block's to store
   store_cells<NT, R, R>(D, n, r0, col0, tx, ty, xc, ahead ? c0 + kT : -kT);
 }
@@ -537,11 +640,18 @@ int small16() {
   return (e && e[0] == '0') ? 0 : 1;
 }
 
+// FWX_SMALL_SQ=0 closes the look-ahead tile by the 32 Floyd-Warshall steps
+// in every form (read at every call); else pk16 tiles close by squaring.
+int small_sq() {
+  const char* e = std::getenv("FWX_SMALL_SQ");
+  return (e && e[0] == '0') ? 0 : 1;
+}
+
 }  // namespace
 
 int enqueue_small_rounds(int32_t* d, int32_t n, hipStream_t st) {
   const int N = (n + kT - 1) / kT;
-  const int R = small_region(n), NT = small_threads(), p16 = small16();
+  const int R = small_region(n), NT = small_threads(), p16 = small16() | (small_sq() << 1);
   const int M = (n + R - 1) / R;
   k_small_close0<<<1, 256, 0, st>>>(d, n);
   FWX_LAUNCHED("k_small_close0");
@@ -560,6 +670,8 @@ int enqueue_small_rounds(int32_t* d, int32_t n, hipStream_t st) {
   return FWX_OK;
 }
 
-int small_region_knob(int32_t n) { return small_region(n) | (small_threads() << 8) | (small16() << 16); }
+int small_region_knob(int32_t n) {
+  return small_region(n) | (small_threads() << 8) | (small16() << 16) | (small_sq() << 17);
+}
 
 }  // namespace fwx

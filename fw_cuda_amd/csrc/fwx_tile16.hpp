@@ -248,4 +248,126 @@ __device__ __forceinline__ void tile_update16(TileSmem16<BM, BN, KC>& s, const V
   }
 }
 
+// ---------------------------------------------------------------------------
+// Closure of a T x T diagonal tile by min-plus squaring in the pk16 form.
+//
+// The tile sits in registers in close_regs's layout (thread (tx, ty) holds
+// cells (ty + RS r, tx + 16 q), RS = NTH / 16).  When every cell lies in
+// [0, kSq16Small) and the tile has no padding, y <- min(y, y (x) y) runs
+// until a squaring changes no cell, at most log2(T) times: after s
+// squarings a cell holds the shortest walk of at most 2^s edges over the
+// tile's vertices; a squaring that changes nothing leaves y obeying every
+// triangle inequality, i.e. y is the closure; after log2(T) squarings,
+// walks of T edges cover every simple path.  With no negative cycle (all
+// cells are >= 0) that is the closure the T Floyd-Warshall steps give, in
+// 2 to log2(T) dependent tile products instead of T dependent steps.  Every
+// value is a min over the tile's values and sums of two of them, so sums
+// stay below 2 kSq16Small <= CAP and nothing saturates.  Otherwise the
+// function returns false and the caller runs the T steps.
+// A operand: (v + BIAS) in both lanes, row-major with stride T + 4 (16-byte
+// rows); B operand: per k pair, column pairs (j, j + 16) with
+// j = (cp % 16) + 32 (cp / 16) -- the pairs a thread's cells form -- one
+// 8-byte slot each (low word row 2p, high word row 2p + 1).
+constexpr uint32_t kSq16Small = 0x1000u;
+
+template <int T, int NTH>
+struct alignas(16) SqSmem16 {
+  alignas(16) uint32_t a[T * (T + 4)];
+  alignas(16) uint64_t b[(T / 2) * (T / 2)];
+  uint32_t votes[2][NTH / 64];
+};
+
+// true if any thread of the block passes pred; one barrier.
+template <int NTH>
+__device__ __forceinline__ bool block_any(uint32_t* words, bool pred) {
+  const bool w = __ballot(pred) != 0;
+  if ((threadIdx.x & 63) == 0) words[threadIdx.x >> 6] = w ? 1u : 0u;
+  __syncthreads();
+  uint32_t any = 0;
+#pragma unroll
+  for (int i = 0; i < NTH / 64; ++i) any |= words[i];
+  return any != 0;
+}
+
+template <int T, int NTH>
+__device__ __forceinline__ bool close_sq16_regs(int32_t (&acc)[T / (NTH / 16)][T / 16], SqSmem16<T, NTH>& s,
+                                                int kmax) {
+  constexpr int RS = NTH / 16, TR = T / RS, TQ = T / 16, TP = T / 32, LDA = T + 4;
+  constexpr int LOG2T = T == 32 ? 5 : T == 64 ? 6 : 7;
+  static_assert(T == 32 || T == 64 || T == 128, "tile width");
+  static_assert(RS % 16 == 0 && T % RS == 0, "layout must tile T");
+  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+  uint32_t o = 0;
+#pragma unroll
+  for (int r = 0; r < TR; ++r)
+#pragma unroll
+    for (int q = 0; q < TQ; ++q) o |= (uint32_t)acc[r][q];
+  if (kmax < T || block_any<NTH>(s.votes[0], o >= kSq16Small)) return false;
+  uint32_t* bw = reinterpret_cast<uint32_t*>(s.b);
+  for (int it = 0; it < LOG2T; ++it) {
+    // stage y as both operands (the previous squaring's reads ended before
+    // its vote barrier)
+#pragma unroll
+    for (int r = 0; r < TR; ++r) {
+      const int row = ty + RS * r;
+#pragma unroll
+      for (int q = 0; q < TQ; ++q) s.a[row * LDA + tx + 16 * q] = ((uint32_t)acc[r][q] + kP16Bias) * 0x10001u;
+#pragma unroll
+      for (int qp = 0; qp < TP; ++qp)
+        bw[((row >> 1) * (T / 2) + tx + 16 * qp) * 2 + (row & 1)] =
+            (uint32_t)acc[r][2 * qp] | ((uint32_t)acc[r][2 * qp + 1] << 16);
+    }
+    __syncthreads();
+    uint32_t c[TR][TP], was[TR][TP];
+#pragma unroll
+    for (int r = 0; r < TR; ++r)
+#pragma unroll
+      for (int qp = 0; qp < TP; ++qp) {
+        c[r][qp] = ((uint32_t)acc[r][2 * qp] | ((uint32_t)acc[r][2 * qp + 1] << 16)) + kP16Bias2;
+        was[r][qp] = c[r][qp];
+      }
+    // c <- min(c, y (x) y): groups of 4 k, the next group read while the
+    // current one is computed
+    uint4 a[2][TR];
+    uint64_t b[2][2][TP];
+    auto load = [&](int buf, int g) {
+#pragma unroll
+      for (int r = 0; r < TR; ++r) a[buf][r] = *reinterpret_cast<const uint4*>(&s.a[(ty + RS * r) * LDA + 4 * g]);
+#pragma unroll
+      for (int qp = 0; qp < TP; ++qp) {
+        b[buf][0][qp] = s.b[(2 * g) * (T / 2) + tx + 16 * qp];
+        b[buf][1][qp] = s.b[(2 * g + 1) * (T / 2) + tx + 16 * qp];
+      }
+    };
+    load(0, 0);
+#pragma unroll 2
+    for (int g = 0; g < T / 4; ++g) {
+      const int cur = g & 1;
+      if (g + 1 < T / 4) load(cur ^ 1, g + 1);
+#pragma unroll
+      for (int r = 0; r < TR; ++r) {
+        const uint64_t a01 = (uint64_t)a[cur][r].x | ((uint64_t)a[cur][r].y << 32);
+        const uint64_t a23 = (uint64_t)a[cur][r].z | ((uint64_t)a[cur][r].w << 32);
+#pragma unroll
+        for (int qp = 0; qp < TP; ++qp) {
+          c[r][qp] = relax4_pk16(c[r][qp], a01, b[cur][0][qp]);
+          c[r][qp] = relax4_pk16(c[r][qp], a23, b[cur][1][qp]);
+        }
+      }
+    }
+    uint32_t changed = 0;
+#pragma unroll
+    for (int r = 0; r < TR; ++r)
+#pragma unroll
+      for (int qp = 0; qp < TP; ++qp) {
+        changed |= c[r][qp] ^ was[r][qp];
+        const uint32_t v = c[r][qp] - kP16Bias2;
+        acc[r][2 * qp] = (int32_t)(v & 0xFFFFu);
+        acc[r][2 * qp + 1] = (int32_t)(v >> 16);
+      }
+    if (!block_any<NTH>(s.votes[(it + 1) & 1], changed != 0)) break;
+  }
+  return true;
+}
+
 }  // namespace fwx

@@ -433,6 +433,7 @@ def test_small_rounds_every_form_and_mode(monkeypatch, region, threads):
             for small, graph in (("1", "1"), ("1", "1"), ("1", "0"), ("0", "1"), ("0", "0"), ("1", "1")):
                 monkeypatch.setenv("FWX_SMALL", small)
                 monkeypatch.setenv("FWX_GRAPH", graph)
+                monkeypatch.setenv("FWX_SMALL_GRAPH", graph)
                 for tag, src, exp in cases:
                     say("small rounds", region, threads, n, small, graph, tag)
                     dm.upload(src)
@@ -440,13 +441,14 @@ def test_small_rounds_every_form_and_mode(monkeypatch, region, threads):
                     assert (dm.download() == exp).all(), (region, threads, n, small, graph, tag)
 
 
-def test_small_rounds_profiled_and_graph_eviction():
+def test_small_rounds_profiled_and_graph_eviction(monkeypatch):
     """Profiled small solves launch directly and account n^3 relaxations each;
-    more distinct buffers than the graph cache holds evict small-n graphs
-    (each holding a range-flag slot), and solves after the evictions stay
-    exact."""
+    with small-n graph replay on (FWX_SMALL_GRAPH=1), more distinct buffers
+    than the graph cache holds evict small-n graphs, and solves after the
+    evictions stay exact."""
     import ctypes
     from fw_cuda_amd import _lib
+    monkeypatch.setenv("FWX_SMALL_GRAPH", "1")
     n = 300
     g = fx.generate_graph(n, 0.5, 17, 1, 1000)
     gneg = reweighted(np.random.default_rng(6), n, 0.5)
@@ -478,25 +480,43 @@ def test_small_rounds_profiled_and_graph_eviction():
             dm.free()
 
 
-@pytest.mark.parametrize("small16", ["1", "0"])
-def test_small_rounds_pk16_blocks(monkeypatch, small16):
+def chain_graph(n, back=4000):
+    """Every edge i -> j present: weight 1 when j = i + 1, else `back` (all
+    below 4096, so every small-n block runs pk16).  The shortest path from i
+    to j > i is the chain through every vertex between them, so closing a
+    diagonal tile needs paths of 31 edges: five squarings."""
+    W = np.full((n, n), back, dtype=np.int32)
+    W[np.arange(n - 1), np.arange(1, n)] = 1
+    np.fill_diagonal(W, 0)
+    return W
+
+
+@pytest.mark.parametrize("small16,squaring", [("1", "1"), ("1", "0"), ("0", "1")])
+def test_small_rounds_pk16_blocks(monkeypatch, small16, squaring):
     """The small-n rounds' pk16 blocks (every loaded operand below 4096, so no
     sum reaches the 16-bit cap): dense small weights (almost every block
     pk16 after round 0), weights straddling the threshold (blocks of both
-    forms in one round), and distances growing past it during the solve, at
-    ragged and aligned sizes, graph-replayed and direct; FWX_SMALL16=0 keeps
-    pk64/min3 with the same results."""
+    forms in one round), distances growing past it during the solve, and a
+    chain whose diagonal tiles need the full five squarings of the
+    look-ahead closure, at ragged and aligned sizes, graph-replayed and
+    direct; FWX_SMALL16=0 keeps pk64/min3 and FWX_SMALL_SQ=0 the 32-step
+    closure, with the same results."""
     monkeypatch.setenv("FWX_SMALL16", small16)
+    monkeypatch.setenv("FWX_SMALL_SQ", squaring)
     rng = np.random.default_rng(99)
     for n in (33, 64, 200, 512, 1024, 1537):
         for tag, lo, hi, p in (("small", 1, 1000, 0.9), ("straddle", 1000, 5000, 0.5),
-                               ("growing", 500, 2000, 0.02)):
-            W = fx.generate_graph(n, p, int(rng.integers(1, 2**31)), lo, hi)
+                               ("growing", 500, 2000, 0.02), ("chain", 0, 0, 0)):
+            if tag == "chain":
+                W = chain_graph(n)
+            else:
+                W = fx.generate_graph(n, p, int(rng.integers(1, 2**31)), lo, hi)
             exp = po.fw_rowpar(W)
             with fx.DeviceMatrix(n) as dm:
                 for graph in ("1", "0", "1"):
                     monkeypatch.setenv("FWX_GRAPH", graph)
-                    say("small pk16", small16, n, tag, graph)
+                    monkeypatch.setenv("FWX_SMALL_GRAPH", graph)
+                    say("small pk16", small16, squaring, n, tag, graph)
                     dm.upload(W)
                     fx.run_device("tiling_gpu", dm.ptr, n, 32, 1)
-                    assert (dm.download() == exp).all(), (small16, n, tag, graph)
+                    assert (dm.download() == exp).all(), (small16, squaring, n, tag, graph)

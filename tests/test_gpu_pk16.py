@@ -118,3 +118,31 @@ def test_pk16_repeated_replays_alternate_finish(cases):
             fx.run_device("multi_layer_tiling_gpu", dm.ptr, 1024, 64, 2)
             assert fx.last_solve_form() == (_lib.FORM_PK16, i % 2)
             assert (dm.download() == want).all(), i
+
+
+def chain(n, back=4000):
+    """i -> i+1 weighs 1, every other edge `back` (< 4096): the shortest
+    path from i to j > i runs through every vertex between them, so closing
+    a diagonal tile of width T needs walks of T - 1 edges (log2(T)
+    squarings)."""
+    W = np.full((n, n), back, dtype=np.int32)
+    W[np.arange(n - 1), np.arange(1, n)] = 1
+    np.fill_diagonal(W, 0)
+    return W
+
+
+@pytest.mark.parametrize("sq", ["1", "0"])
+@pytest.mark.parametrize("tile,kappa", [(64, 4), (64, 1), (32, 4)])
+def test_diag_closure_by_squaring(monkeypatch, sq, tile, kappa):
+    """The lead rounds' diagonal closure by pk16 squaring (k_diag_sq) on
+    dense small weights (every tile after the first stage squares), a chain
+    (tiles needing every squaring), sparse ragged graphs (INF cells and a
+    padded last tile: the T-step closure) and weights around the 4096
+    eligibility bound; FWX_DIAG_SQ=0 closes by the T steps; all exact."""
+    monkeypatch.setenv("FWX_DIAG_SQ", sq)
+    cases = [("dense", graph(1024, 0.9, 1, 1000, 21)), ("chain", chain(1000)),
+             ("sparse ragged", graph(1001, 0.01, 1, 1000, 22)), ("around 4096", graph(777, 0.3, 3000, 5000, 23))]
+    for tag, W in cases:
+        say("diag squaring", sq, tile, kappa, tag)
+        got = fx.floyd_warshall(W, "multi_layer_tiling_gpu", tile, kappa)
+        assert (got == po.fw_rowpar(W)).all(), (tag, sq, tile, kappa)

@@ -78,6 +78,17 @@ __global__ __launch_bounds__(NTH) void k_batch_diag(int32_t* __restrict__ data,
   close_tile<T, NTH>(make_view(data + it.base, it.n, it.c0, it.c0));
 }
 
+// The same closure by pk16 squaring where every cell of the tile is below
+// kSq16Small and the tile has no padding (close_tile_sq), else by the T
+// steps.
+template <int T, int NTH>
+__global__ __launch_bounds__(NTH) void k_batch_diag_sq(int32_t* __restrict__ data,
+                                                       const DiagItem* __restrict__ items) {
+  __shared__ SqSmem16<T, NTH> s;
+  const DiagItem it = items[blockIdx.x];
+  close_tile_sq<T, NTH>(make_view(data + it.base, it.n, it.c0, it.c0), s);
+}
+
 template <int R, bool PK>
 __global__ __launch_bounds__(256) void k_batch_prod(int32_t* __restrict__ data,
                                                     const ProdItem* __restrict__ items) {
@@ -341,6 +352,13 @@ bool batch_diag_wide() {
   return !(e && std::strcmp(e, "256") == 0);
 }
 
+// FWX_DIAG_SQ=0: every diagonal closure by the T steps (read when a plan
+// enqueues or captures its launches).
+bool batch_diag_sq() {
+  const char* e = std::getenv("FWX_DIAG_SQ");
+  return !(e && e[0] == '0');
+}
+
 int diag_kind(int T) { return T == 32 ? kDiag32 : T == 64 ? kDiag64 : kDiag128; }
 int prod_kind(int R, int rest_threads) {
   return R == 32 ? kProd32 : R == 64 ? kProd64 : rest_threads == 256 ? kProd128w : kProd128;
@@ -523,15 +541,28 @@ static int enqueue_plan(fwx_batch_plan* p) {
       }
       st = p->side;
     }
+    const bool sq = batch_diag_sq();
     switch (L.kind) {
-      case kDiag32: k_batch_diag<32, 256><<<L.count, 256, 0, st>>>(p->data, p->d_diag[kDiag32] + L.first); break;
+      case kDiag32:
+        if (sq)
+          k_batch_diag_sq<32, 256><<<L.count, 256, 0, st>>>(p->data, p->d_diag[kDiag32] + L.first);
+        else
+          k_batch_diag<32, 256><<<L.count, 256, 0, st>>>(p->data, p->d_diag[kDiag32] + L.first);
+        break;
       case kDiag64:
-        if (batch_diag_wide())
+        if (sq)
+          k_batch_diag_sq<64, 512><<<L.count, 512, 0, st>>>(p->data, p->d_diag[kDiag64] + L.first);
+        else if (batch_diag_wide())
           k_batch_diag<64, 512><<<L.count, 512, 0, st>>>(p->data, p->d_diag[kDiag64] + L.first);
         else
           k_batch_diag<64, 256><<<L.count, 256, 0, st>>>(p->data, p->d_diag[kDiag64] + L.first);
         break;
-      case kDiag128: k_batch_diag<128, 512><<<L.count, 512, 0, st>>>(p->data, p->d_diag[kDiag128] + L.first); break;
+      case kDiag128:
+        if (sq)
+          k_batch_diag_sq<128, 512><<<L.count, 512, 0, st>>>(p->data, p->d_diag[kDiag128] + L.first);
+        else
+          k_batch_diag<128, 512><<<L.count, 512, 0, st>>>(p->data, p->d_diag[kDiag128] + L.first);
+        break;
       case kProd32: k_batch_prod<32, PK><<<L.count, 256, 0, st>>>(p->data, p->d_prod[kProd32] + L.first); break;
       case kProd64:
         if (P16)

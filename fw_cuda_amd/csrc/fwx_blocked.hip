@@ -65,23 +65,7 @@ template <int T, int NTH>
 __global__ __launch_bounds__(NTH) void k_diag_sq(int32_t* __restrict__ D, int32_t n, int32_t c, int32_t prio) {
   if (prio) lead_wave_priority();
   __shared__ SqSmem16<T, NTH> s;
-  constexpr int RS = NTH / 16, TR = T / RS, TQ = T / 16;
-  const View V = make_view(D, n, c * T, c * T);
-  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
-  int32_t acc[TR][TQ];
-#pragma unroll
-  for (int r = 0; r < TR; ++r)
-#pragma unroll
-    for (int q = 0; q < TQ; ++q)
-      acc[r][q] = V.in(ty + RS * r, tx + 16 * q) ? *V.at(ty + RS * r, tx + 16 * q) : kInf;
-  const int kmax = min(T, V.rlim);
-  // the T-step closure borrows the A staging buffer for its lines (8 T ints)
-  if (!close_sq16_regs<T, NTH>(acc, s, kmax)) close_regs<T, NTH>(acc, reinterpret_cast<int32_t*>(s.a), kmax);
-#pragma unroll
-  for (int r = 0; r < TR; ++r)
-#pragma unroll
-    for (int q = 0; q < TQ; ++q)
-      if (V.in(ty + RS * r, tx + 16 * q)) *V.at(ty + RS * r, tx + 16 * q) = acc[r][q];
+  close_tile_sq<T, NTH>(make_view(D, n, c * T, c * T), s);
 }
 
 // ---------------------------------------------------------------------------
@@ -207,11 +191,58 @@ __global__ __launch_bounds__(256) void k_cross(int32_t* __restrict__ D, int32_t 
 }
 
 // cross in the pk16 form (64-wide tiles).
+// Round c's update of tile (c+1, c+1) (a cross tile when c+1 lies in the
+// stage) followed by its closure -- the next round's diagonal -- in the
+// block that owns the tile, so the next round needs no diagonal launch.  No
+// other block of the launch reads or writes that tile.  The product's
+// result goes through LDS into the closure's register layout: a cell whose
+// pk16 result reached the cap keeps its input value (as store_c16 does).
+template <int T, bool VEC, bool EDGE>
+__device__ __forceinline__ void cross_and_close(TileSmem16<T, T, 16>& ps, SqSmem16<T, 256>& ss, const View& A,
+                                                const View& B, const View& C) {
+  using P = TileProduct16<T, T, 16, VEC, EDGE>;
+  uint32_t acc[P::TM][P::TP];
+  P::load_c16(C, acc);
+  P p;
+  p.run16(ps, A, B, T, acc);  // ends with a barrier: the product's LDS is free
+  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+  int32_t* tile = reinterpret_cast<int32_t*>(ss.a);
+#pragma unroll
+  for (int r = 0; r < P::TM; ++r)
+#pragma unroll
+    for (int g = 0; g < P::TG; ++g) {
+      const int i = ty + 16 * r, j = 4 * (tx + 16 * g);
+      const uint32_t lo = acc[r][2 * g] - kP16Bias2, hi = acc[r][2 * g + 1] - kP16Bias2;
+      const uint32_t e[4] = {lo & 0xFFFFu, lo >> 16, hi & 0xFFFFu, hi >> 16};
+#pragma unroll
+      for (int x = 0; x < 4; ++x)
+        tile[i * T + j + x] = e[x] < kP16Cap ? (int32_t)e[x] : (C.in(i, j + x) ? *C.at(i, j + x) : kInf);
+    }
+  __syncthreads();
+  constexpr int TR = T / 16, TQ = T / 16;
+  int32_t cl[TR][TQ];
+#pragma unroll
+  for (int r = 0; r < TR; ++r)
+#pragma unroll
+    for (int q = 0; q < TQ; ++q) cl[r][q] = tile[(ty + 16 * r) * T + tx + 16 * q];
+  __syncthreads();  // the closure reuses the buffer
+  const int kmax = min(T, C.rlim);
+  if (!close_sq16_regs<T, 256>(cl, ss, kmax)) close_regs<T, 256>(cl, reinterpret_cast<int32_t*>(ss.a), kmax);
+#pragma unroll
+  for (int r = 0; r < TR; ++r)
+#pragma unroll
+    for (int q = 0; q < TQ; ++q)
+      if (C.in(ty + 16 * r, tx + 16 * q)) *C.at(ty + 16 * r, tx + 16 * q) = cl[r][q];
+}
+
 template <int T, bool VEC>
 __global__ __launch_bounds__(256) void k_cross16(int32_t* __restrict__ D, int32_t n, int32_t c, int32_t kb,
-                                                 int32_t Bst, int32_t N, int32_t prio) {
+                                                 int32_t Bst, int32_t N, int32_t prio, int32_t close_next) {
   if (prio) lead_wave_priority();
-  __shared__ TileSmem16<T, T, 16> s;
+  __shared__ union {
+    TileSmem16<T, T, 16> prod;
+    SqSmem16<T, 256> sq;
+  } s;
   int b = blockIdx.x;
   const int l = c - kb;
   const int r1 = (Bst - 1) * (N - 1);
@@ -233,7 +264,15 @@ __global__ __launch_bounds__(256) void k_cross16(int32_t* __restrict__ D, int32_
   const View A = make_view(D, n, i * T, c * T);
   const View B = make_view(D, n, c * T, j * T);
   const View C = make_view(D, n, i * T, j * T);
-  tile_update16<T, T, 16, VEC>(s, A, B, C, T);
+  // block l (N - 1) + c holds tile (c+1, c+1)
+  if (close_next && (int)blockIdx.x == l * (N - 1) + c) {
+    if (C.rlim >= T && C.clim >= T)
+      cross_and_close<T, VEC, false>(s.prod, s.sq, A, B, C);
+    else
+      cross_and_close<T, VEC, true>(s.prod, s.sq, A, B, C);
+    return;
+  }
+  tile_update16<T, T, 16, VEC>(s.prod, A, B, C, T);
 }
 
 // ---------------------------------------------------------------------------
@@ -414,11 +453,24 @@ static bool lead16_enabled() {
   return !(e && e[0] == '0');
 }
 
+// FWX_LEAD_CLOSE=0 gives every pk16 lead round its own diagonal launch;
+// else round c's cross launch also closes tile (c+1, c+1) (cross_and_close)
+// and only a stage's first round launches k_diag.
+static bool lead_close_enabled() {
+  const char* e = std::getenv("FWX_LEAD_CLOSE");
+  return !(e && e[0] == '0');
+}
+
 template <int T, bool VEC, bool PK, bool P16 = false>
 static int lead_round(int32_t* d, int32_t n, int32_t c, int32_t kb, int32_t B, int32_t N, int prio,
                       hipStream_t st) {
   if (P16 && T == 64 && lead16_enabled()) {
-    if (int rc = diag_launch<T>(d, n, c, prio, st)) return rc;
+    // with B > 1 every round but the stage's last has a cross launch, which
+    // then closes the next round's diagonal tile
+    const int l = c - kb;
+    const bool chain = lead_close_enabled() && B > 1;
+    if (l == 0 || !chain)
+      if (int rc = diag_launch<T>(d, n, c, prio, st)) return rc;
     if (N > 1) {
       k_panel16<64, VEC><<<2 * (N - 1), 256, 0, st>>>(d, n, c, N, prio);
       FWX_LAUNCHED("k_panel16");
@@ -426,7 +478,7 @@ static int lead_round(int32_t* d, int32_t n, int32_t c, int32_t kb, int32_t B, i
     if (B > 1) {
       const int cnt = (B - 1) * (2 * N - B - 1);
       if (cnt > 0) {
-        k_cross16<64, VEC><<<cnt, 256, 0, st>>>(d, n, c, kb, B, N, prio);
+        k_cross16<64, VEC><<<cnt, 256, 0, st>>>(d, n, c, kb, B, N, prio, chain && l + 1 < B ? 1 : 0);
         FWX_LAUNCHED("k_cross16");
       }
     }
@@ -987,7 +1039,7 @@ static int run_form(int32_t* d, int32_t n, int32_t tile, int32_t kappa, bool vec
   FWX_HIP(hipGetDevice(&dev));
   const int knobs = (fused_dpanel() ? 1 : 0) | (rest_variant(n) << 1) | (band_narrow() ? 8 : 0) | (diag_threads() << 4) |
                     (lead_priority_enabled() ? 1 << 20 : 0) | (p16_chunk(n) == 16 ? 1 << 21 : 0) |
-                    (diag_sq_enabled() ? 1 << 22 : 0);
+                    (diag_sq_enabled() ? 1 << 22 : 0) | (lead_close_enabled() ? 1 << 23 : 0);
   const GraphKey key{d, n, tile, kappa, vec, form, small, small ? small_region_knob(n) : 0, lookahead_mode(n), knobs, st,
                      dev};
   std::lock_guard<std::mutex> lk(g_graph_mu);

@@ -370,4 +370,27 @@ __device__ __forceinline__ bool close_sq16_regs(int32_t (&acc)[T / (NTH / 16)][T
   return true;
 }
 
+// Close the T x T tile at V in place: by squaring where close_sq16_regs
+// applies, else by the T Floyd-Warshall steps (which borrow the A staging
+// buffer for their lines).  Cells outside the matrix read as INF and are
+// not stored.
+template <int T, int NTH>
+__device__ __forceinline__ void close_tile_sq(const View V, SqSmem16<T, NTH>& s) {
+  constexpr int RS = NTH / 16, TR = T / RS, TQ = T / 16;
+  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+  int32_t acc[TR][TQ];
+#pragma unroll
+  for (int r = 0; r < TR; ++r)
+#pragma unroll
+    for (int q = 0; q < TQ; ++q)
+      acc[r][q] = V.in(ty + RS * r, tx + 16 * q) ? *V.at(ty + RS * r, tx + 16 * q) : kInf;
+  const int kmax = min(T, V.rlim);
+  if (!close_sq16_regs<T, NTH>(acc, s, kmax)) close_regs<T, NTH>(acc, reinterpret_cast<int32_t*>(s.a), kmax);
+#pragma unroll
+  for (int r = 0; r < TR; ++r)
+#pragma unroll
+    for (int q = 0; q < TQ; ++q)
+      if (V.in(ty + RS * r, tx + 16 * q)) *V.at(ty + RS * r, tx + 16 * q) = acc[r][q];
+}
+
 }  // namespace fwx

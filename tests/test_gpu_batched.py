@@ -303,3 +303,40 @@ def test_batch_pk16_products_and_exact_finish(monkeypatch, pk16, table):
         got, flat = unpack(t, sizes, offs)
         check_all(got, expect, flat, flat0, sizes, offs)
     plan.destroy()
+
+
+def _chain(n, back=4000):
+    """i -> i+1 weighs 1, every other edge `back` (< 4096): closing a diagonal
+    tile of width T needs walks of T - 1 edges (every squaring)."""
+    W = np.full((n, n), back, dtype=np.int32)
+    W[np.arange(n - 1), np.arange(1, n)] = 1
+    np.fill_diagonal(W, 0)
+    return W
+
+
+@pytest.mark.parametrize("sq", ["1", "0"])
+@pytest.mark.parametrize("table", ["64:32:2:512,127:128:1:512,32769:128:3:512",
+                                   "64:32:2:512,127:128:1:512,32769:64:4:512"])
+def test_batch_diag_closure_by_squaring(monkeypatch, sq, table):
+    """Diagonal closures of 32-, 64- and 128-wide tiles by pk16 squaring
+    (k_batch_diag_sq) in a batch of dense small weights (tiles square after
+    the first stage), chains (every squaring), sparse graphs with INF cells
+    and ragged last tiles (the T-step closure), and weights around the 4096
+    eligibility bound; FWX_DIAG_SQ=0 closes every tile by the T steps; each
+    graph equals the oracle in both step-launch modes."""
+    import torch
+    from fw_cuda_amd import _lib
+    monkeypatch.setenv("FWX_DIAG_SQ", sq)
+    monkeypatch.setenv("FWX_BATCH_TABLE", table)
+    graphs = [_graph(300, 0.9, 1, 1000, 11), _chain(384), _chain(50), _graph(64, 0.9, 1, 1000, 12),
+              _graph(517, 0.01, 1, 1000, 13), _graph(260, 0.3, 3000, 5000, 14), _chain(700)]
+    expect = [po.fw_rowpar(g) for g in graphs]
+    sizes, offs, t, flat0 = upload(graphs)
+    plan = fx.BatchPlan(sizes, offs, _lib.ptr(t))
+    for mode in ("graph", "direct"):
+        t.copy_(torch.from_numpy(flat0).to("cuda:0"))
+        say("batch diag squaring", sq, table, mode)
+        plan.run(mode)
+        got, flat = unpack(t, sizes, offs)
+        check_all(got, expect, flat, flat0, sizes, offs)
+    plan.destroy()

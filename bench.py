@@ -385,15 +385,17 @@ def traffic_from_profiles(n, tile, kappa, kernel):
     return t.get("bytes_per_launch")
 
 
-def rest_kernel_name(tile, kappa, form):
+def rest_kernel_name(tile, kappa, form, packed=False):
     """The rest-pass kernel the library launches for this configuration: block
     R = 128 if the stage width kappa*tile is a multiple of 128, else 64, else
-    32; R = 128 runs the pk16 kernel in the pk16 form, else the 512-thread
-    kernel."""
+    32; R = 128 runs the pk16 kernel in the pk16 form (on packed stage panels
+    when the solve packed them), else the 512-thread kernel."""
     span = tile * kappa
     R = 128 if span % 128 == 0 else 64 if span % 64 == 0 else 32
     if R == 128:
-        return "k_rest16<128>" if form == 2 else "k_rest512<128>"
+        if form == 2:
+            return "k_rest16p<128>" if packed else "k_rest16<128>"
+        return "k_rest512<128>"
     return f"k_rest<{R}>"
 
 
@@ -429,6 +431,7 @@ def main():
     # the relaxation form the solves ran in (fwx_last_solve_form) fixes the
     # dominant kernel's VALU instruction ceiling
     form, exact_finish = fx.last_solve_form() if a.warmup else (_lib.FORM_PK64, 0)
+    packed = fx.last_solve_packed() if a.warmup else False
     cus = torch.cuda.get_device_properties(dev).multi_processor_count
     ceiling = instruction_ceiling(cus, form)
     # timed region: the steps alone, no profiler markers
@@ -534,14 +537,14 @@ def main():
                    "n": n, "tile": tile, "kappa": kappa, "algorithm": a.algo},
         "pct_valu_minplus_peak": 100.0 * value / PEAK_2OP,
         "frac_of_instruction_ceiling": value / ceiling,
-        "roofline": {"bound": "valu", "kernel": rest_kernel_name(tile, kappa, form), "achieved": achieved,
+        "roofline": {"bound": "valu", "kernel": rest_kernel_name(tile, kappa, form, packed), "achieved": achieved,
                      "peak": ceiling, "unit": "relaxations/s", "frac": achieved / ceiling,
                      "form": FORM_NAMES.get(form, str(form)), "exact_finish": exact_finish,
                      "peak_basis": f"{FORM_NAMES.get(form, form)} form: {cus} CUs x 64 lanes x 2.4 GHz, "
                                    f"{FORM_BASIS.get(form, '')}; the metric's 2-op peak is "
                                    f"{PEAK_2OP:.4g}",
                      "x_2op_peak": achieved / PEAK_2OP,
-                     "traffic": traffic_from_profiles(n, tile, kappa, rest_kernel_name(tile, kappa, form)),
+                     "traffic": traffic_from_profiles(n, tile, kappa, rest_kernel_name(tile, kappa, form, packed)),
                      "measured_over": f"{a.steps} profiled steps after the timed region "
                                       f"({ms_per_step_prof:.3f} ms per step with the markers)",
                      "launches": int(rest_launches), "avg_launch_ms": rest_ms / max(1, rest_launches),

@@ -148,6 +148,14 @@ def last_solve_form():
     return f.value, e.value
 
 
+def last_solve_packed():
+    """True when this thread's last plain blocked solve ran its pk16 rest
+    passes on packed stage panels (fwx_last_solve_packed)."""
+    p = ctypes.c_int32()
+    check(lib().fwx_last_solve_packed(ctypes.byref(p)))
+    return bool(p.value)
+
+
 def run_device(algo, d_ptr, n, tile=0, kappa=0, stream=None):
     """In-place FW on a device buffer through the registry (fwx_run)."""
     s = ctypes.c_void_p(stream) if stream else None

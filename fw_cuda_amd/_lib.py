@@ -51,6 +51,7 @@ SIGNATURES = {
     "fwx_tuned_config": (ctypes.c_int, [_i32, _i32p, _i32p]),
     "fwx_relaxation_form": (ctypes.c_int, [_vp, _i32, _i32p, _vp]),
     "fwx_last_solve_form": (ctypes.c_int, [_i32p, _i32p]),
+    "fwx_last_solve_packed": (ctypes.c_int, [_i32p]),
     "fwx_event_create": (ctypes.c_int, [ctypes.POINTER(_vp)]),
     "fwx_event_record": (ctypes.c_int, [_vp, _vp]),
     "fwx_event_synchronize": (ctypes.c_int, [_vp]),

@@ -341,6 +341,53 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(KC == 16 ? 
   tile_update16<R, R, KC, VEC>(s, A, B, C, S);
 }
 
+// Stage panels packed for k_rest16p (PackedPanels in fwx_tile16.hpp): blocks
+// [0, na) write wa[i * S + k] = p16_dup(D[i][s0 + k]) (i < n, k < S; one
+// 16-byte group of k per thread), the others wb[p * (n / 2) + c] =
+// p16_pair(D[s0 + 2p][2c], D[s0 + 2p][2c + 1]) | p16_pair(row s0 + 2p + 1)
+// << 32 (p < S / 2; two slots per thread).  Needs n % 4 == 0, S % 4 == 0,
+// s0 + S <= n, a 16-byte aligned D and cells in [0, 2^30) (the pk16 form's
+// precondition).
+__global__ __launch_bounds__(256) void k_pack16(const int32_t* __restrict__ D, int32_t n, int32_t s0, int32_t S,
+                                                uint32_t* __restrict__ wa, uint64_t* __restrict__ wb,
+                                                int32_t na, int32_t prio) {
+  if (prio) lead_wave_priority();
+  if ((int32_t)blockIdx.x < na) {
+    const int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t i = q / (S / 4);
+    const int32_t k = (int32_t)(q % (S / 4)) * 4;
+    if (i >= n) return;
+    const int4 v = *reinterpret_cast<const int4*>(D + i * n + s0 + k);
+    *reinterpret_cast<uint4*>(wa + i * S + k) = make_uint4(p16_dup(v.x), p16_dup(v.y), p16_dup(v.z), p16_dup(v.w));
+  } else {
+    const int64_t q = (int64_t)(blockIdx.x - na) * 256 + threadIdx.x;
+    const int64_t p = q / (n / 4);
+    const int32_t c4 = (int32_t)(q % (n / 4)) * 4;
+    if (p >= S / 2) return;
+    const int4 r0 = *reinterpret_cast<const int4*>(D + (s0 + 2 * p) * (int64_t)n + c4);
+    const int4 r1 = *reinterpret_cast<const int4*>(D + (s0 + 2 * p + 1) * (int64_t)n + c4);
+    ulonglong2 v;
+    v.x = (uint64_t)p16_pair(r0.x, r0.y) | ((uint64_t)p16_pair(r1.x, r1.y) << 32);
+    v.y = (uint64_t)p16_pair(r0.z, r0.w) | ((uint64_t)p16_pair(r1.z, r1.w) << 32);
+    *reinterpret_cast<ulonglong2*>(wb + p * (n / 2) + c4 / 2) = v;
+  }
+}
+
+// The pk16 rest pass on packed panels: the tiles and XCD mapping of
+// k_rest16, 16-deep chunks staged by plain copies.  n % R == 0 (every tile
+// whole), S % 16 == 0.
+template <int R, int WAVES>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES, WAVES))) void k_rest16p(
+    int32_t* __restrict__ D, int32_t n, const uint32_t* __restrict__ wa, const uint64_t* __restrict__ wb, int32_t S,
+    int32_t sr0, int32_t srn, int32_t m, int32_t per) {
+  __shared__ TileSmem16<R, R, 16> s;
+  int I, J;
+  if (!rest_tile_of_block(m, per, sr0, srn, &I, &J)) return;
+  const PackedPanels w{wa + (int64_t)I * R * S, S, wb + (int64_t)J * (R / 2), n / 2};
+  const View C = make_view(D, n, I * R, J * R);
+  tile_update16_packed<R, R, 16>(s, w, C, S);
+}
+
 // ---------------------------------------------------------------------------
 // rest_band: the part of stage s's rest pass that the next stage's lead rounds
 // read -- R x R tiles outside stage s's band [e0, e0+en) with a row or a
@@ -398,6 +445,99 @@ __global__ __launch_bounds__(256) void k_rest_band16(int32_t* __restrict__ D, in
 
 // ---------------------------------------------------------------------------
 // host driver
+
+// Packed stage panels for the pk16 rest passes (k_pack16 -> k_rest16p).  A
+// solve holds one workspace of three panel buffers: stage s packs into buffer
+// s % 3 after its lead rounds, on the stream that ran them, and its rest pass
+// reads that buffer.  Three, because in the side-band-ahead schedule the side
+// stream packs stage s + 1 while the caller's stream may still run the rest
+// pass of stage s - 1 (band(s) waits only for X(s - 1), which the caller's
+// stream runs after Y(s - 2)); stage s + 1's buffer is then stage s - 2's,
+// whose rest pass has ended.  The other schedules order packs after older
+// rest passes too.  Workspaces are allocated on first need per device, kept
+// for reuse and never freed; a solve that cannot get one runs unpacked.
+struct PackWs {
+  int dev = -1;
+  size_t cap_a = 0, cap_b = 0;  // words / slots per buffer
+  uint32_t* wa[3] = {};
+  uint64_t* wb[3] = {};
+  bool busy = false;
+  // the holding solve's configuration
+  int32_t tile = 0, kappa = 0;
+};
+
+static std::mutex g_pack_mu;
+static std::vector<PackWs*> g_pack;
+static thread_local PackWs* t_pack = nullptr;
+
+// FWX_PACK16=0 runs the pk16 rest passes on the int32 panels (conversion in
+// every block).
+static bool pack16_enabled() {
+  const char* e = std::getenv("FWX_PACK16");
+  return !(e && e[0] == '0');
+}
+
+// Waves per SIMD of k_rest16p: 5 (its 90 VGPRs and 24 KB of LDS fit five
+// blocks per CU; the int32-panel kernel's 116 VGPRs held it to four), or 4
+// with FWX_REST16P_WAVES=4.  Measured on MI355X: n=8192 (64, 4) rest launch
+// 265.9 vs 269.1 us, alone 279.6 vs 283.0 us; n=32768 (64, 8) 0.528 vs
+// 0.536 s per solve (int32 panels: 271.6 / 291.8 us, 0.547 s).
+static int rest16p_waves() {
+  const char* e = std::getenv("FWX_REST16P_WAVES");
+  return (e && e[0] == '4') ? 4 : 5;
+}
+
+static PackWs* pack_acquire(int32_t n, int32_t tile, int32_t kappa) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return nullptr;
+  const int64_t span = (int64_t)tile * kappa;
+  const size_t need_a = (size_t)n * (size_t)span, need_b = (size_t)(span / 2) * (size_t)(n / 2);
+  std::lock_guard<std::mutex> lk(g_pack_mu);
+  PackWs* w = nullptr;
+  for (PackWs* c : g_pack)
+    if (!c->busy && c->dev == dev && c->cap_a >= need_a && c->cap_b >= need_b) {
+      w = c;
+      break;
+    }
+  if (!w) {
+    char* base = nullptr;
+    if (hipMalloc(reinterpret_cast<void**>(&base), 3 * (need_a * 4 + need_b * 8)) != hipSuccess) {
+      (void)hipGetLastError();
+      return nullptr;
+    }
+    w = new PackWs;
+    w->dev = dev;
+    w->cap_a = need_a;
+    w->cap_b = need_b;
+    for (int i = 0; i < 3; ++i) {
+      w->wb[i] = reinterpret_cast<uint64_t*>(base + i * need_b * 8);
+      w->wa[i] = reinterpret_cast<uint32_t*>(base + 3 * need_b * 8 + i * need_a * 4);
+    }
+    g_pack.push_back(w);
+  }
+  w->busy = true;
+  w->tile = tile;
+  w->kappa = kappa;
+  return w;
+}
+
+static void pack_release(PackWs* w) {
+  if (!w) return;
+  std::lock_guard<std::mutex> lk(g_pack_mu);
+  w->busy = false;
+}
+
+// Packs stage kb's panels (s0 = kb T, width S) into its buffer, on st.
+static int pack_stage(int32_t* d, int32_t n, int32_t kb, int32_t S, int prio, hipStream_t st) {
+  const PackWs* w = t_pack;
+  const int b = (kb / w->kappa) % 3;
+  const int32_t s0 = kb * w->tile;
+  const int64_t ga = (int64_t)n * (S / 4), gb = (int64_t)(S / 2) * (n / 4);
+  const int na = (int)((ga + 255) / 256), nb = (int)((gb + 255) / 256);
+  k_pack16<<<na + nb, 256, 0, st>>>(d, n, s0, S, w->wa[b], w->wb[b], na, prio);
+  FWX_LAUNCHED("k_pack16");
+  return FWX_OK;
+}
 
 // FWX_DPANEL=1 fuses diag and panel into one launch (measured slower on
 // MI355X: the redundant closures contend with the rest pass).
@@ -579,6 +719,16 @@ static int rest_pass16(int32_t* d, int32_t n, int32_t s0, int32_t S, int32_t exc
   if (m <= 0) return FWX_OK;
   const int64_t total = (int64_t)m * m;
   const int per = (int)((total + 7) / 8);
+  if (t_pack && R == 128) {
+    // this stage's packed panels (pack_stage ran after its lead rounds)
+    const int b = (s0 / (t_pack->tile * t_pack->kappa)) % 3;
+    if (rest16p_waves() == 5)
+      k_rest16p<128, 5><<<8 * per, 256, 0, st>>>(d, n, t_pack->wa[b], t_pack->wb[b], S, sr0, srn, m, per);
+    else
+      k_rest16p<128, 4><<<8 * per, 256, 0, st>>>(d, n, t_pack->wa[b], t_pack->wb[b], S, sr0, srn, m, per);
+    FWX_LAUNCHED("k_rest16p");
+    return FWX_OK;
+  }
   if (p16_chunk(n) == 16)
     k_rest16<R, VEC, 16><<<8 * per, 256, 0, st>>>(d, n, s0, S, sr0, srn, m, per);
   else
@@ -620,6 +770,8 @@ template <int T, bool VEC, bool PK, bool P16 = false>
 static int lead_rounds(int32_t* d, int32_t n, int32_t kb, int32_t B, int32_t N, int prio, hipStream_t st) {
   for (int l = 0; l < B; ++l)
     if (int rc = lead_round<T, VEC, PK, P16>(d, n, kb + l, kb, B, N, prio, st)) return rc;
+  // the stage's panels are final: pack them for its rest pass
+  if (P16 && t_pack) return pack_stage(d, n, kb, B * T, prio, st);
   return FWX_OK;
 }
 
@@ -1101,6 +1253,9 @@ int enqueue_blocked_exact(int32_t* d, int32_t n, hipStream_t st) {
 
 static thread_local int t_last_form = -1;
 static thread_local int t_last_exact_finish = 0;
+static thread_local int t_last_packed = 0;
+
+int last_solve_packed() { return t_last_packed; }
 
 void last_solve_form(int32_t* form, int32_t* exact_finish) {
   *form = t_last_form;
@@ -1113,19 +1268,37 @@ int enqueue_blocked(int32_t* d, int32_t n, int32_t tile, int32_t kappa, hipStrea
   const bool direct = n > kGraphMaxN || !graphs_enabled() || profiling();
   t_last_form = -1;
   t_last_exact_finish = 0;
+  t_last_packed = 0;
   if (small) return run_form(d, n, tile, kappa, vec, 0, true, direct || !small_graphs_enabled(), st);
   bool pk = false;
   if (int rc = values_in_pk64_range(d, (int64_t)n * n, st, &pk)) return rc;
   const int form = !pk ? 0 : (pk16_enabled() && rest_width(tile, kappa) == 128) ? 2 : 1;
   t_last_form = form;
-  if (int rc = run_form(d, n, tile, kappa, vec, form, false, direct, st)) return rc;
+  // Direct pk16 solves of whole 128-wide tiles with 16-deep chunks run their
+  // rest passes on packed stage panels; the workspace is held until
+  // values_below below has synchronised the stream.
+  PackWs* ws = nullptr;
+  if (form == 2 && direct && vec && n % 128 == 0 && p16_chunk(n) == 16 && pack16_enabled())
+    ws = pack_acquire(n, tile, kappa);
+  t_pack = ws;
+  t_last_packed = ws ? 1 : 0;
+  int rc = run_form(d, n, tile, kappa, vec, form, false, direct, st);
+  t_pack = nullptr;
+  if (rc) {
+    if (ws) (void)hipStreamSynchronize(st);
+    pack_release(ws);
+    return rc;
+  }
   if (form != 2) return FWX_OK;
   // Every cell below the pk16 cap is exact.  If any cell is at or above it,
   // the matrix (walk lengths of the input graph, each at most its input
   // value) is solved once more in the exact pk64 form: its closure equals
   // the input's.
   bool below = false;
-  if (int rc = values_below(d, (int64_t)n * n, (int32_t)kP16Cap, st, &below)) return rc;
+  rc = values_below(d, (int64_t)n * n, (int32_t)kP16Cap, st, &below);
+  if (rc && ws) (void)hipStreamSynchronize(st);
+  pack_release(ws);
+  if (rc) return rc;
   if (below) return FWX_OK;
   t_last_exact_finish = 1;
   return run_form(d, n, tile, kappa, vec, 1, false, direct, st);

@@ -238,6 +238,13 @@ int fwx_last_solve_form(int32_t* form, int32_t* exact_finish) {
   return FWX_OK;
 }
 
+int fwx_last_solve_packed(int32_t* packed) {
+  clear_error();
+  if (!packed) return fail(FWX_ERR_INVALID_ARG, "null output pointer");
+  *packed = last_solve_packed();
+  return FWX_OK;
+}
+
 // ---- events (the cudaEvent calls of src/timer.cu:48-63) ---------------------
 int fwx_event_create(void** ev) {
   clear_error();

@@ -40,6 +40,8 @@ int enqueue_blocked(int32_t* d, int32_t n, int32_t tile, int32_t kappa, hipStrea
 int enqueue_blocked_exact(int32_t* d, int32_t n, hipStream_t st);
 // The form of this thread's last enqueue_blocked (fwx_last_solve_form).
 void last_solve_form(int32_t* form, int32_t* exact_finish);
+// 1 when that solve's pk16 rest passes ran on packed stage panels.
+int last_solve_packed();
 int enqueue_refshape(int32_t* d, int32_t n, hipStream_t st);
 // The fused rounds of a small solve on 32-wide k-tiles (fwx_small.hip), and
 // the region width they use for size n (a graph-cache key).

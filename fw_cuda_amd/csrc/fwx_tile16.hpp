@@ -249,6 +249,88 @@ __device__ __forceinline__ void tile_update16(TileSmem16<BM, BN, KC>& s, const V
 }
 
 // ---------------------------------------------------------------------------
+// Pre-packed operands.  A stage's panels, packed once (k_pack16 in
+// fwx_blocked.hip) in the exact byte layout of the LDS chunks above:
+//   A: row i holds e(D[i][s0 + k]) + BIAS in both lanes for k < depth (u32,
+//      row stride lda words) -- p16_dup of the int32 cell;
+//   B: k pair p holds, per column pair (2c, 2c + 1), one 8-byte slot
+//      p16_pair(row 2p) | p16_pair(row 2p + 1) << 32 (row stride ldb slots).
+// The product then stages a chunk by plain 16-byte copies: no conversion
+// instructions, and B takes half the bytes of its int32 cells.  Edge-free
+// only: the caller guarantees whole tiles and a depth that is a multiple of
+// KC.
+struct PackedPanels {
+  const uint32_t* a;  // (0, 0) of this tile's A rows
+  int32_t lda;        // words
+  const uint64_t* b;  // (0, 0) of this tile's B column pairs
+  int32_t ldb;        // slots
+};
+
+template <int BM, int BN, int KC>
+struct PackedProduct16 {
+  using P = TileProduct16<BM, BN, KC, true, false>;
+  using Smem = TileSmem16<BM, BN, KC>;
+  static constexpr int NT = 256;
+  static constexpr int LDA = tile_lda<KC>();
+  static constexpr int AG = KC / 4;                     // 16-byte groups per A row
+  static constexpr int AROWS = NT / AG;                 // A rows per pass
+  static constexpr int APASS = BM / AROWS;
+  static constexpr int BROW = BN / 4;                   // threads per k-pair row (2 slots each)
+  static constexpr int BROWS = NT / BROW;               // k-pair rows per pass
+  static constexpr int BPASS = (KC / 2) / BROWS;
+  static_assert(APASS == 2 && BPASS == 1, "the 128 x 128, 16-deep shape");
+
+  // acc <- min(acc, A (x) B) over k in [0, depth), depth a multiple of KC.
+  // The staged groups are locals of this loop (two A groups and one B group
+  // per thread); held as members of a product object, they were placed in
+  // LDS by the compiler.
+  __device__ __forceinline__ static void run(Smem& s, const PackedPanels& w, int32_t depth,
+                                             uint32_t (&acc)[P::TM][P::TP]) {
+    const int tid = threadIdx.x;
+    const uint32_t oa = (uint32_t)((tid / AG) * w.lda + (tid % AG) * 4) * 4u;
+    const uint32_t ob = (uint32_t)((tid / BROW) * w.ldb + (tid % BROW) * 2) * 8u;
+    const int sa0 = ((tid / AG) * LDA + (tid % AG) * 4) / 4;
+    const int sa1 = ((AROWS + tid / AG) * LDA + (tid % AG) * 4) / 4;
+    const int sb = ((tid / BROW) * (BN / 2) + (tid % BROW) * 2) / 2;
+    uint4 ra0, ra1, rb;
+    auto fetch = [&](int32_t kc0) {
+      ra0 = *at_bytes(reinterpret_cast<const uint4*>(w.a + kc0), oa);
+      ra1 = *at_bytes(reinterpret_cast<const uint4*>(w.a + kc0 + AROWS * w.lda), oa);
+      rb = *at_bytes(reinterpret_cast<const uint4*>(w.b + (kc0 / 2) * w.ldb), ob);
+    };
+    // the positions stash16 writes, the same bytes
+    auto stash = [&](int buf) {
+      reinterpret_cast<uint4*>(s.a[buf])[sa0] = ra0;
+      reinterpret_cast<uint4*>(s.a[buf])[sa1] = ra1;
+      reinterpret_cast<uint4*>(s.b[buf])[sb] = rb;
+    };
+    const int nch = depth / KC;
+    fetch(0);
+    stash(0);
+    __syncthreads();
+    for (int ch = 0; ch < nch; ++ch) {
+      const int buf = ch & 1;
+      if (ch + 1 < nch) fetch((ch + 1) * KC);
+      P::compute16(s, buf, acc);
+      if (ch + 1 < nch) stash(buf ^ 1);
+      __syncthreads();
+    }
+  }
+};
+
+// C <- min(C, A (x) B) from packed panels by one 256-thread workgroup; C
+// wholly inside the matrix.
+template <int BM, int BN, int KC>
+__device__ __forceinline__ void tile_update16_packed(TileSmem16<BM, BN, KC>& s, const PackedPanels& w,
+                                                     const View& C, int32_t depth) {
+  using PP = PackedProduct16<BM, BN, KC>;
+  uint32_t acc[PP::P::TM][PP::P::TP];
+  PP::P::load_c16(C, acc);
+  PP::run(s, w, depth, acc);
+  PP::P::store_c16(C, acc);
+}
+
+// ---------------------------------------------------------------------------
 // Closure of a T x T diagonal tile by min-plus squaring in the pk16 form.
 //
 // The tile sits in registers in close_regs's layout (thread (tx, ty) holds

@@ -112,6 +112,16 @@ int fwx_relaxation_form(const int32_t* d_D, int32_t n, int32_t* form, void* stre
 #define FWX_FORM_PK16 2
 #define FWX_PK16_CAP 15359
 int fwx_last_solve_form(int32_t* form, int32_t* exact_finish);
+/*
+ * *packed = 1 when the calling thread's last plain blocked solve ran its
+ * pk16 rest passes on packed stage panels (each stage's panels converted
+ * once to the 16-bit operand layout after its lead rounds, then copied into
+ * LDS by every rest block): direct launches of a pk16 solve with n % 128 ==
+ * 0, 16-deep chunks and a packing workspace (3 x 6 n kappa tile bytes of
+ * device memory, allocated on first need and kept); FWX_PACK16=0 turns it
+ * off.  0 otherwise.  No reference counterpart.
+ */
+int fwx_last_solve_packed(int32_t* packed);
 
 /* ---------------- device memory (reference GPUMemory, src/gpu_memory.cu) - */
 

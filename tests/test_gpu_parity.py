@@ -244,7 +244,8 @@ def test_n8192_all_kappas():
     """n=8192, density 0.9, U[1,1000], seed 42: multi_layer_tiling_gpu with
     the autotuned tile and kappa in {1,2,4,6,8}, tile 64 with the tuned
     kappa's neighbour, and tiling_gpu, bit-exact against the row-parallel
-    oracle (the restatement of fwBaselineCPU)."""
+    oracle (the restatement of fwBaselineCPU); the solves whose rest passes
+    are 128 wide run them on packed stage panels."""
     g = fx.generate_graph(8192, 0.9, 42, 1, 1000)
     say("n=8192: oracle start")
     expect = po.fw_rowpar(g)
@@ -258,6 +259,10 @@ def test_n8192_all_kappas():
             say("n=8192", algo, t, k)
             fx.run_device(algo, dm.ptr, 8192, t, k)
             assert (dm.download() == expect).all(), (algo, t, k)
+            # 128-wide rest passes (stage width a multiple of 128) run on
+            # packed stage panels (tiling_gpu's tuned tile is 128 here)
+            span = 128 if algo == "tiling_gpu" else t * k
+            assert fx.last_solve_packed() == (span % 128 == 0), (algo, t, k)
 
 
 @pytest.mark.slow

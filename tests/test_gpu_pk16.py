@@ -146,3 +146,52 @@ def test_diag_closure_by_squaring(monkeypatch, sq, tile, kappa):
         say("diag squaring", sq, tile, kappa, tag)
         got = fx.floyd_warshall(W, "multi_layer_tiling_gpu", tile, kappa)
         assert (got == po.fw_rowpar(W)).all(), (tag, sq, tile, kappa)
+
+
+PACK_GRAPHS = [
+    # (tag, W, packed expected, exact finish expected)
+    ("dense n=1024", lambda: graph(1024, 0.9, 1, 1000, 31), True, 0),
+    ("unreachable pairs n=1280", lambda: graph(1280, 0.002, 1, 1000, 32), True, 1),
+    ("partial last stage n=1152", lambda: graph(1152, 0.05, 1, 1000, 33), True, None),
+    ("ragged n=1000", lambda: graph(1000, 0.5, 1, 1000, 34), False, 0),
+]
+
+
+@pytest.fixture(scope="module")
+def pack_graphs():
+    out = []
+    for tag, make, packed, finish in PACK_GRAPHS:
+        W = make()
+        out.append((tag, W, po.fw_rowpar(W), packed, finish))
+    return out
+
+
+@pytest.mark.parametrize("la", ["0", "1", "2", "3"])
+@pytest.mark.parametrize("tile,kappa", [(64, 2), (64, 4), (128, 1), (128, 2), (32, 4)])
+def test_pk16_packed_panels(monkeypatch, pack_graphs, la, tile, kappa):
+    """Direct launches (FWX_GRAPH=0) of a pk16 solve with n % 128 == 0 run
+    the rest passes on packed stage panels (k_pack16 after each stage's lead
+    rounds, k_rest16p reading buffer stage % 3) in every schedule: bit-exact
+    against the oracle with and without the exact finish and with a partial
+    last stage; ragged n takes the int32 panels; four waves per SIMD
+    (FWX_REST16P_WAVES=4) and FWX_PACK16=0 give the same bits."""
+    monkeypatch.setenv("FWX_GRAPH", "0")
+    monkeypatch.setenv("FWX_LOOKAHEAD", la)
+    for tag, W, want, packed, finish in pack_graphs:
+        say("packed panels", la, tile, kappa, tag)
+        got = fx.floyd_warshall(W, "multi_layer_tiling_gpu", tile, kappa)
+        form, exact = fx.last_solve_form()
+        assert form == _lib.FORM_PK16, tag
+        assert fx.last_solve_packed() == packed, tag
+        if finish is not None:
+            assert exact == finish, tag
+        assert (got == want).all(), (tag, la, tile, kappa)
+    tag, W, want, _, _ = pack_graphs[1]
+    monkeypatch.setenv("FWX_REST16P_WAVES", "4")
+    got = fx.floyd_warshall(W, "multi_layer_tiling_gpu", tile, kappa)
+    assert fx.last_solve_packed()
+    assert (got == want).all(), (tag, "packed, four waves per SIMD")
+    monkeypatch.setenv("FWX_PACK16", "0")
+    got = fx.floyd_warshall(W, "multi_layer_tiling_gpu", tile, kappa)
+    assert not fx.last_solve_packed()
+    assert (got == want).all(), (tag, "unpacked")

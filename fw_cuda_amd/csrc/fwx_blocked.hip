@@ -945,33 +945,57 @@ static int run_blocked_sideband(int32_t* d, int32_t n, int32_t kappa, hipStream_
 // and writes band s+2 only; X(s) writes cells with a row or column in band
 // s+2 outside bands s and s+1, which the side stream touches only after
 // event X.  Ordering: X(s), Y(s) wait for lead(s) (event L); band(s+1) waits
-// for X(s) (event X).
+// for X(s) (event X).  Each stage enqueues X(s) before its side-stream work:
+// the host takes a few microseconds per launch, and with the side stream's
+// band and lead launches first, X(0) started only after lead(1) had run
+// (~160 us per solve at n=8192).
 template <int T, bool VEC, bool PK, bool P16>
 static int run_blocked_ahead(int32_t* d, int32_t n, int32_t kappa, hipStream_t st, hipStream_t side,
-                             hipEvent_t evX, hipEvent_t evL) {
+                             const hipEvent_t (&evX)[2], const hipEvent_t (&evL)[2]) {
   const int N = (n + T - 1) / T;
   const int span = kappa * T;
   const int R = (span % 128 == 0) ? 128 : (span % 64 == 0) ? 64 : 32;
   const double nn = (double)n;
   const int prio = lead_priority_enabled() ? 1 : 0;
-  FWX_HIP(hipEventRecord(evX, st));  // the side stream starts after the caller's prior work
-  FWX_HIP(hipStreamWaitEvent(side, evX, 0));
+  // Stage s's handoffs use evL[s & 1] (lead(s) done) and evX[s & 1] (X(s)
+  // done): each is recorded once per two stages, and every wait on it is
+  // enqueued before its next record.
+  FWX_HIP(hipEventRecord(evX[1], st));  // the side stream starts after the caller's prior work
+  FWX_HIP(hipStreamWaitEvent(side, evX[1], 0));
   if (int rc = lead_stage<T, VEC, PK, P16>(d, n, 0, min(kappa, N), N, prio, side)) return rc;
-  FWX_HIP(hipEventRecord(evL, side));
+  FWX_HIP(hipEventRecord(evL[0], side));
   hipEvent_t ev = nullptr;  // profiling markers of the caller's stream
-  for (int kb = 0; kb < N; kb += kappa) {
+  int si = 0;
+  for (int kb = 0; kb < N; kb += kappa, ++si) {
     const int B = min(kappa, N - kb);
     const int s0 = kb * T, S = B * T;
     const double L = (double)min(S, n - s0);
     const int en = (S + R - 1) / R;  // band s in R tiles
     const int kb2 = kb + kappa, kb3 = kb2 + kappa;
-    FWX_HIP(hipStreamWaitEvent(st, evL, 0));
+    FWX_HIP(hipStreamWaitEvent(st, evL[si & 1], 0));
     if (int rc = prof_open(st, &ev)) return rc;
     if (kb2 < N) {
-      // side: band(s) -- it waited for X(s-1) when that was enqueued -- and lead(s+1)
       const int B2 = min(kappa, N - kb2);
       const int fn = (B2 * T + R - 1) / R;
       const double L2 = (double)min(B2 * T, n - kb2 * T);
+      const double outside2 = nn - L - L2;  // rows outside bands s and s+1
+      int gn = 0;
+      double L3 = 0;
+      if (kb3 < N) {
+        // caller: X(s), the rest tiles with a row or column in band s+2,
+        // enqueued before this stage's side-stream work so that the host
+        // does not hold it back behind those launches
+        const int B3 = min(kappa, N - kb3);
+        gn = (B3 * T + R - 1) / R;
+        L3 = (double)min(B3 * T, n - kb3 * T);
+        if (int rc = band_dispatch<VEC, PK, P16>(R, d, n, s0, S, en + fn, gn, st)) return rc;
+        if (int rc = prof_chain(st, &ev, FWX_PROF_BAND, (outside2 * outside2 - (outside2 - L3) * (outside2 - L3)) * L))
+          return rc;
+        FWX_HIP(hipEventRecord(evX[si & 1], st));
+      }
+      // side: band(s) -- after X(s-1), which wrote the band s+1 cells it
+      // updates -- and lead(s+1)
+      if (si > 0) FWX_HIP(hipStreamWaitEvent(side, evX[(si - 1) & 1], 0));
       hipEvent_t es;
       if (int rc = prof_open(side, &es)) return rc;
       if (int rc = band_dispatch<VEC, PK, P16>(R, d, n, s0, S, en, fn, side)) return rc;
@@ -979,24 +1003,14 @@ static int run_blocked_ahead(int32_t* d, int32_t n, int32_t kappa, hipStream_t s
         return rc;
       if (int rc = lead_rounds<T, VEC, PK, P16>(d, n, kb2, B2, N, prio, side)) return rc;
       if (int rc = prof_close(side, es, FWX_PROF_LEAD, lead_work(n, T, kb2, B2))) return rc;
-      FWX_HIP(hipEventRecord(evL, side));
-      if (kb3 < N) {
-        // caller: X(s), the rest tiles with a row or column in band s+2, first
-        const int B3 = min(kappa, N - kb3);
-        const int gn = (B3 * T + R - 1) / R;
-        const double L3 = (double)min(B3 * T, n - kb3 * T);
-        const double outside2 = nn - L - L2;  // rows outside bands s and s+1
-        if (int rc = band_dispatch<VEC, PK, P16>(R, d, n, s0, S, en + fn, gn, st)) return rc;
-        if (int rc = prof_chain(st, &ev, FWX_PROF_BAND, (outside2 * outside2 - (outside2 - L3) * (outside2 - L3)) * L))
-          return rc;
-        FWX_HIP(hipEventRecord(evX, st));
-        // the side stream's band(s+1) needs X(s) only
-        FWX_HIP(hipStreamWaitEvent(side, evX, 0));
+      FWX_HIP(hipEventRecord(evL[(si + 1) & 1], side));
+      // caller: Y(s), every other rest tile
+      if (gn > 0) {
         if (int rc = rest_dispatch<VEC, PK, P16>(R, d, n, s0, S, en + fn + gn, st)) return rc;
         if (int rc = prof_chain(st, &ev, FWX_PROF_REST, (outside2 - L3) * (outside2 - L3) * L)) return rc;
       } else {
         if (int rc = rest_dispatch<VEC, PK, P16>(R, d, n, s0, S, en + fn, st)) return rc;
-        if (int rc = prof_chain(st, &ev, FWX_PROF_REST, (nn - L - L2) * (nn - L - L2) * L)) return rc;
+        if (int rc = prof_chain(st, &ev, FWX_PROF_REST, outside2 * outside2 * L)) return rc;
       }
     } else {
       if (int rc = rest_dispatch<VEC, PK, P16>(R, d, n, s0, S, en, st)) return rc;
@@ -1045,18 +1059,22 @@ static int run_any(int32_t* d, int32_t n, int32_t kappa, hipStream_t st) {
   if (mode == 0 || N <= kappa) return run_blocked<T, VEC, PK, P16>(d, n, kappa, st);
   hipStream_t side;
   if (int rc = side_stream(&side)) return rc;
-  hipEvent_t evA, evB;
-  FWX_HIP(hipEventCreateWithFlags(&evA, event_flags(false)));
-  hipError_t e = hipEventCreateWithFlags(&evB, event_flags(false));
-  if (e != hipSuccess) {
-    (void)hipEventDestroy(evA);
-    return hip_fail(e, "hipEventCreateWithFlags", __FILE__, __LINE__);
+  // ordering events of this call (evs[2], evs[3] for the side-band-ahead
+  // schedule's two-stage rings)
+  hipEvent_t evs[4] = {};
+  const int nev = mode == 3 ? 4 : 2;
+  for (int i = 0; i < nev; ++i) {
+    const hipError_t e = hipEventCreateWithFlags(&evs[i], event_flags(false));
+    if (e != hipSuccess) {
+      for (int k = 0; k < i; ++k) (void)hipEventDestroy(evs[k]);
+      return hip_fail(e, "hipEventCreateWithFlags", __FILE__, __LINE__);
+    }
   }
-  const int rc = mode == 3   ? run_blocked_ahead<T, VEC, PK, P16>(d, n, kappa, st, side, evA, evB)
-                 : mode == 2 ? run_blocked_sideband<T, VEC, PK, P16>(d, n, kappa, st, side, evA, evB)
-                             : run_blocked_lookahead<T, VEC, PK, P16>(d, n, kappa, st, side, evA, evB);
-  (void)hipEventDestroy(evA);
-  (void)hipEventDestroy(evB);
+  const hipEvent_t evX[2] = {evs[0], evs[1]}, evL[2] = {evs[2], evs[3]};
+  const int rc = mode == 3   ? run_blocked_ahead<T, VEC, PK, P16>(d, n, kappa, st, side, evX, evL)
+                 : mode == 2 ? run_blocked_sideband<T, VEC, PK, P16>(d, n, kappa, st, side, evs[0], evs[1])
+                             : run_blocked_lookahead<T, VEC, PK, P16>(d, n, kappa, st, side, evs[0], evs[1]);
+  for (int i = 0; i < nev; ++i) (void)hipEventDestroy(evs[i]);
   return rc;
 }
 

@@ -47,8 +47,8 @@ FORM_BASIS = {0: "1.5 VALU lane-ops per relaxation (x 2/3)", 1: "one VALU lane-o
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--n", type=int, default=8192)
     ap.add_argument("--algo", default="multi_layer_tiling_gpu")
     ap.add_argument("--tile", type=int, default=0)

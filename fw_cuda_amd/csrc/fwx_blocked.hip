@@ -276,6 +276,71 @@ __global__ __launch_bounds__(256) void k_cross16(int32_t* __restrict__ D, int32_
 }
 
 // ---------------------------------------------------------------------------
+// Block lead phase (pk16, 64-wide tiles): the stage's kappa lead rounds as
+// the three-phase algorithm with block size L = B*T.
+//
+// block_square: the L x L stage block closes by min-plus squaring, block <-
+// min(block, block (x) block), one launch per squaring: workgroup (a, b)
+// updates tile (kb+a, kb+b) in place with one product of depth L.  A tile
+// reads a mix of the other tiles' old and new values; every value stays a
+// walk length at least the closure's and at most the synchronous squaring's,
+// so log2(L) launches reach the closure, and a launch in which no cell
+// changed leaves the block obeying every triangle inequality: the closure.
+// A workgroup that changed a cell writes the stage's generation `gen` into
+// flags[it]; launch it returns at once unless flags[it-1] holds gen, so the
+// launches after convergence cost only their dispatch.  (A stale flag equal
+// to gen can only add a squaring; it cannot stop one early.)  In the pk16
+// form a cell whose result reaches the cap keeps its value, as in every pk16
+// product: the block is exact below the cap and the solve's final check
+// covers the rest.
+template <int T, bool VEC>
+__global__ __launch_bounds__(256) void k_block_square16(int32_t* __restrict__ D, int32_t n, int32_t kb, int32_t B,
+                                                        int32_t it, uint32_t* __restrict__ flags, uint32_t gen,
+                                                        int32_t prio) {
+  if (it > 0 && flags[it - 1] != gen) return;
+  if (prio) lead_wave_priority();
+  __shared__ TileSmem16<T, T, 16> s;
+  const int a = kb + (int)blockIdx.x / B, b = kb + (int)blockIdx.x % B;
+  const int32_t s0 = kb * T;
+  const View A = make_view(D, n, a * T, s0);
+  const View Bv = make_view(D, n, s0, b * T);
+  const View C = make_view(D, n, a * T, b * T);
+  if (tile_update16_changed<T, T, 16, VEC>(s, A, Bv, C, B * T) && threadIdx.x == 0) flags[it] = gen;
+}
+
+// block_panel: every row-panel tile (k, j) and column-panel tile (i, k) of
+// the stage (k in the stage, i, j outside) takes one product of depth L with
+// the closed block: D[k, j] <- min(D[k, j], D*[k, K] (x) D[K, j]) and
+// D[i, k] <- min(D[i, k], D[i, K] (x) D*[K, k]).  The other operand is the
+// panel being updated in the same launch; a mix of its old and new values
+// gives the same result (the new values lie between the final panel and the
+// old one, and the final panel is closed under the block).  Blocks
+// [0, B (N-B)) take the row tiles, the rest the column tiles.
+template <int T, bool VEC>
+__global__ __launch_bounds__(256) void k_block_panel16(int32_t* __restrict__ D, int32_t n, int32_t kb, int32_t B,
+                                                       int32_t N, int32_t prio) {
+  if (prio) lead_wave_priority();
+  __shared__ TileSmem16<T, T, 16> s;
+  const int r = B * (N - B);
+  int b = blockIdx.x, i, j;
+  if (b < r) {
+    i = kb + b / (N - B);
+    const int jj = b % (N - B);
+    j = jj + (jj >= kb ? B : 0);
+  } else {
+    b -= r;
+    const int ii = b / B;
+    i = ii + (ii >= kb ? B : 0);
+    j = kb + b % B;
+  }
+  const int32_t s0 = kb * T;
+  const View A = make_view(D, n, i * T, s0);
+  const View Bv = make_view(D, n, s0, j * T);
+  const View C = make_view(D, n, i * T, j * T);
+  tile_update16<T, T, 16, VEC>(s, A, Bv, C, B * T);
+}
+
+// ---------------------------------------------------------------------------
 // rest: R x R tiles outside the stage rows/columns, depth S = B*T.
 // The (m x m) reduced tile grid is split into 8 contiguous ranges of `per`
 // tiles, one per XCD (workgroups are dealt to the 8 XCDs round-robin), and
@@ -456,11 +521,15 @@ __global__ __launch_bounds__(256) void k_rest_band16(int32_t* __restrict__ D, in
 // whose rest pass has ended.  The other schedules order packs after older
 // rest passes too.  Workspaces are allocated on first need per device, kept
 // for reuse and never freed; a solve that cannot get one runs unpacked.
+constexpr int kMaxSquarings = 16;  // log2 of the widest stage block (16 x 128) is 11
+
 struct PackWs {
   int dev = -1;
   size_t cap_a = 0, cap_b = 0;  // words / slots per buffer
   uint32_t* wa[3] = {};
   uint64_t* wb[3] = {};
+  uint32_t* flags = nullptr;  // kMaxSquarings words: the block squarings' change flags
+  uint32_t gen = 0;           // generation of the last stage that used them
   bool busy = false;
   // the holding solve's configuration
   int32_t tile = 0, kappa = 0;
@@ -501,7 +570,7 @@ static PackWs* pack_acquire(int32_t n, int32_t tile, int32_t kappa) {
     }
   if (!w) {
     char* base = nullptr;
-    if (hipMalloc(reinterpret_cast<void**>(&base), 3 * (need_a * 4 + need_b * 8)) != hipSuccess) {
+    if (hipMalloc(reinterpret_cast<void**>(&base), 3 * (need_a * 4 + need_b * 8) + 256) != hipSuccess) {
       (void)hipGetLastError();
       return nullptr;
     }
@@ -513,6 +582,7 @@ static PackWs* pack_acquire(int32_t n, int32_t tile, int32_t kappa) {
       w->wb[i] = reinterpret_cast<uint64_t*>(base + i * need_b * 8);
       w->wa[i] = reinterpret_cast<uint32_t*>(base + 3 * need_b * 8 + i * need_a * 4);
     }
+    w->flags = reinterpret_cast<uint32_t*>(base + 3 * (need_a * 4 + need_b * 8));
     g_pack.push_back(w);
   }
   w->busy = true;
@@ -765,11 +835,44 @@ static int band_dispatch(int R, int32_t* d, int32_t n, int32_t s0, int32_t S, in
   return band_pass<32, VEC, PK>(d, n, s0, S, en, fn, st);
 }
 
+// FWX_LEAD_BLOCK=0 keeps the kappa lead rounds in packed pk16 solves.
+static bool lead_block_enabled() {
+  const char* e = std::getenv("FWX_LEAD_BLOCK");
+  return !(e && e[0] == '0');
+}
+
+// The block lead phase of one stage (k_block_square16, k_block_panel16):
+// ceil(log2(B T)) squaring launches, of which those after convergence
+// return at once, and one panel launch.  Replaces B lead rounds of (closure,
+// panels, cross): the same panel work as products of depth B T instead of
+// B products of depth T, in 1 + log2(B T) launches instead of 1 + 2 B.
+template <bool VEC>
+static int lead_block(int32_t* d, int32_t n, int32_t kb, int32_t B, int32_t N, int prio, hipStream_t st) {
+  PackWs* w = t_pack;
+  const uint32_t gen = ++w->gen;
+  int nsq = 0;
+  while ((1 << nsq) < B * 64) ++nsq;
+  if (nsq > kMaxSquarings) return fail(FWX_ERR_INVALID_ARG, "stage block too wide for the block lead phase");
+  for (int it = 0; it < nsq; ++it) {
+    k_block_square16<64, VEC><<<B * B, 256, 0, st>>>(d, n, kb, B, it, w->flags, gen, prio);
+    FWX_LAUNCHED("k_block_square16");
+  }
+  if (N > B) {
+    k_block_panel16<64, VEC><<<2 * B * (N - B), 256, 0, st>>>(d, n, kb, B, N, prio);
+    FWX_LAUNCHED("k_block_panel16");
+  }
+  return FWX_OK;
+}
+
 // The lead rounds of one stage.
 template <int T, bool VEC, bool PK, bool P16 = false>
 static int lead_rounds(int32_t* d, int32_t n, int32_t kb, int32_t B, int32_t N, int prio, hipStream_t st) {
-  for (int l = 0; l < B; ++l)
-    if (int rc = lead_round<T, VEC, PK, P16>(d, n, kb + l, kb, B, N, prio, st)) return rc;
+  if (P16 && T == 64 && t_pack && B > 1 && lead_block_enabled()) {
+    if (int rc = lead_block<VEC>(d, n, kb, B, N, prio, st)) return rc;
+  } else {
+    for (int l = 0; l < B; ++l)
+      if (int rc = lead_round<T, VEC, PK, P16>(d, n, kb + l, kb, B, N, prio, st)) return rc;
+  }
   // the stage's panels are final: pack them for its rest pass
   if (P16 && t_pack) return pack_stage(d, n, kb, B * T, prio, st);
   return FWX_OK;

@@ -258,6 +258,41 @@ __device__ __forceinline__ void tile_update16(TileSmem16<BM, BN, KC>& s, const V
   }
 }
 
+// tile_update16 that also tells whether any cell of C changed (every thread
+// of the block gets the same answer; one extra barrier).
+template <int BM, int BN, int KC, bool VEC>
+__device__ __forceinline__ bool tile_update16_changed(TileSmem16<BM, BN, KC>& s, const View& A, const View& B,
+                                                      const View& C, int32_t depth) {
+  using PF = TileProduct16<BM, BN, KC, VEC, false>;
+  using PE = TileProduct16<BM, BN, KC, VEC, true>;
+  uint32_t acc[PF::TM][PF::TP], was[PF::TM][PF::TP];
+  const bool full = C.rlim >= BM && C.clim >= BN && A.rlim >= BM && A.clim >= depth && B.rlim >= depth &&
+                    B.clim >= BN;
+  if (full)
+    PF::load_c16(C, acc);
+  else
+    PE::load_c16(C, acc);
+  uint32_t diff = 0;
+#pragma unroll
+  for (int r = 0; r < PF::TM; ++r)
+#pragma unroll
+    for (int p = 0; p < PF::TP; ++p) was[r][p] = acc[r][p];
+  if (full) {
+    PF pf;
+    pf.run16(s, A, B, depth, acc);
+    PF::store_c16(C, acc);
+  } else {
+    PE pe;
+    pe.run16(s, A, B, depth, acc);
+    PE::store_c16(C, acc);
+  }
+#pragma unroll
+  for (int r = 0; r < PF::TM; ++r)
+#pragma unroll
+    for (int p = 0; p < PF::TP; ++p) diff |= acc[r][p] ^ was[r][p];
+  return __syncthreads_or(diff != 0) != 0;
+}
+
 // ---------------------------------------------------------------------------
 // Pre-packed operands.  A stage's panels, packed once (k_pack16 in
 // fwx_blocked.hip) in the exact byte layout of the LDS chunks above:

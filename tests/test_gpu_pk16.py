@@ -154,6 +154,7 @@ PACK_GRAPHS = [
     ("unreachable pairs n=1280", lambda: graph(1280, 0.002, 1, 1000, 32), True, 1),
     ("partial last stage n=1152", lambda: graph(1152, 0.05, 1, 1000, 33), True, None),
     ("ragged n=1000", lambda: graph(1000, 0.5, 1, 1000, 34), False, 0),
+    ("chain n=1024 (every block squaring)", lambda: chain(1024, 3000), True, 0),
 ]
 
 
@@ -173,8 +174,11 @@ def test_pk16_packed_panels(monkeypatch, pack_graphs, la, tile, kappa):
     the rest passes on packed stage panels (k_pack16 after each stage's lead
     rounds, k_rest16p reading buffer stage % 3) in every schedule: bit-exact
     against the oracle with and without the exact finish and with a partial
-    last stage; ragged n takes the int32 panels; four waves per SIMD
-    (FWX_REST16P_WAVES=4) and FWX_PACK16=0 give the same bits."""
+    last stage; ragged n takes the int32 panels; with 64-wide tiles and
+    kappa > 1 the lead phase runs as block squarings plus depth-L panels (a
+    chain needs every squaring); the kappa lead rounds (FWX_LEAD_BLOCK=0),
+    four waves per SIMD (FWX_REST16P_WAVES=4) and FWX_PACK16=0 give the same
+    bits."""
     monkeypatch.setenv("FWX_GRAPH", "0")
     monkeypatch.setenv("FWX_LOOKAHEAD", la)
     for tag, W, want, packed, finish in pack_graphs:
@@ -187,6 +191,11 @@ def test_pk16_packed_panels(monkeypatch, pack_graphs, la, tile, kappa):
             assert exact == finish, tag
         assert (got == want).all(), (tag, la, tile, kappa)
     tag, W, want, _, _ = pack_graphs[1]
+    monkeypatch.setenv("FWX_LEAD_BLOCK", "0")
+    got = fx.floyd_warshall(W, "multi_layer_tiling_gpu", tile, kappa)
+    assert fx.last_solve_packed()
+    assert (got == want).all(), (tag, "packed, lead rounds")
+    monkeypatch.delenv("FWX_LEAD_BLOCK")
     monkeypatch.setenv("FWX_REST16P_WAVES", "4")
     got = fx.floyd_warshall(W, "multi_layer_tiling_gpu", tile, kappa)
     assert fx.last_solve_packed()

@@ -118,8 +118,10 @@ int fwx_last_solve_form(int32_t* form, int32_t* exact_finish);
  * once to the 16-bit operand layout after its lead rounds, then copied into
  * LDS by every rest block): direct launches of a pk16 solve with n % 128 ==
  * 0, 16-deep chunks and a packing workspace (3 x 6 n kappa tile bytes of
- * device memory, allocated on first need and kept); FWX_PACK16=0 turns it
- * off.  0 otherwise.  No reference counterpart.
+ * device memory plus 256 bytes of flags, allocated on first need and
+ * kept); FWX_PACK16=0 turns it off.  Such solves on 64-wide tiles with
+ * kappa > 1 also run the block lead phase.  0 otherwise.  No reference
+ * counterpart.
  */
 int fwx_last_solve_packed(int32_t* packed);
 

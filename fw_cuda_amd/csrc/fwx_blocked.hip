@@ -340,6 +340,38 @@ __global__ __launch_bounds__(256) void k_block_panel16(int32_t* __restrict__ D, 
   tile_update16<T, T, 16, VEC>(s, A, Bv, C, B * T);
 }
 
+// block_panel on 64 x 128 tiles (two adjacent 64-wide panel tiles per
+// workgroup, 4 x 8 cells per thread: a quarter fewer LDS reads per
+// relaxation than 64 x 64).  Needs B and kb even, so that the paired columns
+// are adjacent: row-panel pairs (jj, jj + 1) of the columns outside the
+// stage, column-panel pairs of stage columns.  Blocks [0, B (N-B) / 2) take
+// the row pairs.
+template <bool VEC>
+__global__ __launch_bounds__(256) void k_block_panel16w(int32_t* __restrict__ D, int32_t n, int32_t kb, int32_t B,
+                                                        int32_t N, int32_t prio) {
+  if (prio) lead_wave_priority();
+  constexpr int T = 64;
+  __shared__ TileSmem16<T, 2 * T, 16> s;
+  const int h = (N - B) / 2;
+  const int r = B * h;
+  int b = blockIdx.x, i, j;
+  if (b < r) {
+    i = kb + b / h;
+    const int jj = 2 * (b % h);
+    j = jj + (jj >= kb ? B : 0);
+  } else {
+    b -= r;
+    const int ii = b / (B / 2);
+    i = ii + (ii >= kb ? B : 0);
+    j = kb + 2 * (b % (B / 2));
+  }
+  const int32_t s0 = kb * T;
+  const View A = make_view(D, n, i * T, s0);
+  const View Bv = make_view(D, n, s0, j * T);
+  const View C = make_view(D, n, i * T, j * T);
+  tile_update16<T, 2 * T, 16, VEC>(s, A, Bv, C, B * T);
+}
+
 // ---------------------------------------------------------------------------
 // rest: R x R tiles outside the stage rows/columns, depth S = B*T.
 // The (m x m) reduced tile grid is split into 8 contiguous ranges of `per`
@@ -508,8 +540,50 @@ __global__ __launch_bounds__(256) void k_rest_band16(int32_t* __restrict__ D, in
   tile_update16<R, R, 16, VEC>(s, A, B, C, S);
 }
 
+// rest_band16 on 64 x 128 tiles: two adjacent 64-wide band tiles per
+// workgroup (4 x 8 cells per thread).  Needs e0, en, f0, fn and m1 even, so
+// that the paired columns are adjacent.  Blocks [0, fn m1 / 2) take band-f
+// rows x column pairs outside band e; the others rows outside both bands x
+// band-f column pairs.
+template <bool VEC>
+__global__ __launch_bounds__(256) void k_rest_band16w(int32_t* __restrict__ D, int32_t n, int32_t s0, int32_t S,
+                                                      int32_t e0, int32_t en, int32_t f0, int32_t fn, int32_t m1) {
+  constexpr int R = 64;
+  __shared__ TileSmem16<R, 2 * R, 16> s;
+  int b = blockIdx.x;
+  int I, J;
+  const int r1 = fn * (m1 / 2);
+  if (b < r1) {
+    I = f0 + b / (m1 / 2);
+    const int Jr = 2 * (b % (m1 / 2));
+    J = Jr + (Jr >= e0 ? en : 0);
+  } else {
+    b -= r1;
+    const int Ir = b / (fn / 2);
+    I = Ir + (Ir >= e0 ? en + fn : 0);
+    J = f0 + 2 * (b % (fn / 2));
+  }
+  const View A = make_view(D, n, I * R, s0);
+  const View B = make_view(D, n, s0, J * R);
+  const View C = make_view(D, n, I * R, J * R);
+  tile_update16<R, 2 * R, 16, VEC>(s, A, B, C, S);
+}
+
 // ---------------------------------------------------------------------------
 // host driver
+
+// FWX_PANEL_WIDE=0 / FWX_BAND_WIDE=0 keep the block panels / the pk16 band
+// and X passes on 64 x 64 tiles instead of 64 x 128 (measured at n=8192 on
+// (64, 8): block panels on 64 x 128 tiles 9.47 -> 9.36 ms per step, then
+// band and X passes on 64 x 128 tiles 9.32 -> 9.25 ms).
+static bool panel_wide() {
+  const char* e = std::getenv("FWX_PANEL_WIDE");
+  return !(e && e[0] == '0');
+}
+static bool band_wide() {
+  const char* e = std::getenv("FWX_BAND_WIDE");
+  return !(e && e[0] == '0');
+}
 
 // Packed stage panels for the pk16 rest passes (k_pack16 -> k_rest16p).  A
 // solve holds one workspace of three panel buffers: stage s packs into buffer
@@ -763,7 +837,10 @@ static int band_pass(int32_t* d, int32_t n, int32_t s0, int32_t S, int32_t en, i
   if (fn <= 0 || m1 <= 0) return FWX_OK;
   const int cnt = fn * m1 + (m1 - fn) * fn;
   if (cnt <= 0) return FWX_OK;
-  if (P16 && R == 64 && S % 16 == 0 && band16_enabled())
+  if (P16 && R == 64 && S % 16 == 0 && band16_enabled() && band_wide() && e0 % 2 == 0 && en % 2 == 0 &&
+      f0 % 2 == 0 && fn % 2 == 0 && m1 % 2 == 0)
+    k_rest_band16w<VEC><<<cnt / 2, 256, 0, st>>>(d, n, s0, S, e0, en, f0, fn, m1);
+  else if (P16 && R == 64 && S % 16 == 0 && band16_enabled())
     k_rest_band16<64, VEC><<<cnt, 256, 0, st>>>(d, n, s0, S, e0, en, f0, fn, m1);
   else
     k_rest_band<R, VEC, PK><<<cnt, 256, 0, st>>>(d, n, s0, S, e0, en, f0, fn, m1);
@@ -858,8 +935,13 @@ static int lead_block(int32_t* d, int32_t n, int32_t kb, int32_t B, int32_t N, i
     FWX_LAUNCHED("k_block_square16");
   }
   if (N > B) {
-    k_block_panel16<64, VEC><<<2 * B * (N - B), 256, 0, st>>>(d, n, kb, B, N, prio);
-    FWX_LAUNCHED("k_block_panel16");
+    if (panel_wide() && B % 2 == 0 && kb % 2 == 0 && (N - B) % 2 == 0) {
+      k_block_panel16w<VEC><<<B * (N - B), 256, 0, st>>>(d, n, kb, B, N, prio);
+      FWX_LAUNCHED("k_block_panel16w");
+    } else {
+      k_block_panel16<64, VEC><<<2 * B * (N - B), 256, 0, st>>>(d, n, kb, B, N, prio);
+      FWX_LAUNCHED("k_block_panel16");
+    }
   }
   return FWX_OK;
 }

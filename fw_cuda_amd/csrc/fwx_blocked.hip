@@ -547,7 +547,9 @@ __global__ __launch_bounds__(256) void k_rest_band16(int32_t* __restrict__ D, in
 // band-f column pairs.
 template <bool VEC>
 __global__ __launch_bounds__(256) void k_rest_band16w(int32_t* __restrict__ D, int32_t n, int32_t s0, int32_t S,
-                                                      int32_t e0, int32_t en, int32_t f0, int32_t fn, int32_t m1) {
+                                                      int32_t e0, int32_t en, int32_t f0, int32_t fn, int32_t m1,
+                                                      int32_t prio) {
+  if (prio) lead_wave_priority();
   constexpr int R = 64;
   __shared__ TileSmem16<R, 2 * R, 16> s;
   int b = blockIdx.x;
@@ -828,7 +830,7 @@ static bool band16_enabled() {
 
 template <int R, bool VEC, bool PK, bool P16 = false>
 static int band_pass(int32_t* d, int32_t n, int32_t s0, int32_t S, int32_t en, int32_t fn_tiles,
-                     hipStream_t st) {
+                     hipStream_t st, int band_prio = 0) {
   const int nr = (n + R - 1) / R;
   const int e0 = s0 / R;
   const int f0 = e0 + en;
@@ -839,7 +841,7 @@ static int band_pass(int32_t* d, int32_t n, int32_t s0, int32_t S, int32_t en, i
   if (cnt <= 0) return FWX_OK;
   if (P16 && R == 64 && S % 16 == 0 && band16_enabled() && band_wide() && e0 % 2 == 0 && en % 2 == 0 &&
       f0 % 2 == 0 && fn % 2 == 0 && m1 % 2 == 0)
-    k_rest_band16w<VEC><<<cnt / 2, 256, 0, st>>>(d, n, s0, S, e0, en, f0, fn, m1);
+    k_rest_band16w<VEC><<<cnt / 2, 256, 0, st>>>(d, n, s0, S, e0, en, f0, fn, m1, band_prio);
   else if (P16 && R == 64 && S % 16 == 0 && band16_enabled())
     k_rest_band16<64, VEC><<<cnt, 256, 0, st>>>(d, n, s0, S, e0, en, f0, fn, m1);
   else
@@ -903,9 +905,9 @@ static bool band_narrow() {
 
 template <bool VEC, bool PK, bool P16 = false>
 static int band_dispatch(int R, int32_t* d, int32_t n, int32_t s0, int32_t S, int32_t en, int32_t fn,
-                         hipStream_t st) {
+                         hipStream_t st, int band_prio = 0) {
   if (R == 128) {
-    if (band_narrow()) return band_pass<64, VEC, PK, P16>(d, n, s0, S, 2 * en, 2 * fn, st);
+    if (band_narrow()) return band_pass<64, VEC, PK, P16>(d, n, s0, S, 2 * en, 2 * fn, st, band_prio);
     return band_pass<128, VEC, PK>(d, n, s0, S, en, fn, st);
   }
   if (R == 64) return band_pass<64, VEC, PK>(d, n, s0, S, en, fn, st);
@@ -1112,6 +1114,18 @@ static int run_blocked_sideband(int32_t* d, int32_t n, int32_t kappa, hipStream_
   return FWX_OK;
 }
 
+// Raised wave priority for the 64 x 128 pk16 band kernel in the
+// side-band-ahead schedule (FWX_BAND_PRIO, default 1): bit 0 for the X(s)
+// passes on the caller's stream, bit 1 for the band passes on the side
+// stream.  X(s) shares the SIMDs with the side stream's lead kernels, which
+// raise theirs; at the same priority it finishes sooner and the caller's
+// stream -- the longer of the two -- waits less.  Measured at n=8192 on
+// (64, 8): 9.26 ms per step at 0, 9.04 at 1, 9.95 at 2, 9.82 at 3.
+static int band_prio_mode() {
+  const char* e = std::getenv("FWX_BAND_PRIO");
+  return e ? std::atoi(e) & 3 : 1;
+}
+
 // Side-band-ahead schedule (FWX_LOOKAHEAD=3): the side-band schedule with
 // the side stream one stage ahead.  Measured at n=8192 on (64, 4): 17.31-17.34
 // vs 17.39-17.40 ms per step for the side-band schedule -- the step is
@@ -1173,7 +1187,7 @@ static int run_blocked_ahead(int32_t* d, int32_t n, int32_t kappa, hipStream_t s
         const int B3 = min(kappa, N - kb3);
         gn = (B3 * T + R - 1) / R;
         L3 = (double)min(B3 * T, n - kb3 * T);
-        if (int rc = band_dispatch<VEC, PK, P16>(R, d, n, s0, S, en + fn, gn, st)) return rc;
+        if (int rc = band_dispatch<VEC, PK, P16>(R, d, n, s0, S, en + fn, gn, st, band_prio_mode() & 1)) return rc;
         if (int rc = prof_chain(st, &ev, FWX_PROF_BAND, (outside2 * outside2 - (outside2 - L3) * (outside2 - L3)) * L))
           return rc;
         FWX_HIP(hipEventRecord(evX[si & 1], st));
@@ -1183,7 +1197,7 @@ static int run_blocked_ahead(int32_t* d, int32_t n, int32_t kappa, hipStream_t s
       if (si > 0) FWX_HIP(hipStreamWaitEvent(side, evX[(si - 1) & 1], 0));
       hipEvent_t es;
       if (int rc = prof_open(side, &es)) return rc;
-      if (int rc = band_dispatch<VEC, PK, P16>(R, d, n, s0, S, en, fn, side)) return rc;
+      if (int rc = band_dispatch<VEC, PK, P16>(R, d, n, s0, S, en, fn, side, (band_prio_mode() >> 1) & 1)) return rc;
       if (int rc = prof_chain(side, &es, FWX_PROF_BAND, ((nn - L) * (nn - L) - (nn - L - L2) * (nn - L - L2)) * L))
         return rc;
       if (int rc = lead_rounds<T, VEC, PK, P16>(d, n, kb2, B2, N, prio, side)) return rc;
@@ -1394,7 +1408,8 @@ static int run_form(int32_t* d, int32_t n, int32_t tile, int32_t kappa, bool vec
   FWX_HIP(hipGetDevice(&dev));
   const int knobs = (fused_dpanel() ? 1 : 0) | (rest_variant(n) << 1) | (band_narrow() ? 8 : 0) | (diag_threads() << 4) |
                     (lead_priority_enabled() ? 1 << 20 : 0) | (p16_chunk(n) == 16 ? 1 << 21 : 0) |
-                    (diag_sq_enabled() ? 1 << 22 : 0) | (lead_close_enabled() ? 1 << 23 : 0);
+                    (diag_sq_enabled() ? 1 << 22 : 0) | (lead_close_enabled() ? 1 << 23 : 0) |
+                    (band_wide() ? 1 << 24 : 0) | (panel_wide() ? 1 << 25 : 0) | (band_prio_mode() << 26);
   const GraphKey key{d, n, tile, kappa, vec, form, small, small ? small_region_knob(n) : 0, lookahead_mode(n), knobs, st,
                      dev};
   std::lock_guard<std::mutex> lk(g_graph_mu);

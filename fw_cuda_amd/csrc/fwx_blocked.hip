@@ -276,6 +276,17 @@ __global__ __launch_bounds__(256) void k_cross16(int32_t* __restrict__ D, int32_
 }
 
 // ---------------------------------------------------------------------------
+// Range gate of the kernels of a packed pk16 solve (k_block_square16,
+// k_block_panel16[w], k_pack16, k_rest16p, k_rest_band16[w]).  Such a solve
+// enqueues its range check without waiting for it; `gate` is the word that
+// check ORs into, and a kernel returns at entry, writing nothing, when an
+// input cell lies outside [0, 2^30).  The host reads the word after the
+// solve and, if it is set, solves the untouched input in the exact form.
+// Null in every other solve.  The check completed before any gated kernel
+// starts (stream and event order), so the word is final when it is read.
+__device__ __forceinline__ bool gate_closed(const int* gate) { return gate != nullptr && *gate != 0; }
+
+// ---------------------------------------------------------------------------
 // Block lead phase (pk16, 64-wide tiles): the stage's kappa lead rounds as
 // the three-phase algorithm with block size L = B*T.
 //
@@ -296,8 +307,8 @@ __global__ __launch_bounds__(256) void k_cross16(int32_t* __restrict__ D, int32_
 template <int T, bool VEC>
 __global__ __launch_bounds__(256) void k_block_square16(int32_t* __restrict__ D, int32_t n, int32_t kb, int32_t B,
                                                         int32_t it, uint32_t* __restrict__ flags, uint32_t gen,
-                                                        int32_t prio) {
-  if (it > 0 && flags[it - 1] != gen) return;
+                                                        int32_t prio, const int* __restrict__ gate) {
+  if (gate_closed(gate) || (it > 0 && flags[it - 1] != gen)) return;
   if (prio) lead_wave_priority();
   __shared__ TileSmem16<T, T, 16> s;
   const int a = kb + (int)blockIdx.x / B, b = kb + (int)blockIdx.x % B;
@@ -318,7 +329,8 @@ __global__ __launch_bounds__(256) void k_block_square16(int32_t* __restrict__ D,
 // [0, B (N-B)) take the row tiles, the rest the column tiles.
 template <int T, bool VEC>
 __global__ __launch_bounds__(256) void k_block_panel16(int32_t* __restrict__ D, int32_t n, int32_t kb, int32_t B,
-                                                       int32_t N, int32_t prio) {
+                                                       int32_t N, int32_t prio, const int* __restrict__ gate) {
+  if (gate_closed(gate)) return;
   if (prio) lead_wave_priority();
   __shared__ TileSmem16<T, T, 16> s;
   const int r = B * (N - B);
@@ -348,7 +360,8 @@ __global__ __launch_bounds__(256) void k_block_panel16(int32_t* __restrict__ D, 
 // the row pairs.
 template <bool VEC>
 __global__ __launch_bounds__(256) void k_block_panel16w(int32_t* __restrict__ D, int32_t n, int32_t kb, int32_t B,
-                                                        int32_t N, int32_t prio) {
+                                                        int32_t N, int32_t prio, const int* __restrict__ gate) {
+  if (gate_closed(gate)) return;
   if (prio) lead_wave_priority();
   constexpr int T = 64;
   __shared__ TileSmem16<T, 2 * T, 16> s;
@@ -447,7 +460,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(KC == 16 ? 
 // precondition).
 __global__ __launch_bounds__(256) void k_pack16(const int32_t* __restrict__ D, int32_t n, int32_t s0, int32_t S,
                                                 uint32_t* __restrict__ wa, uint64_t* __restrict__ wb,
-                                                int32_t na, int32_t prio) {
+                                                int32_t na, int32_t prio, const int* __restrict__ gate) {
+  if (gate_closed(gate)) return;
   if (prio) lead_wave_priority();
   if ((int32_t)blockIdx.x < na) {
     const int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x;
@@ -476,10 +490,10 @@ __global__ __launch_bounds__(256) void k_pack16(const int32_t* __restrict__ D, i
 template <int R, int WAVES>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES, WAVES))) void k_rest16p(
     int32_t* __restrict__ D, int32_t n, const uint32_t* __restrict__ wa, const uint64_t* __restrict__ wb, int32_t S,
-    int32_t sr0, int32_t srn, int32_t m, int32_t per) {
+    int32_t sr0, int32_t srn, int32_t m, int32_t per, const int* __restrict__ gate) {
   __shared__ TileSmem16<R, R, 16> s;
   int I, J;
-  if (!rest_tile_of_block(m, per, sr0, srn, &I, &J)) return;
+  if (gate_closed(gate) || !rest_tile_of_block(m, per, sr0, srn, &I, &J)) return;
   const PackedPanels w{wa + (int64_t)I * R * S, S, wb + (int64_t)J * (R / 2), n / 2};
   const View C = make_view(D, n, I * R, J * R);
   tile_update16_packed<R, R, 16>(s, w, C, S);
@@ -520,7 +534,9 @@ __global__ __launch_bounds__(256) void k_rest_band(int32_t* __restrict__ D, int3
 // value; the solve's check afterwards finds it).
 template <int R, bool VEC>
 __global__ __launch_bounds__(256) void k_rest_band16(int32_t* __restrict__ D, int32_t n, int32_t s0, int32_t S,
-                                                     int32_t e0, int32_t en, int32_t f0, int32_t fn, int32_t m1) {
+                                                     int32_t e0, int32_t en, int32_t f0, int32_t fn, int32_t m1,
+                                                     const int* __restrict__ gate) {
+  if (gate_closed(gate)) return;
   __shared__ TileSmem16<R, R, 16> s;
   int b = blockIdx.x;
   int I, J;
@@ -548,7 +564,8 @@ __global__ __launch_bounds__(256) void k_rest_band16(int32_t* __restrict__ D, in
 template <bool VEC>
 __global__ __launch_bounds__(256) void k_rest_band16w(int32_t* __restrict__ D, int32_t n, int32_t s0, int32_t S,
                                                       int32_t e0, int32_t en, int32_t f0, int32_t fn, int32_t m1,
-                                                      int32_t prio) {
+                                                      int32_t prio, const int* __restrict__ gate) {
+  if (gate_closed(gate)) return;
   if (prio) lead_wave_priority();
   constexpr int R = 64;
   __shared__ TileSmem16<R, 2 * R, 16> s;
@@ -606,6 +623,7 @@ struct PackWs {
   uint64_t* wb[3] = {};
   uint32_t* flags = nullptr;  // kMaxSquarings words: the block squarings' change flags
   uint32_t gen = 0;           // generation of the last stage that used them
+  int* gate = nullptr;        // two words after the flags: the deferred range and cap checks
   bool busy = false;
   // the holding solve's configuration
   int32_t tile = 0, kappa = 0;
@@ -614,6 +632,17 @@ struct PackWs {
 static std::mutex g_pack_mu;
 static std::vector<PackWs*> g_pack;
 static thread_local PackWs* t_pack = nullptr;
+// The range gate of the solve being enqueued (its workspace's gate word), or
+// null when the range check has already been read.
+static thread_local const int* t_gate = nullptr;
+
+// A launch site without the range gate, reached while one is set: the
+// eligibility test of the deferred check (range_deferrable) admitted a
+// configuration that leaves the gated kernels.  Refused rather than run on
+// an input that may be outside the form's range.
+static int ungated_launch(const char* what) {
+  return fail(FWX_ERR_UNSUPPORTED, std::string("internal: ") + what + " launched in a solve with a deferred range check");
+}
 
 // FWX_PACK16=0 runs the pk16 rest passes on the int32 panels (conversion in
 // every block).
@@ -659,6 +688,7 @@ static PackWs* pack_acquire(int32_t n, int32_t tile, int32_t kappa) {
       w->wa[i] = reinterpret_cast<uint32_t*>(base + 3 * need_b * 8 + i * need_a * 4);
     }
     w->flags = reinterpret_cast<uint32_t*>(base + 3 * (need_a * 4 + need_b * 8));
+    w->gate = reinterpret_cast<int*>(w->flags + kMaxSquarings);
     g_pack.push_back(w);
   }
   w->busy = true;
@@ -680,7 +710,7 @@ static int pack_stage(int32_t* d, int32_t n, int32_t kb, int32_t S, int prio, hi
   const int32_t s0 = kb * w->tile;
   const int64_t ga = (int64_t)n * (S / 4), gb = (int64_t)(S / 2) * (n / 4);
   const int na = (int)((ga + 255) / 256), nb = (int)((gb + 255) / 256);
-  k_pack16<<<na + nb, 256, 0, st>>>(d, n, s0, S, w->wa[b], w->wb[b], na, prio);
+  k_pack16<<<na + nb, 256, 0, st>>>(d, n, s0, S, w->wa[b], w->wb[b], na, prio, t_gate);
   FWX_LAUNCHED("k_pack16");
   return FWX_OK;
 }
@@ -750,6 +780,7 @@ static bool lead_close_enabled() {
 template <int T, bool VEC, bool PK, bool P16 = false>
 static int lead_round(int32_t* d, int32_t n, int32_t c, int32_t kb, int32_t B, int32_t N, int prio,
                       hipStream_t st) {
+  if (t_gate) return ungated_launch("lead round");
   if (P16 && T == 64 && lead16_enabled()) {
     // with B > 1 every round but the stage's last has a cross launch, which
     // then closes the next round's diagonal tile
@@ -804,6 +835,7 @@ static int rest_variant(int32_t n) {
 // s alone (plain schedule) or stage s and the next stage (look-ahead).
 template <int R, bool VEC, bool PK>
 static int rest_pass(int32_t* d, int32_t n, int32_t s0, int32_t S, int32_t excl, hipStream_t st) {
+  if (t_gate) return ungated_launch("k_rest");
   const int nr = (n + R - 1) / R;
   const int sr0 = s0 / R;
   const int srn = min(excl, nr - sr0);
@@ -841,9 +873,11 @@ static int band_pass(int32_t* d, int32_t n, int32_t s0, int32_t S, int32_t en, i
   if (cnt <= 0) return FWX_OK;
   if (P16 && R == 64 && S % 16 == 0 && band16_enabled() && band_wide() && e0 % 2 == 0 && en % 2 == 0 &&
       f0 % 2 == 0 && fn % 2 == 0 && m1 % 2 == 0)
-    k_rest_band16w<VEC><<<cnt / 2, 256, 0, st>>>(d, n, s0, S, e0, en, f0, fn, m1, band_prio);
+    k_rest_band16w<VEC><<<cnt / 2, 256, 0, st>>>(d, n, s0, S, e0, en, f0, fn, m1, band_prio, t_gate);
   else if (P16 && R == 64 && S % 16 == 0 && band16_enabled())
-    k_rest_band16<64, VEC><<<cnt, 256, 0, st>>>(d, n, s0, S, e0, en, f0, fn, m1);
+    k_rest_band16<64, VEC><<<cnt, 256, 0, st>>>(d, n, s0, S, e0, en, f0, fn, m1, t_gate);
+  else if (t_gate)
+    return ungated_launch("k_rest_band");
   else
     k_rest_band<R, VEC, PK><<<cnt, 256, 0, st>>>(d, n, s0, S, e0, en, f0, fn, m1);
   FWX_LAUNCHED("k_rest_band");
@@ -872,12 +906,13 @@ static int rest_pass16(int32_t* d, int32_t n, int32_t s0, int32_t S, int32_t exc
     // this stage's packed panels (pack_stage ran after its lead rounds)
     const int b = (s0 / (t_pack->tile * t_pack->kappa)) % 3;
     if (rest16p_waves() == 5)
-      k_rest16p<128, 5><<<8 * per, 256, 0, st>>>(d, n, t_pack->wa[b], t_pack->wb[b], S, sr0, srn, m, per);
+      k_rest16p<128, 5><<<8 * per, 256, 0, st>>>(d, n, t_pack->wa[b], t_pack->wb[b], S, sr0, srn, m, per, t_gate);
     else
-      k_rest16p<128, 4><<<8 * per, 256, 0, st>>>(d, n, t_pack->wa[b], t_pack->wb[b], S, sr0, srn, m, per);
+      k_rest16p<128, 4><<<8 * per, 256, 0, st>>>(d, n, t_pack->wa[b], t_pack->wb[b], S, sr0, srn, m, per, t_gate);
     FWX_LAUNCHED("k_rest16p");
     return FWX_OK;
   }
+  if (t_gate) return ungated_launch("k_rest16");
   if (p16_chunk(n) == 16)
     k_rest16<R, VEC, 16><<<8 * per, 256, 0, st>>>(d, n, s0, S, sr0, srn, m, per);
   else
@@ -933,15 +968,15 @@ static int lead_block(int32_t* d, int32_t n, int32_t kb, int32_t B, int32_t N, i
   while ((1 << nsq) < B * 64) ++nsq;
   if (nsq > kMaxSquarings) return fail(FWX_ERR_INVALID_ARG, "stage block too wide for the block lead phase");
   for (int it = 0; it < nsq; ++it) {
-    k_block_square16<64, VEC><<<B * B, 256, 0, st>>>(d, n, kb, B, it, w->flags, gen, prio);
+    k_block_square16<64, VEC><<<B * B, 256, 0, st>>>(d, n, kb, B, it, w->flags, gen, prio, t_gate);
     FWX_LAUNCHED("k_block_square16");
   }
   if (N > B) {
     if (panel_wide() && B % 2 == 0 && kb % 2 == 0 && (N - B) % 2 == 0) {
-      k_block_panel16w<VEC><<<B * (N - B), 256, 0, st>>>(d, n, kb, B, N, prio);
+      k_block_panel16w<VEC><<<B * (N - B), 256, 0, st>>>(d, n, kb, B, N, prio, t_gate);
       FWX_LAUNCHED("k_block_panel16w");
     } else {
-      k_block_panel16<64, VEC><<<2 * B * (N - B), 256, 0, st>>>(d, n, kb, B, N, prio);
+      k_block_panel16<64, VEC><<<2 * B * (N - B), 256, 0, st>>>(d, n, kb, B, N, prio, t_gate);
       FWX_LAUNCHED("k_block_panel16");
     }
   }
@@ -1480,6 +1515,61 @@ void last_solve_form(int32_t* form, int32_t* exact_finish) {
   *exact_finish = t_last_exact_finish;
 }
 
+// FWX_RANGE_DEFER=0 makes every packed solve wait for its range check
+// before enqueuing its kernels.
+static bool range_defer_enabled() {
+  const char* e = std::getenv("FWX_RANGE_DEFER");
+  return !(e && e[0] == '0');
+}
+
+// A packed pk16 solve whose every launch goes to a gated kernel: 64-wide
+// tiles with kappa > 1 and no one-tile last stage (every stage takes the
+// block lead phase, never the lead rounds), the band in pk16 on 64-wide
+// tiles, the rest passes on packed panels (128-wide, which the packed solve
+// implies).  Such a solve enqueues its range check and its kernels without
+// waiting (solve_deferred).
+static bool range_deferrable(int32_t n, int32_t tile, int32_t kappa) {
+  const int32_t N = n / 64;
+  return range_defer_enabled() && tile == 64 && kappa > 1 && N % kappa != 1 && lead_block_enabled() &&
+         band_narrow() && band16_enabled();
+}
+
+// A packed pk16 solve with its range check deferred.  The check ORs into
+// ws->gate[0] on st ahead of the solve's kernels, which return at entry when
+// it is set; after the solve one read-back brings it with the cap check
+// (ws->gate[1]).  Outside the range, d still holds the input and is solved
+// in the min3 form; with a cell at the cap, the exact pk64 finish follows,
+// as in the waiting path.  The workspace is held until the read-back.
+static int solve_deferred(int32_t* d, int32_t n, int32_t tile, int32_t kappa, bool vec, PackWs* ws,
+                          hipStream_t st) {
+  const int64_t count = (int64_t)n * n;
+  int rc = range_check_enqueue(d, count, ws->gate, st);
+  if (!rc) {
+    t_pack = ws;
+    t_gate = ws->gate;
+    rc = run_form(d, n, tile, kappa, vec, 2, false, true, st);
+    t_pack = nullptr;
+    t_gate = nullptr;
+  }
+  int h[2] = {1, 1};
+  if (!rc) rc = deferred_checks_read(d, count, (int32_t)kP16Cap, ws->gate, st, h);
+  if (rc) {
+    (void)hipStreamSynchronize(st);
+    pack_release(ws);
+    return rc;
+  }
+  pack_release(ws);
+  if (h[0] != 0) {
+    t_last_form = 0;
+    return run_form(d, n, tile, kappa, vec, 0, false, true, st);
+  }
+  t_last_form = 2;
+  t_last_packed = 1;
+  if (h[1] == 0) return FWX_OK;
+  t_last_exact_finish = 1;
+  return run_form(d, n, tile, kappa, vec, 1, false, true, st);
+}
+
 int enqueue_blocked(int32_t* d, int32_t n, int32_t tile, int32_t kappa, hipStream_t st) {
   const bool vec = (n % 4 == 0) && ((reinterpret_cast<uintptr_t>(d) & 15u) == 0);
   const bool small = small_path(n, tile, kappa);
@@ -1488,16 +1578,19 @@ int enqueue_blocked(int32_t* d, int32_t n, int32_t tile, int32_t kappa, hipStrea
   t_last_exact_finish = 0;
   t_last_packed = 0;
   if (small) return run_form(d, n, tile, kappa, vec, 0, true, direct || !small_graphs_enabled(), st);
+  // Direct pk16 solves of whole 128-wide tiles with 16-deep chunks run their
+  // rest passes on packed stage panels; the workspace is held until the
+  // check after the solve has synchronised the stream.
+  const bool packable = pk16_enabled() && rest_width(tile, kappa) == 128 && direct && vec && n % 128 == 0 &&
+                        p16_chunk(n) == 16 && pack16_enabled();
+  if (packable && range_deferrable(n, tile, kappa))
+    if (PackWs* ws = pack_acquire(n, tile, kappa)) return solve_deferred(d, n, tile, kappa, vec, ws, st);
   bool pk = false;
   if (int rc = values_in_pk64_range(d, (int64_t)n * n, st, &pk)) return rc;
   const int form = !pk ? 0 : (pk16_enabled() && rest_width(tile, kappa) == 128) ? 2 : 1;
   t_last_form = form;
-  // Direct pk16 solves of whole 128-wide tiles with 16-deep chunks run their
-  // rest passes on packed stage panels; the workspace is held until
-  // values_below below has synchronised the stream.
   PackWs* ws = nullptr;
-  if (form == 2 && direct && vec && n % 128 == 0 && p16_chunk(n) == 16 && pack16_enabled())
-    ws = pack_acquire(n, tile, kappa);
+  if (form == 2 && packable) ws = pack_acquire(n, tile, kappa);
   t_pack = ws;
   t_last_packed = ws ? 1 : 0;
   int rc = run_form(d, n, tile, kappa, vec, form, false, direct, st);

@@ -9,6 +9,11 @@
 // one of its 64 slots from the zero fill until its result is back on the
 // host; a 65th concurrent call waits for a slot to be released, so no two
 // calls in flight ever share one.
+//
+// A solve on packed stage panels does not wait for the pass: it enqueues it
+// into a word of its own workspace (range_check_enqueue), every kernel of
+// that solve returns at entry when the word is set, and one read-back after
+// the solve brings the word back with the cap check's (deferred_checks_read).
 #include <hip/hip_runtime.h>
 
 #include <condition_variable>
@@ -177,6 +182,34 @@ __global__ __launch_bounds__(256) void k_at_least(const int32_t* __restrict__ d,
 }
 
 }  // namespace
+
+namespace {
+
+__global__ void k_flags_zero2(int* f) { f[threadIdx.x] = 0; }
+
+}  // namespace
+
+int range_check_enqueue(const int32_t* d, int64_t count, int* flags, hipStream_t st) {
+  const int64_t work = (count + 3) / 4;
+  const int blocks = (int)std::min<int64_t>((work + 255) / 256, 4096);
+  const int vec4 = (reinterpret_cast<uintptr_t>(d) & 15u) == 0;
+  k_flags_zero2<<<1, 2, 0, st>>>(flags);
+  FWX_LAUNCHED("k_flags_zero2");
+  k_range<<<blocks > 0 ? blocks : 1, 256, 0, st>>>(d, count, vec4, flags);
+  FWX_LAUNCHED("k_range");
+  return FWX_OK;
+}
+
+int deferred_checks_read(const int32_t* d, int64_t count, int32_t cap, int* flags, hipStream_t st, int out[2]) {
+  const int64_t work = (count + 3) / 4;
+  const int blocks = (int)std::min<int64_t>((work + 255) / 256, 4096);
+  const int vec4 = (reinterpret_cast<uintptr_t>(d) & 15u) == 0;
+  k_at_least<<<blocks > 0 ? blocks : 1, 256, 0, st>>>(d, count, vec4, cap, flags + 1);
+  FWX_LAUNCHED("k_at_least");
+  hipError_t e = hipMemcpyAsync(out, flags, 2 * sizeof(int), hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  return e == hipSuccess ? FWX_OK : hip_fail(e, "flag read-back", __FILE__, __LINE__);
+}
 
 int values_below(const int32_t* d, int64_t count, int32_t cap, hipStream_t st, bool* ok) {
   const int64_t work = (count + 3) / 4;

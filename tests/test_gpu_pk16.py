@@ -158,6 +158,52 @@ PACK_GRAPHS = [
 ]
 
 
+def negative_matching(n, seed):
+    """Weights in [2100, 3000] plus 32 edges of weight -2000 from the first
+    half of the vertices to the second: no vertex has a negative edge both in
+    and out, so every cycle pairs each negative edge with a following edge
+    of weight >= 2100 and no cycle is negative.  -2000 lies below the pk16
+    form's bias, so a pk16 kernel that ran on it would produce wrong bits."""
+    rng = np.random.default_rng(seed)
+    W = graph(n, 0.3, 2100, 3000, seed)
+    src = rng.choice(n // 2, 32, replace=False)
+    dst = n // 2 + rng.choice(n // 2, 32, replace=False)
+    W[src, dst] = -2000
+    return W
+
+
+@pytest.mark.parametrize("la", ["0", "1", "2", "3"])
+@pytest.mark.parametrize("kappa", [2, 4])
+def test_deferred_range_check(monkeypatch, la, kappa):
+    """Packed solves on 64-wide tiles with kappa > 1 enqueue the range check
+    and every kernel without waiting for it; the kernels return at entry
+    when the check fails, and the host, reading the check after the solve,
+    solves the untouched input in the min3 form.  In range: pk16 on packed
+    panels.  Out of range (negative weights, operands >= 2^30): min3, not
+    packed.  FWX_RANGE_DEFER=0 waits for the check first.  Each bit-exact
+    against the oracle, in every schedule."""
+    monkeypatch.setenv("FWX_GRAPH", "0")
+    monkeypatch.setenv("FWX_LOOKAHEAD", la)
+    n = 1024
+    rng = np.random.default_rng(45)
+    big = graph(n, 0.3, 1, 1000, 44)
+    big[rng.integers(0, n, 40), rng.integers(0, n, 40)] = 1 << 30
+    np.fill_diagonal(big, 0)
+    cases = [
+        ("in range", graph(n, 0.5, 1, 1000, 43), _lib.FORM_PK16, True),
+        ("negative weights", negative_matching(n, 46), _lib.FORM_MIN3, False),
+        ("operands >= 2^30", big, _lib.FORM_MIN3, False),
+    ]
+    for defer in ("1", "0"):
+        monkeypatch.setenv("FWX_RANGE_DEFER", defer)
+        for tag, W, form, packed in cases:
+            say("deferred range check", la, kappa, defer, tag)
+            got = fx.floyd_warshall(W, "multi_layer_tiling_gpu", 64, kappa)
+            assert fx.last_solve_form()[0] == form, (tag, defer)
+            assert fx.last_solve_packed() == packed, (tag, defer)
+            assert (got == po.fw_rowpar(W)).all(), (tag, la, kappa, defer)
+
+
 @pytest.fixture(scope="module")
 def pack_graphs():
     out = []

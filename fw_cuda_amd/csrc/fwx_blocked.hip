@@ -319,6 +319,34 @@ __global__ __launch_bounds__(256) void k_block_square16(int32_t* __restrict__ D,
   if (tile_update16_changed<T, T, 16, VEC>(s, A, Bv, C, B * T) && threadIdx.x == 0) flags[it] = gen;
 }
 
+// block_square with each tile's depth split over KS workgroups (grid
+// B B KS, a tile's parts in adjacent workgroups): part q takes k in
+// [q L / KS, (q + 1) L / KS) and lowers the tile's changed cells by atomic
+// min (tile_update16_part).  Every value written is still a walk length at
+// least the closure's, and after the launch each cell is at most the
+// minimum over all k of the values read, hence at most the synchronous
+// squaring's: the launch count and the early exit of k_block_square16 hold
+// unchanged.  KS times the workgroups of depth L / KS, for a stage block
+// whose lead phase has the GPU to itself (the first stage).  T = 64, full
+// tiles (n % 64 == 0).
+template <int KS, bool VEC>
+__global__ __launch_bounds__(256) void k_block_square16s(int32_t* __restrict__ D, int32_t n, int32_t kb, int32_t B,
+                                                         int32_t it, uint32_t* __restrict__ flags, uint32_t gen,
+                                                         int32_t prio, const int* __restrict__ gate) {
+  if (gate_closed(gate) || (it > 0 && flags[it - 1] != gen)) return;
+  if (prio) lead_wave_priority();
+  constexpr int T = 64;
+  __shared__ TileSmem16<T, T, 16> s;
+  const int t = (int)blockIdx.x / KS, q = (int)blockIdx.x % KS;
+  const int a = kb + t / B, b = kb + t % B;
+  const int32_t part = B * T / KS;
+  const int32_t k0 = kb * T + q * part;
+  const View A = make_view(D, n, a * T, k0);
+  const View Bv = make_view(D, n, k0, b * T);
+  const View C = make_view(D, n, a * T, b * T);
+  if (tile_update16_part<T, T, 16, VEC>(s, A, Bv, C, part) && threadIdx.x == 0) flags[it] = gen;
+}
+
 // block_panel: every row-panel tile (k, j) and column-panel tile (i, k) of
 // the stage (k in the stage, i, j outside) takes one product of depth L with
 // the closed block: D[k, j] <- min(D[k, j], D*[k, K] (x) D[K, j]) and
@@ -960,6 +988,19 @@ static bool lead_block_enabled() {
 // return at once, and one panel launch.  Replaces B lead rounds of (closure,
 // panels, cross): the same panel work as products of depth B T instead of
 // B products of depth T, in 1 + log2(B T) launches instead of 1 + 2 B.
+// Depth parts per tile of the first stage's block squarings
+// (k_block_square16s): FWX_SQ_SPLIT = 4 (default) or 8; 0 or 1 keeps one
+// workgroup per tile.  The first stage's lead phase runs with nothing beside
+// it, on B^2 = 64 workgroups at (64, 8).  Measured at n=8192 on (64, 8):
+// 9.045-9.077 ms per step unsplit, 9.012-9.014 with 4 parts, 9.029-9.032
+// with 8; splitting every stage's squarings (which run beside the rest
+// passes) 9.00 against 8.94-8.97 for the first stage alone.
+static int sq_split() {
+  const char* e = std::getenv("FWX_SQ_SPLIT");
+  const int v = e ? std::atoi(e) : 4;
+  return (v == 4 || v == 8) ? v : 0;
+}
+
 template <bool VEC>
 static int lead_block(int32_t* d, int32_t n, int32_t kb, int32_t B, int32_t N, int prio, hipStream_t st) {
   PackWs* w = t_pack;
@@ -967,8 +1008,16 @@ static int lead_block(int32_t* d, int32_t n, int32_t kb, int32_t B, int32_t N, i
   int nsq = 0;
   while ((1 << nsq) < B * 64) ++nsq;
   if (nsq > kMaxSquarings) return fail(FWX_ERR_INVALID_ARG, "stage block too wide for the block lead phase");
+  // parts of depth B 64 / ks, a multiple of the 16-deep chunk
+  const int ks = kb == 0 && n % 64 == 0 ? sq_split() : 0;
+  const bool split = ks > 0 && (B * 64) % (16 * ks) == 0;
   for (int it = 0; it < nsq; ++it) {
-    k_block_square16<64, VEC><<<B * B, 256, 0, st>>>(d, n, kb, B, it, w->flags, gen, prio, t_gate);
+    if (split && ks == 8)
+      k_block_square16s<8, VEC><<<B * B * 8, 256, 0, st>>>(d, n, kb, B, it, w->flags, gen, prio, t_gate);
+    else if (split)
+      k_block_square16s<4, VEC><<<B * B * 4, 256, 0, st>>>(d, n, kb, B, it, w->flags, gen, prio, t_gate);
+    else
+      k_block_square16<64, VEC><<<B * B, 256, 0, st>>>(d, n, kb, B, it, w->flags, gen, prio, t_gate);
     FWX_LAUNCHED("k_block_square16");
   }
   if (N > B) {

@@ -234,6 +234,30 @@ struct TileProduct16 : TileProduct<BM, BN, KC, VEC, 16, true, EDGE> {
         }
       }
   }
+
+  // store_c16 for a tile that other workgroups update in the same launch:
+  // every cell whose result is below CAP and differs from the value `was`
+  // it was loaded with is lowered by an agent-scope atomic min, so no
+  // workgroup's result overwrites a lower one.  Full tiles only.
+  __device__ __forceinline__ static void min_c16(const View& C, const uint32_t (&acc)[TM][TP],
+                                                 const uint32_t (&was)[TM][TP]) {
+    static_assert(!EDGE, "min_c16: full tiles only");
+    const int tid = threadIdx.x;
+    const int tx = tid % 16;
+    const int ty = tid / 16;
+#pragma unroll
+    for (int r = 0; r < TM; ++r)
+#pragma unroll
+      for (int p = 0; p < TP; ++p) {
+        const int i = ty + 16 * r, j = 4 * (tx + 16 * (p / 2)) + 2 * (p % 2);
+        const uint32_t v = acc[r][p] - kP16Bias2, w = acc[r][p] ^ was[r][p];
+        const int32_t lo = (int32_t)(v & 0xFFFFu), hi = (int32_t)(v >> 16);
+        if ((w & 0xFFFFu) != 0u && lo < (int32_t)kP16Cap)
+          __hip_atomic_fetch_min(C.at(i, j), lo, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if ((w >> 16) != 0u && hi < (int32_t)kP16Cap)
+          __hip_atomic_fetch_min(C.at(i, j + 1), hi, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+  }
 };
 
 // C <- min(C, A (x) B) in the pk16 form by one 256-thread workgroup; blocks
@@ -290,6 +314,30 @@ __device__ __forceinline__ bool tile_update16_changed(TileSmem16<BM, BN, KC>& s,
   for (int r = 0; r < PF::TM; ++r)
 #pragma unroll
     for (int p = 0; p < PF::TP; ++p) diff |= acc[r][p] ^ was[r][p];
+  return __syncthreads_or(diff != 0) != 0;
+}
+
+// tile_update16_changed for one of several workgroups that update the same
+// full C tile over disjoint parts of the depth in one launch: changed cells
+// are lowered by atomic min (min_c16), so C ends at most min over the parts.
+template <int BM, int BN, int KC, bool VEC>
+__device__ __forceinline__ bool tile_update16_part(TileSmem16<BM, BN, KC>& s, const View& A, const View& B,
+                                                   const View& C, int32_t depth) {
+  using PF = TileProduct16<BM, BN, KC, VEC, false>;
+  uint32_t acc[PF::TM][PF::TP], was[PF::TM][PF::TP];
+  PF::load_c16(C, acc);
+#pragma unroll
+  for (int r = 0; r < PF::TM; ++r)
+#pragma unroll
+    for (int p = 0; p < PF::TP; ++p) was[r][p] = acc[r][p];
+  PF pf;
+  pf.run16(s, A, B, depth, acc);
+  uint32_t diff = 0;
+#pragma unroll
+  for (int r = 0; r < PF::TM; ++r)
+#pragma unroll
+    for (int p = 0; p < PF::TP; ++p) diff |= acc[r][p] ^ was[r][p];
+  PF::min_c16(C, acc, was);
   return __syncthreads_or(diff != 0) != 0;
 }
 

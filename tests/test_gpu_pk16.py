@@ -222,9 +222,10 @@ def test_pk16_packed_panels(monkeypatch, pack_graphs, la, tile, kappa):
     against the oracle with and without the exact finish and with a partial
     last stage; ragged n takes the int32 panels; with 64-wide tiles and
     kappa > 1 the lead phase runs as block squarings plus depth-L panels (a
-    chain needs every squaring); the kappa lead rounds (FWX_LEAD_BLOCK=0),
-    four waves per SIMD (FWX_REST16P_WAVES=4) and FWX_PACK16=0 give the same
-    bits."""
+    chain needs every squaring; the first stage's squarings split each tile's
+    depth over 4 workgroups, or 8 or none with FWX_SQ_SPLIT); the kappa lead
+    rounds (FWX_LEAD_BLOCK=0), four waves per SIMD (FWX_REST16P_WAVES=4) and
+    FWX_PACK16=0 give the same bits."""
     monkeypatch.setenv("FWX_GRAPH", "0")
     monkeypatch.setenv("FWX_LOOKAHEAD", la)
     for tag, W, want, packed, finish in pack_graphs:
@@ -236,6 +237,15 @@ def test_pk16_packed_panels(monkeypatch, pack_graphs, la, tile, kappa):
         if finish is not None:
             assert exact == finish, tag
         assert (got == want).all(), (tag, la, tile, kappa)
+    for split in ("8", "0"):
+        # the first stage's block squarings with each tile's depth in 8 parts
+        # (atomic-min merge) and in one workgroup per tile
+        monkeypatch.setenv("FWX_SQ_SPLIT", split)
+        for tag, W, want, packed, _ in (pack_graphs[0], pack_graphs[4]):
+            got = fx.floyd_warshall(W, "multi_layer_tiling_gpu", tile, kappa)
+            assert fx.last_solve_packed() == packed
+            assert (got == want).all(), (tag, "squaring split", split)
+    monkeypatch.delenv("FWX_SQ_SPLIT")
     tag, W, want, _, _ = pack_graphs[1]
     monkeypatch.setenv("FWX_LEAD_BLOCK", "0")
     got = fx.floyd_warshall(W, "multi_layer_tiling_gpu", tile, kappa)

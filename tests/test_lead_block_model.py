@@ -18,7 +18,10 @@ reading the matrix as it stands (the in-place mix), with the pk16 arithmetic
 of tests/test_pk16_model.py.  Every cell below the cap must be exact and the
 exact finish must restore the rest.  Planted mistakes -- a single squaring,
 stopping the squarings when one tile is unchanged, panels of depth T --
-fail.
+fail.  The first stage's squarings may split each tile's depth over several
+workgroups that merge by atomic min (k_block_square16s); the model
+interleaves their reads and writes at random, and plain stores in place of
+the atomic min (lost updates) fail.
 """
 import math
 
@@ -29,7 +32,46 @@ from oracle import pyoracle as po
 from test_pk16_model import CAP, exact_finish, graph, product_pk16
 
 
-def lead_block_fw(W, T, kappa, seed=0, squarings=None, stop_on_any_unchanged=False, panel_depth=None):
+def square_split(D, K, span, s0, s1, parts, rng, plain_store=False, reads_first=False):
+    """One squaring launch of k_block_square16s: each tile of the stage
+    block by `parts` workgroups over disjoint depth ranges.  Every part's
+    read (its C tile and operands as the matrix stands) and write are two
+    events of a random interleaving, the read first; the write lowers the
+    cells whose result differs from the value read and lies below the cap by
+    an atomic min (`plain_store` stores them instead: a planted lost
+    update).  `reads_first` puts every read before every write (one
+    synchronous squaring per launch).  Returns whether any part changed a
+    cell."""
+    B = len(K)
+    depth = (s1 - s0) // parts
+    items = [(t, q) for t in range(B * B) for q in range(parts)]
+    if reads_first:
+        events = np.concatenate([rng.permutation(len(items)), rng.permutation(len(items))])
+    else:
+        events = rng.permutation(np.repeat(np.arange(len(items)), 2))
+    pending = {}
+    changed = False
+    for e in events:
+        t, q = items[e]
+        a0, a1 = span(K[t // B])
+        b0, b1 = span(K[t % B])
+        k0, k1 = s0 + q * depth, s0 + (q + 1) * depth
+        if e not in pending:
+            was = D[a0:a1, b0:b1].copy()
+            res = was.copy()
+            product_pk16(res, D[a0:a1, k0:k1].copy(), D[k0:k1, b0:b1].copy())
+            pending[e] = (was, res)
+            continue
+        was, res = pending[e]
+        m = (res != was) & (res < CAP)
+        changed = changed or bool((res != was).any())
+        C = D[a0:a1, b0:b1]
+        C[m] = res[m] if plain_store else np.minimum(C[m], res[m])
+    return changed
+
+
+def lead_block_fw(W, T, kappa, seed=0, squarings=None, stop_on_any_unchanged=False, panel_depth=None,
+                  split=1, plain_store=False, reads_first=False):
     n = W.shape[0]
     D = W.astype(np.int64)
     N = (n + T - 1) // T
@@ -45,6 +87,10 @@ def lead_block_fw(W, T, kappa, seed=0, squarings=None, stop_on_any_unchanged=Fal
         for it in range(nsq):
             if not changed_prev:
                 break
+            if split > 1 and kb == 0:
+                # the first stage's launches split each tile's depth
+                changed_prev = square_split(D, K, span, s0, s1, split, rng, plain_store, reads_first)
+                continue
             changed = False
             unchanged_tiles = 0
             for t in rng.permutation(B * B):
@@ -111,6 +157,35 @@ def test_block_lead_is_exact_below_the_cap(name, W, saturates, T, kappa):
         if saturates is not None:
             assert bool((~below).any()) == saturates, name
         assert (exact_finish(got) == want).all(), f"{name}: the exact finish does not restore the distances"
+
+
+@pytest.mark.parametrize("name,W,saturates", CASES, ids=[c[0] for c in CASES])
+@pytest.mark.parametrize("T,kappa,split", [(8, 4, 4), (8, 4, 2), (16, 2, 4)])
+def test_split_squarings_are_exact_below_the_cap(name, W, saturates, T, kappa, split):
+    """The first stage's squarings with each tile's depth split over `split`
+    workgroups, merged by atomic min, in random read/write interleavings."""
+    want = po.fw_serial(W).astype(np.int64)
+    for seed, reads_first in ((0, False), (1, False), (2, True)):
+        got = lead_block_fw(W, T, kappa, seed=seed, split=split, reads_first=reads_first)
+        below = got < CAP
+        assert (got[below] == want[below]).all(), f"{name}: a cell below the cap is not exact"
+        assert (exact_finish(got) == want).all(), name
+
+
+def test_planted_split_with_plain_stores_fails():
+    """Parts that store their result instead of lowering by atomic min lose
+    other parts' lower values.  A chain with decoy shortcuts i -> i+2 of
+    weight 3 gives a cell several improving parts of different values; with
+    every read ahead of every write (no in-place mix to make up for it) the
+    fixed launch count then leaves the stage block unclosed, while the
+    atomic min closes it."""
+    W = chain(64)
+    W[np.arange(62), np.arange(2, 64)] = 3
+    want = po.fw_serial(W).astype(np.int64)
+    assert (lead_block_fw(W, 8, 8, split=8, reads_first=True) == want).all()
+    bad = [not (lead_block_fw(W, 8, 8, seed=s, split=8, plain_store=True, reads_first=True) == want).all()
+           for s in range(4)]
+    assert all(bad)
 
 
 def test_planted_single_squaring_fails():

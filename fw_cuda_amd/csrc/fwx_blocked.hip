@@ -488,28 +488,36 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(KC == 16 ? 
 // precondition).
 __global__ __launch_bounds__(256) void k_pack16(const int32_t* __restrict__ D, int32_t n, int32_t s0, int32_t S,
                                                 uint32_t* __restrict__ wa, uint64_t* __restrict__ wb,
-                                                int32_t na, int32_t prio, const int* __restrict__ gate) {
+                                                int32_t na, int32_t prio, const int* __restrict__ gate,
+                                                int* __restrict__ capped) {
   if (gate_closed(gate)) return;
   if (prio) lead_wave_priority();
+  int32_t mx = 0;
   if ((int32_t)blockIdx.x < na) {
     const int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x;
     const int64_t i = q / (S / 4);
     const int32_t k = (int32_t)(q % (S / 4)) * 4;
-    if (i >= n) return;
-    const int4 v = *reinterpret_cast<const int4*>(D + i * n + s0 + k);
-    *reinterpret_cast<uint4*>(wa + i * S + k) = make_uint4(p16_dup(v.x), p16_dup(v.y), p16_dup(v.z), p16_dup(v.w));
+    if (i < n) {
+      const int4 v = *reinterpret_cast<const int4*>(D + i * n + s0 + k);
+      *reinterpret_cast<uint4*>(wa + i * S + k) =
+          make_uint4(p16_dup(v.x), p16_dup(v.y), p16_dup(v.z), p16_dup(v.w));
+      mx = max(max(v.x, v.y), max(v.z, v.w));
+    }
   } else {
     const int64_t q = (int64_t)(blockIdx.x - na) * 256 + threadIdx.x;
     const int64_t p = q / (n / 4);
     const int32_t c4 = (int32_t)(q % (n / 4)) * 4;
-    if (p >= S / 2) return;
-    const int4 r0 = *reinterpret_cast<const int4*>(D + (s0 + 2 * p) * (int64_t)n + c4);
-    const int4 r1 = *reinterpret_cast<const int4*>(D + (s0 + 2 * p + 1) * (int64_t)n + c4);
-    ulonglong2 v;
-    v.x = (uint64_t)p16_pair(r0.x, r0.y) | ((uint64_t)p16_pair(r1.x, r1.y) << 32);
-    v.y = (uint64_t)p16_pair(r0.z, r0.w) | ((uint64_t)p16_pair(r1.z, r1.w) << 32);
-    *reinterpret_cast<ulonglong2*>(wb + p * (n / 2) + c4 / 2) = v;
+    if (p < S / 2) {
+      const int4 r0 = *reinterpret_cast<const int4*>(D + (s0 + 2 * p) * (int64_t)n + c4);
+      const int4 r1 = *reinterpret_cast<const int4*>(D + (s0 + 2 * p + 1) * (int64_t)n + c4);
+      ulonglong2 v;
+      v.x = (uint64_t)p16_pair(r0.x, r0.y) | ((uint64_t)p16_pair(r1.x, r1.y) << 32);
+      v.y = (uint64_t)p16_pair(r0.z, r0.w) | ((uint64_t)p16_pair(r1.z, r1.w) << 32);
+      *reinterpret_cast<ulonglong2*>(wb + p * (n / 2) + c4 / 2) = v;
+      mx = max(max(max(r0.x, r0.y), max(r0.z, r0.w)), max(max(r1.x, r1.y), max(r1.z, r1.w)));
+    }
   }
+  if (capped && __ballot(mx >= (int32_t)kP16Cap) != 0ull && (threadIdx.x & 63) == 0) atomicOr(capped, 1);
 }
 
 // The pk16 rest pass on packed panels: the tiles and XCD mapping of
@@ -518,13 +526,13 @@ __global__ __launch_bounds__(256) void k_pack16(const int32_t* __restrict__ D, i
 template <int R, int WAVES>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES, WAVES))) void k_rest16p(
     int32_t* __restrict__ D, int32_t n, const uint32_t* __restrict__ wa, const uint64_t* __restrict__ wb, int32_t S,
-    int32_t sr0, int32_t srn, int32_t m, int32_t per, const int* __restrict__ gate) {
+    int32_t sr0, int32_t srn, int32_t m, int32_t per, const int* __restrict__ gate, int* __restrict__ capped) {
   __shared__ TileSmem16<R, R, 16> s;
   int I, J;
   if (gate_closed(gate) || !rest_tile_of_block(m, per, sr0, srn, &I, &J)) return;
   const PackedPanels w{wa + (int64_t)I * R * S, S, wb + (int64_t)J * (R / 2), n / 2};
   const View C = make_view(D, n, I * R, J * R);
-  tile_update16_packed<R, R, 16>(s, w, C, S);
+  tile_update16_packed<R, R, 16>(s, w, C, S, capped);
 }
 
 // ---------------------------------------------------------------------------
@@ -663,6 +671,10 @@ static thread_local PackWs* t_pack = nullptr;
 // The range gate of the solve being enqueued (its workspace's gate word), or
 // null when the range check has already been read.
 static thread_local const int* t_gate = nullptr;
+// The word that the last stage's pack and rest pass OR "a final cell at the
+// pk16 cap" into (the gate's second word), or null: then a separate pass
+// over the matrix checks it after the solve.
+static thread_local int* t_capped = nullptr;
 
 // A launch site without the range gate, reached while one is set: the
 // eligibility test of the deferred check (range_deferrable) admitted a
@@ -738,7 +750,10 @@ static int pack_stage(int32_t* d, int32_t n, int32_t kb, int32_t S, int prio, hi
   const int32_t s0 = kb * w->tile;
   const int64_t ga = (int64_t)n * (S / 4), gb = (int64_t)(S / 2) * (n / 4);
   const int na = (int)((ga + 255) / 256), nb = (int)((gb + 255) / 256);
-  k_pack16<<<na + nb, 256, 0, st>>>(d, n, s0, S, w->wa[b], w->wb[b], na, prio, t_gate);
+  // the last stage's panels are final when packed: every cell with a row or
+  // a column in the stage
+  int* capped = t_capped && s0 + S >= n ? t_capped : nullptr;
+  k_pack16<<<na + nb, 256, 0, st>>>(d, n, s0, S, w->wa[b], w->wb[b], na, prio, t_gate, capped);
   FWX_LAUNCHED("k_pack16");
   return FWX_OK;
 }
@@ -933,10 +948,14 @@ static int rest_pass16(int32_t* d, int32_t n, int32_t s0, int32_t S, int32_t exc
   if (t_pack && R == 128) {
     // this stage's packed panels (pack_stage ran after its lead rounds)
     const int b = (s0 / (t_pack->tile * t_pack->kappa)) % 3;
+    // the last stage's rest pass leaves every cell outside the stage final
+    int* capped = t_capped && s0 + S >= n ? t_capped : nullptr;
     if (rest16p_waves() == 5)
-      k_rest16p<128, 5><<<8 * per, 256, 0, st>>>(d, n, t_pack->wa[b], t_pack->wb[b], S, sr0, srn, m, per, t_gate);
+      k_rest16p<128, 5><<<8 * per, 256, 0, st>>>(d, n, t_pack->wa[b], t_pack->wb[b], S, sr0, srn, m, per, t_gate,
+                                                 capped);
     else
-      k_rest16p<128, 4><<<8 * per, 256, 0, st>>>(d, n, t_pack->wa[b], t_pack->wb[b], S, sr0, srn, m, per, t_gate);
+      k_rest16p<128, 4><<<8 * per, 256, 0, st>>>(d, n, t_pack->wa[b], t_pack->wb[b], S, sr0, srn, m, per, t_gate,
+                                                 capped);
     FWX_LAUNCHED("k_rest16p");
     return FWX_OK;
   }
@@ -1583,25 +1602,39 @@ static bool range_deferrable(int32_t n, int32_t tile, int32_t kappa) {
          band_narrow() && band16_enabled();
 }
 
+// FWX_CAP_FUSED=0 checks a deferred solve's result for cells at the pk16 cap
+// by a pass over the matrix (k_at_least) instead of in its last stage's pack
+// and rest pass.
+static bool cap_fused() {
+  const char* e = std::getenv("FWX_CAP_FUSED");
+  return !(e && e[0] == '0');
+}
+
 // A packed pk16 solve with its range check deferred.  The check ORs into
 // ws->gate[0] on st ahead of the solve's kernels, which return at entry when
 // it is set; after the solve one read-back brings it with the cap check
-// (ws->gate[1]).  Outside the range, d still holds the input and is solved
-// in the min3 form; with a cell at the cap, the exact pk64 finish follows,
-// as in the waiting path.  The workspace is held until the read-back.
+// (ws->gate[1]: the last stage's k_pack16 reads every cell with a row or a
+// column in that stage after its last write, and the last rest pass holds
+// every other cell's final value).  Outside the range, d still holds the
+// input and is solved in the min3 form; with a cell at the cap, the exact
+// pk64 finish follows, as in the waiting path.  The workspace is held until
+// the read-back.
 static int solve_deferred(int32_t* d, int32_t n, int32_t tile, int32_t kappa, bool vec, PackWs* ws,
                           hipStream_t st) {
   const int64_t count = (int64_t)n * n;
+  const bool fused = cap_fused();
   int rc = range_check_enqueue(d, count, ws->gate, st);
   if (!rc) {
     t_pack = ws;
     t_gate = ws->gate;
+    t_capped = fused ? ws->gate + 1 : nullptr;
     rc = run_form(d, n, tile, kappa, vec, 2, false, true, st);
     t_pack = nullptr;
     t_gate = nullptr;
+    t_capped = nullptr;
   }
   int h[2] = {1, 1};
-  if (!rc) rc = deferred_checks_read(d, count, (int32_t)kP16Cap, ws->gate, st, h);
+  if (!rc) rc = deferred_checks_read(d, count, (int32_t)kP16Cap, !fused, ws->gate, st, h);
   if (rc) {
     (void)hipStreamSynchronize(st);
     pack_release(ws);

@@ -86,11 +86,13 @@ int values_below(const int32_t* d, int64_t count, int32_t cap, hipStream_t st, b
 // The two checks of a solve that does not wait for its range check
 // (fwx_range.hip).  range_check_enqueue zeroes flags[0] and flags[1] on st
 // and ORs "a cell outside [0, 2^30)" into flags[0], without synchronising.
-// deferred_checks_read, after the solve, ORs "a cell >= cap" into flags[1],
-// copies both words into out and synchronises st.  `flags` is two device
-// ints that the caller holds for the whole solve.
+// deferred_checks_read, after the solve, ORs "a cell >= cap" into flags[1]
+// by a pass over the matrix if `scan` (else the solve's kernels have done
+// it), copies both words into out and synchronises st.  `flags` is two
+// device ints that the caller holds for the whole solve.
 int range_check_enqueue(const int32_t* d, int64_t count, int* flags, hipStream_t st);
-int deferred_checks_read(const int32_t* d, int64_t count, int32_t cap, int* flags, hipStream_t st, int out[2]);
+int deferred_checks_read(const int32_t* d, int64_t count, int32_t cap, bool scan, int* flags, hipStream_t st,
+                         int out[2]);
 // The per-device second stream of the look-ahead schedules (fwx_blocked.hip).
 int side_stream(hipStream_t* out);
 // Any D[i][i] < 0 on a device matrix (fwx_range.hip); synchronises `st`.

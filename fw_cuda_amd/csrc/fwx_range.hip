@@ -200,12 +200,15 @@ int range_check_enqueue(const int32_t* d, int64_t count, int* flags, hipStream_t
   return FWX_OK;
 }
 
-int deferred_checks_read(const int32_t* d, int64_t count, int32_t cap, int* flags, hipStream_t st, int out[2]) {
-  const int64_t work = (count + 3) / 4;
-  const int blocks = (int)std::min<int64_t>((work + 255) / 256, 4096);
-  const int vec4 = (reinterpret_cast<uintptr_t>(d) & 15u) == 0;
-  k_at_least<<<blocks > 0 ? blocks : 1, 256, 0, st>>>(d, count, vec4, cap, flags + 1);
-  FWX_LAUNCHED("k_at_least");
+int deferred_checks_read(const int32_t* d, int64_t count, int32_t cap, bool scan, int* flags, hipStream_t st,
+                         int out[2]) {
+  if (scan) {
+    const int64_t work = (count + 3) / 4;
+    const int blocks = (int)std::min<int64_t>((work + 255) / 256, 4096);
+    const int vec4 = (reinterpret_cast<uintptr_t>(d) & 15u) == 0;
+    k_at_least<<<blocks > 0 ? blocks : 1, 256, 0, st>>>(d, count, vec4, cap, flags + 1);
+    FWX_LAUNCHED("k_at_least");
+  }
   hipError_t e = hipMemcpyAsync(out, flags, 2 * sizeof(int), hipMemcpyDeviceToHost, st);
   if (e == hipSuccess) e = hipStreamSynchronize(st);
   return e == hipSuccess ? FWX_OK : hip_fail(e, "flag read-back", __FILE__, __LINE__);

@@ -415,12 +415,22 @@ struct PackedProduct16 {
 // wholly inside the matrix.
 template <int BM, int BN, int KC>
 __device__ __forceinline__ void tile_update16_packed(TileSmem16<BM, BN, KC>& s, const PackedPanels& w,
-                                                     const View& C, int32_t depth) {
+                                                     const View& C, int32_t depth, int* __restrict__ capped) {
   using PP = PackedProduct16<BM, BN, KC>;
   uint32_t acc[PP::P::TM][PP::P::TP];
   PP::P::load_c16(C, acc);
   PP::run(s, w, depth, acc);
   PP::P::store_c16(C, acc);
+  if (capped) {
+    // a result lane at CAP is a cell left at its old value >= CAP
+    constexpr uint32_t kTop = kP16Cap + kP16Bias;
+    bool any = false;
+#pragma unroll
+    for (int r = 0; r < PP::P::TM; ++r)
+#pragma unroll
+      for (int p = 0; p < PP::P::TP; ++p) any |= (acc[r][p] & 0xFFFFu) == kTop || (acc[r][p] >> 16) == kTop;
+    if (__ballot(any) != 0ull && (threadIdx.x & 63) == 0) atomicOr(capped, 1);
+  }
 }
 
 // ---------------------------------------------------------------------------

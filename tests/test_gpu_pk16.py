@@ -172,6 +172,15 @@ def negative_matching(n, seed):
     return W
 
 
+def unreachable_vertex(n, v, seed):
+    """A dense graph in which no edge enters vertex v: column v stays INF
+    (at the pk16 cap) off the diagonal, every other cell is reachable."""
+    W = graph(n, 0.5, 1, 1000, seed)
+    W[:, v] = INF
+    W[v, v] = 0
+    return W
+
+
 @pytest.mark.parametrize("la", ["0", "1", "2", "3"])
 @pytest.mark.parametrize("kappa", [2, 4])
 def test_deferred_range_check(monkeypatch, la, kappa):
@@ -179,9 +188,13 @@ def test_deferred_range_check(monkeypatch, la, kappa):
     and every kernel without waiting for it; the kernels return at entry
     when the check fails, and the host, reading the check after the solve,
     solves the untouched input in the min3 form.  In range: pk16 on packed
-    panels.  Out of range (negative weights, operands >= 2^30): min3, not
-    packed.  FWX_RANGE_DEFER=0 waits for the check first.  Each bit-exact
-    against the oracle, in every schedule."""
+    panels, and the cells left at the cap are found by the last stage's pack
+    (its rows and columns: vertex n-1 unreachable) and its rest pass (every
+    other cell: vertex 0 unreachable), or by the pass over the matrix
+    (FWX_CAP_FUSED=0); either way the exact finish follows.  Out of range
+    (negative weights, operands >= 2^30): min3, not packed.
+    FWX_RANGE_DEFER=0 waits for the check first.  Each bit-exact against the
+    oracle, in every schedule."""
     monkeypatch.setenv("FWX_GRAPH", "0")
     monkeypatch.setenv("FWX_LOOKAHEAD", la)
     n = 1024
@@ -190,18 +203,22 @@ def test_deferred_range_check(monkeypatch, la, kappa):
     big[rng.integers(0, n, 40), rng.integers(0, n, 40)] = 1 << 30
     np.fill_diagonal(big, 0)
     cases = [
-        ("in range", graph(n, 0.5, 1, 1000, 43), _lib.FORM_PK16, True),
-        ("negative weights", negative_matching(n, 46), _lib.FORM_MIN3, False),
-        ("operands >= 2^30", big, _lib.FORM_MIN3, False),
+        # (tag, W, form, packed, exact finish)
+        ("in range", graph(n, 0.5, 1, 1000, 43), _lib.FORM_PK16, True, 0),
+        ("last vertex unreachable", unreachable_vertex(n, n - 1, 47), _lib.FORM_PK16, True, 1),
+        ("vertex 0 unreachable", unreachable_vertex(n, 0, 48), _lib.FORM_PK16, True, 1),
+        ("negative weights", negative_matching(n, 46), _lib.FORM_MIN3, False, 0),
+        ("operands >= 2^30", big, _lib.FORM_MIN3, False, 0),
     ]
-    for defer in ("1", "0"):
+    for defer, fused in (("1", "1"), ("1", "0"), ("0", "1")):
         monkeypatch.setenv("FWX_RANGE_DEFER", defer)
-        for tag, W, form, packed in cases:
-            say("deferred range check", la, kappa, defer, tag)
+        monkeypatch.setenv("FWX_CAP_FUSED", fused)
+        for tag, W, form, packed, finish in cases:
+            say("deferred range check", la, kappa, defer, fused, tag)
             got = fx.floyd_warshall(W, "multi_layer_tiling_gpu", 64, kappa)
-            assert fx.last_solve_form()[0] == form, (tag, defer)
-            assert fx.last_solve_packed() == packed, (tag, defer)
-            assert (got == po.fw_rowpar(W)).all(), (tag, la, kappa, defer)
+            assert fx.last_solve_form() == (form, finish), (tag, defer, fused)
+            assert fx.last_solve_packed() == packed, (tag, defer, fused)
+            assert (got == po.fw_rowpar(W)).all(), (tag, la, kappa, defer, fused)
 
 
 @pytest.fixture(scope="module")
